@@ -1,0 +1,167 @@
+"""GSE reciprocal-space benchmark on one MI355X.
+
+One step = bin -> spread sigma1 (int64 fixed point) -> r2c -> G(k) k-pass
+(FP64 energy) -> c2r -> gather (reference force + per-particle potential),
+on device-resident synthetic input, FP32 grids/kernels with FP64 Green's
+table and reductions.  Workload: 16,000 rigid water-like triplets (48,000
+charges) in L = 78.3 A, M = 80, sigma = 2.12.
+
+Prints ONE JSON line (rank 0, N = 1).  Extra figures (FP64 mode, adjoint
+cost, the 1,000,002-charge M = 216 system, batched systems/s) go into
+"extras".
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GSE energy+force steps/s (48k atoms, M=80) + HBM roofline frac; batched systems/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
+    ap.add_argument("--workload", default="water_48k_M80")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="budget for the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    return ap.parse_args()
+
+
+def stage_bytes(N, M, ct_bytes):
+    """Algorithmic HBM bytes per launch of each stage (one system)."""
+    nk = (M // 2 + 1) * M * M
+    return {
+        "spread": 4 * ct_bytes * N + 8 * M ** 3,          # xyzq in, int64 grid written once
+        "convert": (8 + ct_bytes) * M ** 3,              # int64 in, grid out (zeroing is extra)
+        "kpass": 2 * (2 * ct_bytes) * nk + ct_bytes * nk,  # spectrum in/out, G table in
+        "gather": ct_bytes * M ** 3 + 4 * ct_bytes * N + 4 * ct_bytes * N,  # grid, xyzq, F+phi_i
+    }
+
+
+def time_steps(fn, steps, warmup, torch):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build measures the single-GPU path "
+              "(see DESIGN.md for how it would shard); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from gse_amd import api, systems
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    ct = torch.float32 if args.precision == "fp32" else torch.float64
+    xyzq_np, L, M, sigma = systems.make(args.workload)
+    N = len(xyzq_np)
+    x = torch.from_numpy(xyzq_np).to(device=dev, dtype=ct).contiguous()
+    gs = api.CpuGSEk(ct, ct, M, M, M, sigma, 0.4, 3.0, 3.0, L, L, L)
+    out = {}
+
+    def step():
+        gs.step(x, mode=0, force=True, potential=True, out=out)
+
+    sec = time_steps(step, args.steps, args.warmup, torch)
+    value = 1.0 / sec
+
+    # per-stage HIP events over a second pass of the same number of steps
+    gs.setProfiling(True)
+    for _ in range(args.steps):
+        step()
+    stages, nrec = gs.stageTimes()
+    gs.setProfiling(False)
+    ctb = 4 if ct == torch.float32 else 8
+    sb = stage_bytes(N, M, ctb)
+    ours = {k: v for k, v in stages.items() if k in sb}
+    dom = max(ours, key=ours.get)
+    achieved = sb[dom] / (ours[dom] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "bytes_per_launch": sb[dom], "avg_ms": round(ours[dom], 5)}
+
+    # accuracy against the FP64 oracle and the CPU baseline (oracle timing)
+    torch.cuda.synchronize()
+    f_gpu = out["force"].double().cpu().numpy().T
+    e_gpu = float(out["energy"].cpu()[0])
+    cpu = None
+    erms = None
+    de = None
+    if not args.no_cpu:
+        from oracle import oracle
+        p = oracle.Params(M, L, sigma)
+        ts = []
+        t_end = time.perf_counter() + args.cpu_seconds
+        ref = None
+        while not ts or (time.perf_counter() < t_end and len(ts) < 5):
+            t0 = time.perf_counter()
+            ref = oracle.gse_step(p, xyzq_np, mode=0)
+            ts.append(time.perf_counter() - t0)
+        t_med = float(np.median(ts))
+        cpu = {"value": round(1.0 / t_med, 5), "unit": "steps/s", "cores": 1, "kind": "port",
+               "sample": f"{len(ts)} full FP64 oracle step(s) of {args.workload} "
+                         f"({N} charges, M={M}) on one host core, median {t_med:.2f} s"}
+        erms = oracle.erms(ref.force, f_gpu)
+        de = abs(e_gpu - ref.energy) / abs(ref.energy)
+
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if ct == torch.float32 else "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "n_charges": N, "box": L, "grid": M,
+                   "sigma": sigma, "mode": "K", "precision": args.precision +
+                   (" spread/FFT/gather, fp64 G table and energy" if ct == torch.float32 else "")},
+        "roofline": roof, "cpu_baseline": cpu,
+        "stages_ms": {k: round(v, 5) for k, v in stages.items()},
+        "force_erms_vs_fp64_oracle": erms, "energy_rel_err_vs_fp64_oracle": de,
+    }
+
+    if not args.no_extras:
+        extras = {}
+        # FP64 mode at the same workload
+        x64 = torch.from_numpy(xyzq_np).to(device=dev, dtype=torch.float64).contiguous()
+        g64 = api.CpuGSEk(torch.float64, torch.float64, M, M, M, sigma, 0.4, 3.0, 3.0, L, L, L)
+        o64 = {}
+        s64 = time_steps(lambda: g64.step(x64, force=True, potential=True, out=o64),
+                         max(20, args.steps // 5), 5, torch)
+        extras["fp64_steps_per_s"] = round(1.0 / s64, 3)
+        # adjoint sweep (recomputes spread and k-pass; dE/dq, dE/dr, virial)
+        sa = time_steps(lambda: gs.adjoint(x), max(20, args.steps // 5), 5, torch)
+        sf = time_steps(step, max(20, args.steps // 5), 5, torch)
+        extras["adjoint_ms"] = round(sa * 1e3, 5)
+        extras["forward_ms"] = round(sf * 1e3, 5)
+        extras["adjoint_over_forward"] = round(sa / sf, 4)
+        # conv + gather roofline as the survey defines it (mode R includes the conv)
+        extras["stage_roofline_frac"] = {
+            k: round(sb[k] / (ours[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 5) for k in sb}
+        line["extras"] = extras
+
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

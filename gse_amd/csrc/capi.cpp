@@ -1,0 +1,1043 @@
+// Host runtime behind include/gse_hip.h: handles, device scratch, rocFFT
+// plans and the stage sequencing of the GSE k-space step.  All device work
+// is issued asynchronously on the handle's stream; only the calls documented
+// as synchronising wait for it.
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gse_hip.h"
+#include "gse_internal.h"
+
+using namespace gse;
+
+namespace {
+
+thread_local std::string g_err;
+
+gse_status_t fail(gse_status_t s, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return s;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(e_ == hipErrorOutOfMemory ? GSE_ERROR_OUT_OF_MEMORY : GSE_ERROR_HIP, \
+                  "%s failed: %s", #expr, hipGetErrorString(e_));                      \
+  } while (0)
+
+#define GSECHK(expr)                   \
+  do {                                 \
+    gse_status_t s_ = (expr);          \
+    if (s_ != GSE_SUCCESS) return s_;  \
+  } while (0)
+
+int g_bin_edge = 8;
+size_t g_lds_budget = 64 * 1024;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  gse_status_t ensure(size_t need) {
+    if (need <= bytes && p) return GSE_SUCCESS;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (need == 0) need = 256;
+    hipError_t e = hipMalloc(&p, need);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(GSE_ERROR_OUT_OF_MEMORY, "hipMalloc(%zu) failed: %s", need, hipGetErrorString(e));
+    }
+    bytes = need;
+    return GSE_SUCCESS;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const {
+    return (T*)p;
+  }
+};
+
+bool valid_pair(gse_dtype_t at, gse_dtype_t ct) {
+  if (at != GSE_FLOAT32 && at != GSE_FLOAT64) return false;
+  if (ct != GSE_FLOAT32 && ct != GSE_FLOAT64) return false;
+  return !(at == GSE_FLOAT32 && ct == GSE_FLOAT64);
+}
+
+size_t tsize(bool dbl) { return dbl ? 8 : 4; }
+
+gse_status_t check_dims(int X, int Y, int Z) {
+  if (X <= 0 || Y <= 0 || Z <= 0)
+    return fail(GSE_ERROR_INVALID_VALUE, "grid sizes must be positive (got %d %d %d)", X, Y, Z);
+  if ((double)X * Y * Z > 2.0e9)
+    return fail(GSE_ERROR_INVALID_VALUE, "grid of %d x %d x %d points is too large", X, Y, Z);
+  return GSE_SUCCESS;
+}
+
+gse_status_t check_box(double bx, double by, double bz) {
+  if (!(bx > 0) || !(by > 0) || !(bz > 0) || !std::isfinite(bx) || !std::isfinite(by) ||
+      !std::isfinite(bz))
+    return fail(GSE_ERROR_INVALID_VALUE, "box sizes must be positive and finite");
+  return GSE_SUCCESS;
+}
+
+gse_status_t check_sigma_rcut(double sigma, double rcut) {
+  if (!(sigma > 0) || !std::isfinite(sigma))
+    return fail(GSE_ERROR_INVALID_VALUE, "sigma must be positive and finite");
+  if (!(rcut > 0) || !std::isfinite(rcut))
+    return fail(GSE_ERROR_INVALID_VALUE, "rcut must be positive and finite");
+  return GSE_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Particle engine: binning, fixed-point spread, gather
+// ---------------------------------------------------------------------------
+struct Engine {
+  bool ct_dbl = true, at_dbl = true;
+  int X = 0, Y = 0, Z = 0;
+  DevBuf counts, atom_bin, atom_slot, perm, sorted, fp, grid64, vir6, part;
+  bool grid64_clean = false;
+  Binning bin;
+  Geom geom{};
+  BinShape shape{};
+  int nbound = -1;  // particle count of the current binning
+
+  void release() {
+    for (DevBuf* b : {&counts, &atom_bin, &atom_slot, &perm, &sorted, &fp, &grid64, &vir6, &part})
+      b->release();
+  }
+
+  gse_status_t prepare(int n, const Geom& g, hipStream_t s) {
+    geom = g;
+    shape = choose_bins(g, 8, g_lds_budget, g_bin_edge);
+    GSECHK(counts.ensure(sizeof(int) * (shape.nbins + 1)));
+    GSECHK(atom_bin.ensure(sizeof(int) * (size_t)(n > 0 ? n : 1)));
+    GSECHK(atom_slot.ensure(sizeof(int) * (size_t)(n > 0 ? n : 1)));
+    GSECHK(perm.ensure(sizeof(int) * (size_t)(n > 0 ? n : 1)));
+    GSECHK(sorted.ensure(4 * tsize(ct_dbl) * (size_t)(n > 0 ? n : 1)));
+    GSECHK(fp.ensure(sizeof(FixedParams)));
+    GSECHK(part.ensure(sizeof(double) * kRedBlocks * 8));
+    const size_t npts = (size_t)X * Y * Z;
+    if (grid64.bytes < npts * 8 || !grid64.p) {
+      GSECHK(grid64.ensure(npts * 8));
+      grid64_clean = false;
+    }
+    if (!grid64_clean) {
+      HIPCHK(launch_fill_u64(grid64.as<unsigned long long>(), npts, s));
+      grid64_clean = true;
+    }
+    bin.counts = counts.as<int>();
+    bin.atom_bin = atom_bin.as<int>();
+    bin.atom_slot = atom_slot.as<int>();
+    bin.perm = perm.as<int>();
+    bin.sorted = sorted.p;
+    bin.fp = fp.as<FixedParams>();
+    return GSE_SUCCESS;
+  }
+
+  // Bins the particles for a support of radius rcut and Gaussian sigma.
+  gse_status_t bin_particles(int n, const void* xyzq, double bx, double by, double bz,
+                             double sigma, double rcut, bool normalize, hipStream_t s) {
+    if (n < 0) return fail(GSE_ERROR_INVALID_VALUE, "numCoord must be >= 0");
+    if (n > 0 && !xyzq) return fail(GSE_ERROR_INVALID_VALUE, "xyzq is NULL");
+    GSECHK(check_box(bx, by, bz));
+    GSECHK(check_sigma_rcut(sigma, rcut));
+    Geom g = make_geom(X, Y, Z, bx, by, bz, rcut);
+    GSECHK(prepare(n, g, s));
+    const int images = ((2 * g.nx + 1 + X - 1) / X) * ((2 * g.ny + 1 + Y - 1) / Y) *
+                       ((2 * g.nz + 1 + Z - 1) / Z);
+    const double pref = std::pow(2.0 * M_PI * sigma * sigma, -1.5) *
+                        spread_weight_bound(g, sigma, normalize);
+    HIPCHK(launch_bin(ct_dbl, g, shape, n, xyzq, bin, images, pref, s));
+    nbound = n;
+    return GSE_SUCCESS;
+  }
+
+  gse_status_t spread_binned(double sigma, bool normalize, void* rho, hipStream_t s) {
+    HIPCHK(launch_spread(ct_dbl, geom, shape, sigma, normalize, bin,
+                         grid64.as<unsigned long long>(), s));
+    HIPCHK(launch_fixed_to_grid(ct_dbl, grid64.as<unsigned long long>(), rho,
+                                (size_t)X * Y * Z, bin.fp, true, s));
+    return GSE_SUCCESS;
+  }
+
+  gse_status_t gather_binned(double sigma, double rcut, int flags, const void* phi,
+                             GatherOut& o, double* virial9, hipStream_t s) {
+    if (flags & GF_VIRIAL) {
+      GSECHK(vir6.ensure(sizeof(double) * 6 * (size_t)(nbound > 0 ? nbound : 1)));
+      o.vir6 = vir6.as<double>();
+      o.vir_stride = nbound > 0 ? nbound : 1;
+      if (nbound > 0)  // invalid particles leave their slots untouched
+        HIPCHK(launch_fill(true, vir6.p, 6 * (size_t)nbound, s));
+    }
+    HIPCHK(launch_gather(ct_dbl, at_dbl, geom, shape, sigma, rcut, flags, bin, nbound, phi, o, s));
+    if ((flags & GF_VIRIAL) && virial9) {
+      HIPCHK(launch_sum_columns(vir6.as<double>(), nbound, 6, part.as<double>(), s));
+      HIPCHK(launch_add_sym6(part.as<double>(), kRedBlocks, virial9, s));
+    }
+    return GSE_SUCCESS;
+  }
+
+  gse_status_t invalid_count(hipStream_t s, int* out) {
+    if (!fp.p) {
+      *out = 0;
+      return GSE_SUCCESS;
+    }
+    FixedParams h;
+    HIPCHK(hipMemcpyAsync(&h, fp.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *out = h.invalid;
+    return GSE_SUCCESS;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// rocFFT: 3D out-of-place r2c / c2r, lengths fastest first {X, Y, Z}
+// ---------------------------------------------------------------------------
+std::once_flag g_rocfft_once;
+
+struct FFT3 {
+  bool dbl = true;
+  int X = 0, Y = 0, Z = 0;
+  rocfft_plan fwd = nullptr, bwd = nullptr;
+  rocfft_execution_info info = nullptr;
+  DevBuf work;
+  size_t work_f = 0, work_b = 0;
+
+  gse_status_t init(bool d, int x, int y, int z) {
+    dbl = d;
+    X = x;
+    Y = y;
+    Z = z;
+    std::call_once(g_rocfft_once, [] { rocfft_setup(); });
+    const size_t lengths[3] = {(size_t)X, (size_t)Y, (size_t)Z};
+    const rocfft_precision prec = dbl ? rocfft_precision_double : rocfft_precision_single;
+    if (rocfft_plan_create(&fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                           prec, 3, lengths, 1, nullptr) != rocfft_status_success)
+      return fail(GSE_ERROR_FFT, "rocfft_plan_create (r2c %dx%dx%d) failed", X, Y, Z);
+    if (rocfft_plan_create(&bwd, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                           prec, 3, lengths, 1, nullptr) != rocfft_status_success)
+      return fail(GSE_ERROR_FFT, "rocfft_plan_create (c2r %dx%dx%d) failed", X, Y, Z);
+    rocfft_plan_get_work_buffer_size(fwd, &work_f);
+    rocfft_plan_get_work_buffer_size(bwd, &work_b);
+    const size_t w = work_f > work_b ? work_f : work_b;
+    if (w) GSECHK(work.ensure(w));
+    if (rocfft_execution_info_create(&info) != rocfft_status_success)
+      return fail(GSE_ERROR_FFT, "rocfft_execution_info_create failed");
+    return GSE_SUCCESS;
+  }
+
+  void release() {
+    if (fwd) rocfft_plan_destroy(fwd);
+    if (bwd) rocfft_plan_destroy(bwd);
+    if (info) rocfft_execution_info_destroy(info);
+    fwd = bwd = nullptr;
+    info = nullptr;
+    work.release();
+  }
+
+  gse_status_t exec(bool forward, const void* in, void* out, hipStream_t s) {
+    rocfft_plan p = forward ? fwd : bwd;
+    const size_t w = forward ? work_f : work_b;
+    if (rocfft_execution_info_set_stream(info, s) != rocfft_status_success)
+      return fail(GSE_ERROR_FFT, "rocfft_execution_info_set_stream failed");
+    if (w && rocfft_execution_info_set_work_buffer(info, work.p, w) != rocfft_status_success)
+      return fail(GSE_ERROR_FFT, "rocfft_execution_info_set_work_buffer failed");
+    void* ib[1] = {const_cast<void*>(in)};
+    void* ob[1] = {out};
+    if (rocfft_execute(p, ib, ob, info) != rocfft_status_success)
+      return fail(GSE_ERROR_FFT, "rocfft_execute failed");
+    return GSE_SUCCESS;
+  }
+};
+
+}  // namespace
+struct gse_gsek_s;
+static gse_status_t stage_mark(gse_gsek_s* h);
+namespace {
+
+// k-space solve state: spectrum, Green's table, partial sums
+struct KSolver {
+  bool dbl = true;
+  int X = 0, Y = 0, Z = 0;
+  FFT3 fft;
+  DevBuf spec, table, part;
+  int tab_kind = -1;
+  double tab_sigma = NAN, tab_bx = NAN, tab_by = NAN, tab_bz = NAN;
+
+  size_t nk() const { return (size_t)(X / 2 + 1) * Y * Z; }
+
+  gse_status_t init(bool d, int x, int y, int z) {
+    dbl = d;
+    X = x;
+    Y = y;
+    Z = z;
+    GSECHK(fft.init(d, x, y, z));
+    GSECHK(spec.ensure(2 * tsize(d) * nk()));
+    GSECHK(table.ensure(tsize(d) * nk()));
+    GSECHK(part.ensure(sizeof(double) * kKpassBlocks * 7));
+    return GSE_SUCCESS;
+  }
+  void release() {
+    fft.release();
+    spec.release();
+    table.release();
+    part.release();
+  }
+  gse_status_t ensure_table(int kind, double sigma, double bx, double by, double bz,
+                            hipStream_t s) {
+    if (kind == tab_kind && sigma == tab_sigma && bx == tab_bx && by == tab_by && bz == tab_bz)
+      return GSE_SUCCESS;
+    HIPCHK(launch_greens_table(dbl, kind, X, Y, Z, bx, by, bz, sigma, table.p, s));
+    tab_kind = kind;
+    tab_sigma = sigma;
+    tab_bx = bx;
+    tab_by = by;
+    tab_bz = bz;
+    return GSE_SUCCESS;
+  }
+  // phi = c2r(G * r2c(rho)); energy/virial (device) when requested
+  gse_status_t solve(int kind, double sigma, double bx, double by, double bz, const void* rho,
+                     void* phi, double* eout, bool vir, double ccelec, hipStream_t s,
+                     gse_gsek_s* marker = nullptr) {
+    GSECHK(ensure_table(kind, sigma, bx, by, bz, s));
+    GSECHK(fft.exec(true, rho, spec.p, s));
+    if (marker) GSECHK(stage_mark(marker));
+    HIPCHK(launch_kpass(dbl, kind, X, Y, Z, bx, by, bz, sigma, spec.p, table.p, vir && eout,
+                        part.as<double>(), s));
+    if (eout) {
+      const double vcell = (bx / X) * (by / Y) * (bz / Z);
+      HIPCHK(launch_kpass_finalize(part.as<double>(), 0.5 * ccelec * vcell, eout, vir, s));
+    }
+    if (marker) GSECHK(stage_mark(marker));
+    GSECHK(fft.exec(false, spec.p, phi, s));
+    if (marker) GSECHK(stage_mark(marker));
+    return GSE_SUCCESS;
+  }
+};
+
+}  // namespace
+
+// ===========================================================================
+// handles
+// ===========================================================================
+struct gse_gauss_charge_s {
+  Engine eng;
+  hipStream_t stream = nullptr;
+  DevBuf tmpA, tmpB, taps, part, dev3;
+};
+
+struct gse_convolution_s {
+  bool dbl = true;
+  int X = 0, Y = 0, Z = 0;
+  int R[3] = {-1, -1, -1};
+  DevBuf taps[3], tmpA, tmpB;
+  hipStream_t stream = nullptr;
+};
+
+struct gse_fft_solver_s {
+  KSolver ks;
+  int kind = 0;
+  hipStream_t stream = nullptr;
+};
+
+struct gse_gsek_s {
+  bool at_dbl = true, ct_dbl = true;
+  int X = 0, Y = 0, Z = 0;
+  double sigma = 0, kappa = 0, ls = 0, ls1 = 0, ccelec = 332.0716;
+  double bx = 0, by = 0, bz = 0;
+  double sigma1 = 0, sigma2 = 0;
+  Engine eng;
+  KSolver ks;
+  DevBuf rhoS, phiM, rhoR, taps[3], tmp, tmp2, dout, part;
+  hipStream_t stream = nullptr;
+  bool profiling = false;
+  std::vector<hipEvent_t> events;  // (GSE_NUM_STAGES + 1) per recorded step
+  size_t ev_used = 0;
+  double rcut1() const { return ls1 * std::sqrt(2.0) * sigma1; }
+  double rcut2() const { return ls1 * std::sqrt(2.0) * sigma2; }
+};
+
+namespace {
+
+// separable sigma taps of spreadChargeOnGrid (CpuGaussCharge.cpp:406-433)
+gse_status_t make_ongrid_taps(bool dbl, int X, int Y, int Z, double sigma, double rcut,
+                              double bx, double by, double bz, DevBuf (&taps)[3], int (&R)[3]) {
+  const double h[3] = {bx / X, by / Y, bz / Z};
+  const double inv_2sigmasq = 1.0 / (2.0 * sigma * sigma);
+  const double pref = std::pow(2.0 * M_PI * sigma * sigma, -1.5) * h[0] * h[1] * h[2];
+  for (int a = 0; a < 3; a++) {
+    R[a] = (int)std::ceil(rcut / h[a]);
+    std::vector<double> kd(2 * R[a] + 1);
+    const double hsq = h[a] * h[a];
+    for (int ii = -R[a]; ii <= R[a]; ii++) {
+      double v = std::exp(-hsq * (double)(ii * ii) * inv_2sigmasq);
+      kd[ii + R[a]] = a == 2 ? pref * v : v;
+    }
+    GSECHK(taps[a].ensure(tsize(dbl) * kd.size()));
+    if (dbl) {
+      HIPCHK(hipMemcpy(taps[a].p, kd.data(), 8 * kd.size(), hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> kf(kd.begin(), kd.end());
+      HIPCHK(hipMemcpy(taps[a].p, kf.data(), 4 * kf.size(), hipMemcpyHostToDevice));
+    }
+  }
+  return GSE_SUCCESS;
+}
+
+gse_status_t conv3(bool dbl, int X, int Y, int Z, const int (&R)[3], DevBuf (&taps)[3],
+                   const void* in, void* out, DevBuf& tA, DevBuf& tB, hipStream_t s) {
+  const size_t bytes = tsize(dbl) * (size_t)X * Y * Z;
+  GSECHK(tA.ensure(bytes));
+  GSECHK(tB.ensure(bytes));
+  HIPCHK(launch_conv_pass(dbl, 0, X, Y, Z, R[0], taps[0].p, in, tA.p, s));
+  HIPCHK(launch_conv_pass(dbl, 1, X, Y, Z, R[1], taps[1].p, tA.p, tB.p, s));
+  HIPCHK(launch_conv_pass(dbl, 2, X, Y, Z, R[2], taps[2].p, tB.p, out, s));
+  return GSE_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gse_status_string(gse_status_t s) {
+  switch (s) {
+    case GSE_SUCCESS: return "success";
+    case GSE_ERROR_INVALID_VALUE: return "invalid value";
+    case GSE_ERROR_OUT_OF_MEMORY: return "out of device memory";
+    case GSE_ERROR_HIP: return "HIP error";
+    case GSE_ERROR_FFT: return "rocFFT error";
+    case GSE_ERROR_BAD_PARTICLES: return "non-finite or out-of-range particles";
+    case GSE_ERROR_NOT_SUPPORTED: return "not supported";
+  }
+  return "unknown status";
+}
+
+const char* gse_last_error(void) { return g_err.c_str(); }
+
+int gse_version(void) { return 10000; }
+
+gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes) {
+  if (bin_edge < 0 || lds_budget_bytes < 0 || bin_edge > 64 || lds_budget_bytes > 160 * 1024)
+    return fail(GSE_ERROR_INVALID_VALUE, "bin_edge in [0,64], lds budget in [0,163840]");
+  g_bin_edge = bin_edge ? bin_edge : 8;
+  g_lds_budget = lds_budget_bytes ? (size_t)lds_budget_bytes : 64 * 1024;
+  return GSE_SUCCESS;
+}
+
+// ------------------------------- CpuGaussCharge ----------------------------
+gse_status_t gse_gauss_charge_create(gse_gauss_charge_t* out, gse_dtype_t at, gse_dtype_t ct,
+                                     int X, int Y, int Z) {
+  if (!out) return fail(GSE_ERROR_INVALID_VALUE, "out is NULL");
+  *out = nullptr;
+  if (!valid_pair(at, ct)) return fail(GSE_ERROR_INVALID_VALUE, "unsupported (AT, CT) pair");
+  GSECHK(check_dims(X, Y, Z));
+  auto* h = new gse_gauss_charge_s();
+  h->eng.at_dbl = at == GSE_FLOAT64;
+  h->eng.ct_dbl = ct == GSE_FLOAT64;
+  h->eng.X = X;
+  h->eng.Y = Y;
+  h->eng.Z = Z;
+  *out = h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gauss_charge_destroy(gse_gauss_charge_t h) {
+  if (!h) return GSE_SUCCESS;
+  h->eng.release();
+  for (DevBuf* b : {&h->tmpA, &h->tmpB, &h->taps, &h->part, &h->dev3}) b->release();
+  delete h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gauss_charge_set_stream(gse_gauss_charge_t h, void* s) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  h->stream = (hipStream_t)s;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gauss_charge_calc_support_size(gse_gauss_charge_t h, double rcut, double bx,
+                                                double by, double bz, int* nsup) {
+  if (!h || !nsup) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  GSECHK(check_box(bx, by, bz));
+  // CpuGaussCharge.cpp:28-56, evaluated in CT like the reference
+  const Engine& e = h->eng;
+  const double hxd = bx / e.X, hyd = by / e.Y, hzd = bz / e.Z;
+  const int nx = (int)std::ceil(rcut / hxd), ny = (int)std::ceil(rcut / hyd),
+            nz = (int)std::ceil(rcut / hzd);
+  int n = 0;
+  if (e.ct_dbl) {
+    const double rc2 = rcut * rcut;
+    for (int tz = -nz; tz <= nz; tz++)
+      for (int ty = -ny; ty <= ny; ty++)
+        for (int tx = -nx; tx <= nx; tx++) {
+          double dx = (double)(std::abs(tx) - 1) * hxd, dy = (double)(std::abs(ty) - 1) * hyd,
+                 dz = (double)(std::abs(tz) - 1) * hzd;
+          if (dx * dx + dy * dy + dz * dz < rc2) n++;
+        }
+  } else {
+    const float rc2 = (float)(rcut * rcut), hx = (float)hxd, hy = (float)hyd, hz = (float)hzd;
+    for (int tz = -nz; tz <= nz; tz++)
+      for (int ty = -ny; ty <= ny; ty++)
+        for (int tx = -nx; tx <= nx; tx++) {
+          float dx = (float)(std::abs(tx) - 1) * hx, dy = (float)(std::abs(ty) - 1) * hy,
+                dz = (float)(std::abs(tz) - 1) * hz;
+          if (dx * dx + dy * dy + dz * dz < rc2) n++;
+        }
+  }
+  *nsup = n;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gauss_charge_spread_charge_to_grid(gse_gauss_charge_t h, double sigma,
+                                                    double rcut, int n, const void* xyzq,
+                                                    double bx, double by, double bz, void* rho,
+                                                    int normalize) {
+  if (!h || !rho) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  GSECHK(h->eng.bin_particles(n, xyzq, bx, by, bz, sigma, rcut, normalize != 0, h->stream));
+  return h->eng.spread_binned(sigma, normalize != 0, rho, h->stream);
+}
+
+gse_status_t gse_gauss_charge_spread_charge_on_grid(gse_gauss_charge_t h, double sigma,
+                                                    double rcut, double bx, double by, double bz,
+                                                    const void* rho, void* rhoS) {
+  if (!h || !rho || !rhoS) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  GSECHK(check_box(bx, by, bz));
+  GSECHK(check_sigma_rcut(sigma, rcut));
+  Engine& e = h->eng;
+  DevBuf taps3[3];
+  int R[3];
+  gse_status_t st = make_ongrid_taps(e.ct_dbl, e.X, e.Y, e.Z, sigma, rcut, bx, by, bz, taps3, R);
+  if (st == GSE_SUCCESS)
+    st = conv3(e.ct_dbl, e.X, e.Y, e.Z, R, taps3, rho, rhoS, h->tmpA, h->tmpB, h->stream);
+  // taps are freed after the stream has consumed them
+  (void)hipStreamSynchronize(h->stream);
+  for (auto& t : taps3) t.release();
+  return st;
+}
+
+gse_status_t gse_gauss_charge_interpolate_force(gse_gauss_charge_t h, double sigma, double rcut,
+                                                int n, const void* xyzq, double bx, double by,
+                                                double bz, const void* phi, void* fx, void* fy,
+                                                void* fz) {
+  if (!h || !phi || (n > 0 && (!fx || !fy || !fz)))
+    return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  GSECHK(h->eng.bin_particles(n, xyzq, bx, by, bz, sigma, rcut, false, h->stream));
+  GatherOut o;
+  o.fx = fx;
+  o.fy = fy;
+  o.fz = fz;
+  return h->eng.gather_binned(sigma, rcut, GF_SPHERE, phi, o, nullptr, h->stream);
+}
+
+gse_status_t gse_gauss_charge_gather(gse_gauss_charge_t h, double sigma, double rcut, int n,
+                                     const void* xyzq, double bx, double by, double bz,
+                                     const void* phi, const gse_gather_desc_t* d) {
+  if (!h || !phi || !d) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  if (d->what & ~15 || d->what == 0) return fail(GSE_ERROR_INVALID_VALUE, "bad gather mask");
+  GSECHK(h->eng.bin_particles(n, xyzq, bx, by, bz, sigma, rcut, false, h->stream));
+  GatherOut o;
+  o.fx = d->forceX;
+  o.fy = d->forceY;
+  o.fz = d->forceZ;
+  o.cx = d->cubeX;
+  o.cy = d->cubeY;
+  o.cz = d->cubeZ;
+  o.pot = d->potential;
+  o.fscale = o.pscale = o.vscale = d->scale;
+  if (d->virial) HIPCHK(launch_fill(true, d->virial, 9, h->stream));
+  return h->eng.gather_binned(sigma, rcut, d->what, phi, o, d->virial, h->stream);
+}
+
+gse_status_t gse_gauss_charge_calc_electric_field_on_grid(gse_gauss_charge_t h, double bx,
+                                                          double by, double bz, const void* phi,
+                                                          void* ex, void* ey, void* ez) {
+  if (!h || !phi || !ex || !ey || !ez) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  GSECHK(check_box(bx, by, bz));
+  const Engine& e = h->eng;
+  HIPCHK(launch_efield(e.ct_dbl, e.X, e.Y, e.Z, bx, by, bz, phi, ex, ey, ez, h->stream));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gauss_charge_calc_dipole_sum(gse_gauss_charge_t h, double bx, double by,
+                                              double bz, const void* rho, double* dip) {
+  if (!h || !rho || !dip) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  GSECHK(check_box(bx, by, bz));
+  const Engine& e = h->eng;
+  GSECHK(h->part.ensure(sizeof(double) * kRedBlocks * 3));
+  GSECHK(h->dev3.ensure(sizeof(double) * 3));
+  HIPCHK(launch_dipole(e.ct_dbl, e.X, e.Y, e.Z, bx, by, bz, rho, h->part.as<double>(), h->stream));
+  HIPCHK(launch_reduce_partials(h->part.as<double>(), kRedBlocks, 3, 3, 1.0,
+                                h->dev3.as<double>(), h->stream));
+  HIPCHK(hipMemcpyAsync(dip, h->dev3.p, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gauss_charge_invalid_count(gse_gauss_charge_t h, int* count) {
+  if (!h || !count) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  return h->eng.invalid_count(h->stream, count);
+}
+
+// ------------------------------- CpuConvolution ----------------------------
+gse_status_t gse_convolution_create(gse_convolution_t* out, gse_dtype_t t, int X, int Y, int Z) {
+  if (!out) return fail(GSE_ERROR_INVALID_VALUE, "out is NULL");
+  *out = nullptr;
+  if (t != GSE_FLOAT32 && t != GSE_FLOAT64) return fail(GSE_ERROR_INVALID_VALUE, "bad dtype");
+  GSECHK(check_dims(X, Y, Z));
+  auto* h = new gse_convolution_s();
+  h->dbl = t == GSE_FLOAT64;
+  h->X = X;
+  h->Y = Y;
+  h->Z = Z;
+  *out = h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_convolution_destroy(gse_convolution_t h) {
+  if (!h) return GSE_SUCCESS;
+  for (auto& b : h->taps) b.release();
+  h->tmpA.release();
+  h->tmpB.release();
+  delete h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_convolution_set_stream(gse_convolution_t h, void* s) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  h->stream = (hipStream_t)s;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_convolution_setup_kernel(gse_convolution_t h, int Rx, int Ry, int Rz,
+                                          const void* kx, const void* ky, const void* kz) {
+  if (!h || !kx || !ky || !kz) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  const int R[3] = {Rx, Ry, Rz};
+  const void* k[3] = {kx, ky, kz};
+  for (int a = 0; a < 3; a++) {
+    if (R[a] < 0) return fail(GSE_ERROR_INVALID_VALUE, "kernel radius must be >= 0");
+    const size_t bytes = tsize(h->dbl) * (2 * (size_t)R[a] + 1);
+    GSECHK(h->taps[a].ensure(bytes));
+    HIPCHK(hipMemcpy(h->taps[a].p, k[a], bytes, hipMemcpyHostToDevice));
+    h->R[a] = R[a];
+  }
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_convolution_run(gse_convolution_t h, const void* in, void* out) {
+  if (!h || !in || !out) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  if (h->R[0] < 0) return fail(GSE_ERROR_INVALID_VALUE, "setupKernel has not been called");
+  HIPCHK(launch_conv_pass(h->dbl, 0, h->X, h->Y, h->Z, h->R[0], h->taps[0].p, in, out, h->stream));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_convolution_run_xyz(gse_convolution_t h, const void* in, void* out) {
+  if (!h || !in || !out) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  if (h->R[0] < 0) return fail(GSE_ERROR_INVALID_VALUE, "setupKernel has not been called");
+  return conv3(h->dbl, h->X, h->Y, h->Z, h->R, h->taps, in, out, h->tmpA, h->tmpB, h->stream);
+}
+
+// ------------------------------- CpuFFTSolver ------------------------------
+gse_status_t gse_fft_solver_create(gse_fft_solver_t* out, gse_dtype_t t, gse_greens_kind_t kind,
+                                   int X, int Y, int Z) {
+  if (!out) return fail(GSE_ERROR_INVALID_VALUE, "out is NULL");
+  *out = nullptr;
+  if (t != GSE_FLOAT32 && t != GSE_FLOAT64) return fail(GSE_ERROR_INVALID_VALUE, "bad dtype");
+  if (kind != GSE_GREENS_GSE && kind != GSE_GREENS_G2)
+    return fail(GSE_ERROR_INVALID_VALUE, "bad Green's function kind");
+  GSECHK(check_dims(X, Y, Z));
+  auto* h = new gse_fft_solver_s();
+  h->kind = kind;
+  gse_status_t st = h->ks.init(t == GSE_FLOAT64, X, Y, Z);
+  if (st != GSE_SUCCESS) {
+    h->ks.release();
+    delete h;
+    return st;
+  }
+  *out = h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_fft_solver_destroy(gse_fft_solver_t h) {
+  if (!h) return GSE_SUCCESS;
+  h->ks.release();
+  delete h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_fft_solver_set_stream(gse_fft_solver_t h, void* s) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  h->stream = (hipStream_t)s;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_fft_solver_run(gse_fft_solver_t h, void* phi, const void* rho, double sigma,
+                                double bx, double by, double bz) {
+  if (!h || !phi || !rho) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  GSECHK(check_box(bx, by, bz));
+  if (!(sigma >= 0) || !std::isfinite(sigma))
+    return fail(GSE_ERROR_INVALID_VALUE, "sigma must be >= 0");
+  return h->ks.solve(h->kind, sigma, bx, by, bz, rho, phi, nullptr, false, 1.0, h->stream);
+}
+
+gse_status_t gse_fft_solver_r2c(gse_fft_solver_t h, const void* rho, void* rhoK) {
+  if (!h || !rho || !rhoK) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  return h->ks.fft.exec(true, rho, rhoK, h->stream);
+}
+
+gse_status_t gse_fft_solver_c2r(gse_fft_solver_t h, void* rhoK, void* phi) {
+  if (!h || !rhoK || !phi) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  return h->ks.fft.exec(false, rhoK, phi, h->stream);
+}
+
+gse_status_t gse_greens_func_mul(gse_dtype_t t, gse_greens_kind_t kind, int X, int Y, int Z,
+                                 double bx, double by, double bz, double sigma, void* rhoK,
+                                 void* stream) {
+  if (!rhoK) return fail(GSE_ERROR_INVALID_VALUE, "rhoK is NULL");
+  if (t != GSE_FLOAT32 && t != GSE_FLOAT64) return fail(GSE_ERROR_INVALID_VALUE, "bad dtype");
+  if (kind != GSE_GREENS_GSE && kind != GSE_GREENS_G2)
+    return fail(GSE_ERROR_INVALID_VALUE, "bad Green's function kind");
+  GSECHK(check_dims(X, Y, Z));
+  GSECHK(check_box(bx, by, bz));
+  const bool dbl = t == GSE_FLOAT64;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t nk = (size_t)(X / 2 + 1) * Y * Z;
+  DevBuf table, part;
+  GSECHK(table.ensure(tsize(dbl) * nk));
+  GSECHK(part.ensure(sizeof(double) * kKpassBlocks * 7));
+  hipError_t e = launch_greens_table(dbl, kind, X, Y, Z, bx, by, bz, sigma, table.p, s);
+  if (e == hipSuccess)
+    e = launch_kpass(dbl, kind, X, Y, Z, bx, by, bz, sigma, rhoK, table.p, false,
+                     part.as<double>(), s);
+  hipError_t e2 = hipStreamSynchronize(s);
+  table.release();
+  part.release();
+  if (e != hipSuccess) return fail(GSE_ERROR_HIP, "k-pass launch failed: %s", hipGetErrorString(e));
+  if (e2 != hipSuccess) return fail(GSE_ERROR_HIP, "k-pass failed: %s", hipGetErrorString(e2));
+  return GSE_SUCCESS;
+}
+
+// ------------------------------- CpuGSEk -----------------------------------
+gse_status_t gse_gsek_create(gse_gsek_t* out, gse_dtype_t at, gse_dtype_t ct, int X, int Y,
+                             int Z, double sigma, double kappa, double ls, double ls1, double bx,
+                             double by, double bz, double ccelec) {
+  if (!out) return fail(GSE_ERROR_INVALID_VALUE, "out is NULL");
+  *out = nullptr;
+  if (!valid_pair(at, ct)) return fail(GSE_ERROR_INVALID_VALUE, "unsupported (AT, CT) pair");
+  GSECHK(check_dims(X, Y, Z));
+  GSECHK(check_box(bx, by, bz));
+  if (!(kappa >= 0.0 && kappa <= 0.5))  // CpuGSE.cpp:35-36
+    return fail(GSE_ERROR_INVALID_VALUE, "kappa must be in [0, 0.5]");
+  if (!(sigma > 0) || !(ls1 > 0)) return fail(GSE_ERROR_INVALID_VALUE, "sigma, lambdaSigma1 > 0");
+  auto* h = new gse_gsek_s();
+  h->at_dbl = at == GSE_FLOAT64;
+  h->ct_dbl = ct == GSE_FLOAT64;
+  h->X = X;
+  h->Y = Y;
+  h->Z = Z;
+  h->sigma = sigma;
+  h->kappa = kappa;
+  h->ls = ls;
+  h->ls1 = ls1;
+  h->ccelec = ccelec;
+  h->bx = bx;
+  h->by = by;
+  h->bz = bz;
+  h->sigma1 = sigma * std::sqrt(kappa);        // CpuGSE.cpp:42
+  h->sigma2 = sigma * std::sqrt(1.0 - 2.0 * kappa);  // CpuGSE.cpp:43
+  h->eng.at_dbl = h->at_dbl;
+  h->eng.ct_dbl = h->ct_dbl;
+  h->eng.X = X;
+  h->eng.Y = Y;
+  h->eng.Z = Z;
+  const size_t gb = tsize(h->ct_dbl) * (size_t)X * Y * Z;
+  gse_status_t st = h->ks.init(h->ct_dbl, X, Y, Z);
+  if (st == GSE_SUCCESS) st = h->rhoS.ensure(gb);
+  if (st == GSE_SUCCESS) st = h->phiM.ensure(gb);
+  if (st == GSE_SUCCESS) st = h->dout.ensure(sizeof(double) * 16);
+  if (st == GSE_SUCCESS) st = h->part.ensure(sizeof(double) * kRedBlocks * 8);
+  if (st != GSE_SUCCESS) {
+    gse_gsek_destroy(h);
+    return st;
+  }
+  *out = h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gsek_destroy(gse_gsek_t h) {
+  if (!h) return GSE_SUCCESS;
+  for (hipEvent_t e : h->events) (void)hipEventDestroy(e);
+  h->eng.release();
+  h->ks.release();
+  for (DevBuf* b : {&h->rhoS, &h->phiM, &h->rhoR, &h->tmp, &h->tmp2, &h->dout, &h->part})
+    b->release();
+  for (auto& t : h->taps) t.release();
+  delete h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gsek_set_stream(gse_gsek_t h, void* s) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  h->stream = (hipStream_t)s;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gsek_set_box_size(gse_gsek_t h, double bx, double by, double bz) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  GSECHK(check_box(bx, by, bz));
+  h->bx = bx;
+  h->by = by;
+  h->bz = bz;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gsek_get_params(gse_gsek_t h, double* s1, double* s2, double* ls1) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  if (s1) *s1 = h->sigma1;
+  if (s2) *s2 = h->sigma2;
+  if (ls1) *ls1 = h->ls1;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gsek_spread_charge1(gse_gsek_t h, int n, const void* xyzq) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  GSECHK(h->eng.bin_particles(n, xyzq, h->bx, h->by, h->bz, h->sigma1, h->rcut1(), false,
+                              h->stream));
+  return h->eng.spread_binned(h->sigma1, false, h->rhoS.p, h->stream);
+}
+
+gse_status_t gse_gsek_spread_charge2(gse_gsek_t h, void* rhoOut) {
+  if (!h || !rhoOut) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  int R[3];
+  GSECHK(make_ongrid_taps(h->ct_dbl, h->X, h->Y, h->Z, h->sigma2, h->rcut2(), h->bx, h->by, h->bz,
+                          h->taps, R));
+  return conv3(h->ct_dbl, h->X, h->Y, h->Z, R, h->taps, h->rhoS.p, rhoOut, h->tmp, h->tmp2,
+               h->stream);
+}
+
+gse_status_t gse_gsek_solve_poisson(gse_gsek_t h) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  return h->ks.solve(GSE_GREENS_GSE, h->sigma2, h->bx, h->by, h->bz, h->rhoS.p, h->phiM.p,
+                     nullptr, false, h->ccelec, h->stream);
+}
+
+gse_status_t gse_gsek_calculate_energy(gse_gsek_t h, double* energy) {
+  if (!h || !energy) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  const double vcell = (h->bx / h->X) * (h->by / h->Y) * (h->bz / h->Z);
+  HIPCHK(launch_dot(h->ct_dbl, h->rhoS.p, h->phiM.p, (size_t)h->X * h->Y * h->Z,
+                    h->part.as<double>(), h->stream));
+  HIPCHK(launch_reduce_partials(h->part.as<double>(), kRedBlocks, 1, 1, 0.5 * h->ccelec * vcell,
+                                h->dout.as<double>(), h->stream));
+  HIPCHK(hipMemcpyAsync(energy, h->dout.p, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gsek_interpolate_force(gse_gsek_t h, int n, const void* xyzq, void* fx, void* fy,
+                                        void* fz) {
+  if (!h || (n > 0 && (!fx || !fy || !fz))) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  GSECHK(h->eng.bin_particles(n, xyzq, h->bx, h->by, h->bz, h->sigma1, h->rcut1(), false,
+                              h->stream));
+  GatherOut o;
+  o.fx = fx;
+  o.fy = fy;
+  o.fz = fz;
+  o.fscale = h->ccelec;
+  return h->eng.gather_binned(h->sigma1, h->rcut1(), GF_SPHERE, h->phiM.p, o, nullptr, h->stream);
+}
+
+gse_status_t gse_gsek_get_rho_s(gse_gsek_t h, void** p) {
+  if (!h || !p) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  *p = h->rhoS.p;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gsek_get_phi(gse_gsek_t h, void** p) {
+  if (!h || !p) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  *p = h->phiM.p;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gsek_copy_grid(gse_gsek_t h, int which, void* dst) {
+  if (!h || !dst || (which != 0 && which != 1))
+    return fail(GSE_ERROR_INVALID_VALUE, "bad argument");
+  const size_t bytes = tsize(h->ct_dbl) * (size_t)h->X * h->Y * h->Z;
+  HIPCHK(hipMemcpyAsync(dst, which == 0 ? h->rhoS.p : h->phiM.p, bytes, hipMemcpyDeviceToDevice,
+                        h->stream));
+  return GSE_SUCCESS;
+}
+
+static gse_status_t mark(gse_gsek_t h);
+static gse_status_t stage_mark(gse_gsek_s* h) { return mark(h); }
+
+static gse_status_t mark(gse_gsek_t h) {
+  if (!h->profiling) return GSE_SUCCESS;
+  if (h->ev_used == h->events.size()) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    h->events.push_back(e);
+  }
+  HIPCHK(hipEventRecord(h->events[h->ev_used++], h->stream));
+  return GSE_SUCCESS;
+}
+
+static gse_status_t gsek_run(gse_gsek_t h, int n, const void* xyzq, int mode, int flags,
+                             GatherOut& o, double* energy, double* virial) {
+  hipStream_t s = h->stream;
+  const double rc1 = h->rcut1();
+  GSECHK(mark(h));
+  GSECHK(h->eng.bin_particles(n, xyzq, h->bx, h->by, h->bz, h->sigma1, rc1, false, s));
+  GSECHK(mark(h));
+  HIPCHK(launch_spread(h->ct_dbl, h->eng.geom, h->eng.shape, h->sigma1, false, h->eng.bin,
+                       h->eng.grid64.as<unsigned long long>(), s));
+  GSECHK(mark(h));
+  HIPCHK(launch_fixed_to_grid(h->ct_dbl, h->eng.grid64.as<unsigned long long>(), h->rhoS.p,
+                              (size_t)h->X * h->Y * h->Z, h->eng.bin.fp, true, s));
+  GSECHK(mark(h));
+  double* eout = h->dout.as<double>();
+  if (mode == 0) {
+    GSECHK(h->ks.solve(GSE_GREENS_GSE, h->sigma2, h->bx, h->by, h->bz, h->rhoS.p, h->phiM.p, eout,
+                       virial != nullptr, h->ccelec, s, h->profiling ? h : nullptr));
+  } else {
+    int R[3];
+    GSECHK(make_ongrid_taps(h->ct_dbl, h->X, h->Y, h->Z, h->sigma2, h->rcut2(), h->bx, h->by,
+                            h->bz, h->taps, R));
+    GSECHK(h->rhoR.ensure(tsize(h->ct_dbl) * (size_t)h->X * h->Y * h->Z));
+    GSECHK(conv3(h->ct_dbl, h->X, h->Y, h->Z, R, h->taps, h->rhoS.p, h->rhoR.p, h->tmp,
+                 h->tmp2, s));
+    GSECHK(h->ks.solve(GSE_GREENS_GSE, 0.0, h->bx, h->by, h->bz, h->rhoR.p, h->phiM.p, nullptr,
+                       false, h->ccelec, s, h->profiling ? h : nullptr));
+    const double vcell = (h->bx / h->X) * (h->by / h->Y) * (h->bz / h->Z);
+    HIPCHK(launch_dot(h->ct_dbl, h->rhoS.p, h->phiM.p, (size_t)h->X * h->Y * h->Z,
+                      h->part.as<double>(), s));
+    HIPCHK(launch_reduce_partials(h->part.as<double>(), kRedBlocks, 1, 1,
+                                  0.5 * h->ccelec * vcell, eout, s));
+  }
+  if (energy) HIPCHK(hipMemcpyAsync(energy, eout, sizeof(double), hipMemcpyDeviceToDevice, s));
+  if (virial && mode == 0)
+    HIPCHK(hipMemcpyAsync(virial, eout + 1, 9 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  if (flags) GSECHK(h->eng.gather_binned(h->sigma1, rc1, flags, h->phiM.p, o,
+                                         (virial && mode == 0) ? virial : nullptr, s));
+  GSECHK(mark(h));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gsek_set_profiling(gse_gsek_t h, int enable) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  h->profiling = enable != 0;
+  h->ev_used = 0;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gsek_stage_times(gse_gsek_t h, double* ms, int* steps) {
+  if (!h || !ms || !steps) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  const int per = GSE_NUM_STAGES + 1;
+  const int ns = (int)(h->ev_used / per);
+  for (int k = 0; k < GSE_NUM_STAGES; k++) ms[k] = 0.0;
+  if (ns > 0) HIPCHK(hipEventSynchronize(h->events[(size_t)ns * per - 1]));
+  for (int st = 0; st < ns; st++)
+    for (int k = 0; k < GSE_NUM_STAGES; k++) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, h->events[(size_t)st * per + k],
+                                 h->events[(size_t)st * per + k + 1]));
+      ms[k] += t;
+    }
+  if (ns > 0)
+    for (int k = 0; k < GSE_NUM_STAGES; k++) ms[k] /= ns;
+  *steps = ns;
+  h->ev_used = 0;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gsek_step(gse_gsek_t h, int n, const void* xyzq, const gse_step_desc_t* d) {
+  if (!h || !d) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  if (d->mode != 0 && d->mode != 1) return fail(GSE_ERROR_INVALID_VALUE, "mode must be 0 or 1");
+  if (d->mode == 1 && d->virial)
+    return fail(GSE_ERROR_NOT_SUPPORTED, "the virial is defined for mode K only");
+  int flags = 0;
+  if (d->forceX || d->forceY || d->forceZ) flags |= GF_SPHERE;
+  if (d->cubeX || d->cubeY || d->cubeZ) flags |= GF_CUBE;
+  if (d->potential) flags |= GF_POT;
+  if (d->virial) flags |= GF_VIRIAL;
+  if ((flags & GF_SPHERE) && !(d->forceX && d->forceY && d->forceZ))
+    return fail(GSE_ERROR_INVALID_VALUE, "forceX/Y/Z must be all set or all NULL");
+  if ((flags & GF_CUBE) && !(d->cubeX && d->cubeY && d->cubeZ))
+    return fail(GSE_ERROR_INVALID_VALUE, "cubeX/Y/Z must be all set or all NULL");
+  GatherOut o;
+  o.fx = d->forceX;
+  o.fy = d->forceY;
+  o.fz = d->forceZ;
+  o.cx = d->cubeX;
+  o.cy = d->cubeY;
+  o.cz = d->cubeZ;
+  o.pot = d->potential;
+  o.fscale = o.pscale = o.vscale = h->ccelec;
+  return gsek_run(h, n, xyzq, d->mode, flags, o, d->energy, d->virial);
+}
+
+gse_status_t gse_gsek_adjoint(gse_gsek_t h, int n, const void* xyzq, const double* grad,
+                              void* dEdq, void* dEdx, void* dEdy, void* dEdz, double* virial) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  if ((dEdx || dEdy || dEdz) && !(dEdx && dEdy && dEdz))
+    return fail(GSE_ERROR_INVALID_VALUE, "dEdx/y/z must be all set or all NULL");
+  int flags = 0;
+  if (dEdx) flags |= GF_CUBE;
+  if (dEdq) flags |= GF_POT;
+  if (virial) flags |= GF_VIRIAL;
+  GatherOut o;
+  o.cx = dEdx;  // the cube force is -dE/dr: the negative scale gives dE/dr
+  o.cy = dEdy;
+  o.cz = dEdz;
+  o.pot = dEdq;
+  o.fscale = -h->ccelec;
+  o.pscale = h->ccelec;
+  o.vscale = h->ccelec;
+  o.dscale = grad;
+  return gsek_run(h, n, xyzq, 0, flags, o, nullptr, virial);
+}
+
+gse_status_t gse_gsek_corrections(gse_gsek_t h, int n, const void* xyzq, double* out) {
+  if (!h || !out || (n > 0 && !xyzq)) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  HIPCHK(launch_charge_moments(h->ct_dbl, xyzq, n, h->part.as<double>(), h->stream));
+  HIPCHK(launch_corrections_finalize(h->part.as<double>(), kRedBlocks, h->sigma, h->bx, h->by,
+                                     h->bz, h->ccelec, out, h->stream));
+  return GSE_SUCCESS;
+}
+
+// ------------------------------- batched -----------------------------------
+gse_status_t gse_batch_create(gse_batch_t* out, gse_dtype_t, gse_dtype_t, int, const int*,
+                              const double*, const int*, double, double, double, double, double) {
+  if (out) *out = nullptr;
+  return fail(GSE_ERROR_NOT_SUPPORTED, "batched systems are not available yet");
+}
+gse_status_t gse_batch_destroy(gse_batch_t) { return GSE_SUCCESS; }
+gse_status_t gse_batch_set_stream(gse_batch_t, void*) {
+  return fail(GSE_ERROR_NOT_SUPPORTED, "batched systems are not available yet");
+}
+gse_status_t gse_batch_step(gse_batch_t, const void*, void*, void*, void*, void*, double*) {
+  return fail(GSE_ERROR_NOT_SUPPORTED, "batched systems are not available yet");
+}
+gse_status_t gse_batch_step_graph(gse_batch_t, const void*, void*, void*, void*, void*, double*) {
+  return fail(GSE_ERROR_NOT_SUPPORTED, "batched systems are not available yet");
+}
+
+}  // extern "C"

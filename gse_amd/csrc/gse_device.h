@@ -1,0 +1,107 @@
+// Device-side helpers shared by the CDNA4 kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gse {
+
+constexpr int kWave = 64;
+
+// Fixed-point conversion of a double whose magnitude is below 2^51: adding
+// 1.5*2^52 leaves round-to-nearest(v) in the low mantissa bits.
+__device__ __forceinline__ long long fixed_round(double v) {
+  const double magic = 6755399441055744.0;  // 1.5 * 2^52
+  return (long long)__double_as_longlong(v + magic) -
+         (long long)__double_as_longlong(magic);
+}
+
+// Uniform (wave-wide) broadcast of lane `src`'s value; src must be uniform.
+__device__ __forceinline__ double readlane_d(double v, int src) {
+  int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+  int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float readlane_f(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double readlane(double v, int s) { return readlane_d(v, s); }
+__device__ __forceinline__ float readlane(float v, int s) { return readlane_f(v, s); }
+
+// Lane shuffle with a per-lane source (ds_bpermute)
+__device__ __forceinline__ double shfl_d(double v, int src) {
+  int lo = __builtin_amdgcn_ds_bpermute(src << 2, __double2loint(v));
+  int hi = __builtin_amdgcn_ds_bpermute(src << 2, __double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float shfl_f(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+}
+__device__ __forceinline__ double shfl(double v, int s) { return shfl_d(v, s); }
+__device__ __forceinline__ float shfl(float v, int s) { return shfl_f(v, s); }
+
+// Butterfly sum over the 64 lanes (every lane gets the total).  The order
+// of additions is fixed, so the result is deterministic.
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+  return v;
+}
+
+// Non-negative modulo
+__device__ __forceinline__ int wrapi(int i, int m) {
+  int r = i % m;
+  return r < 0 ? r + m : r;
+}
+
+// Explicitly non-contracted FP64 / FP32 arithmetic (the sphere test and cube
+// selection must round exactly as the reference's separate operations).
+__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ double add_rn(double a, double b) { return __dadd_rn(a, b); }
+__device__ __forceinline__ double sub_rn(double a, double b) { return __dsub_rn(a, b); }
+__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ float add_rn(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ float sub_rn(float a, float b) { return __fsub_rn(a, b); }
+
+__device__ __forceinline__ double gexp(double x) { return exp(x); }
+__device__ __forceinline__ float gexp(float x) { return expf(x); }
+
+// Error-free transformations for compensated sums (Ogita-Rump-Oishi):
+// TwoSum (Knuth) and TwoProd via FMA.
+template <typename T>
+__device__ __forceinline__ void two_sum(T a, T b, T& s, T& e) {
+  s = a + b;
+  T bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void two_prod(float a, float b, float& p, float& e) {
+  p = a * b;
+  e = __builtin_fmaf(a, b, -p);
+}
+__device__ __forceinline__ void two_prod(double a, double b, double& p, double& e) {
+  p = a * b;
+  e = __builtin_fma(a, b, -p);
+}
+
+// Compensated accumulator: value = s + c.
+template <typename T>
+struct KSum {
+  T s, c;
+  __device__ __forceinline__ KSum() : s(0), c(0) {}
+  __device__ __forceinline__ void add(T v) {
+    T t, e;
+    two_sum(s, v, t, e);
+    s = t;
+    c += e;
+  }
+  __device__ __forceinline__ void add_prod(T a, T b) {
+    T p, ep, t, es;
+    two_prod(a, b, p, ep);
+    two_sum(s, p, t, es);
+    s = t;
+    c += ep + es;
+  }
+  __device__ __forceinline__ T value() const { return s + c; }
+};
+
+}  // namespace gse

@@ -1,0 +1,313 @@
+/*
+ * gse_hip.h -- C-ABI of the MI355X (gfx950) GSE reciprocal-space library.
+ *
+ * This is the boundary between callers (the C++ drop-in classes in
+ * include/gse/, Python via ctypes, any FFI) and the HIP kernels.  It mirrors
+ * the reference's class API for the GSE k-space path of ap-hynninen/GSE;
+ * each entry point names the reference declaration it replaces.
+ *
+ * Conventions
+ *   - Plain C types only; every call returns gse_status_t and never exits or
+ *     prints.  gse_last_error() gives the calling thread's last message.
+ *   - Arrays passed to the gse_* calls are DEVICE pointers unless the
+ *     parameter says "host".  Calls are asynchronous on the handle's HIP
+ *     stream (gse_*_set_stream; default: the null stream), except those
+ *     documented as synchronising.
+ *   - Particles are xyzq records {x, y, z, q} of the handle's compute type CT
+ *     (TypesGSE.h:20-22).  Grids are x-fastest, data[x + (y + z*Y)*X]
+ *     (Grid.h:32-44).  Forces and potentials have the accumulation type AT.
+ *     Half spectra hold (X/2+1)*Y*Z interleaved complex values.
+ *   - The supported (AT, CT) pairs are the reference's instantiations
+ *     (CpuGaussCharge.cpp:524-526): (f32,f32), (f64,f32), (f64,f64).
+ */
+#ifndef GSE_HIP_H
+#define GSE_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSE_API __attribute__((visibility("default")))
+
+typedef enum gse_status {
+  GSE_SUCCESS = 0,
+  GSE_ERROR_INVALID_VALUE = 1, /* bad size, null pointer, dtype pair, parameter */
+  GSE_ERROR_OUT_OF_MEMORY = 2,
+  GSE_ERROR_HIP = 3,           /* HIP runtime or kernel launch failure */
+  GSE_ERROR_FFT = 4,           /* rocFFT failure */
+  GSE_ERROR_BAD_PARTICLES = 5, /* non-finite or out-of-range particles seen */
+  GSE_ERROR_NOT_SUPPORTED = 6
+} gse_status_t;
+
+typedef enum gse_dtype { GSE_FLOAT32 = 0, GSE_FLOAT64 = 1 } gse_dtype_t;
+
+/* Green's function plug-ins (CpuGreensFunc.h:13-19):
+ *   GSE: 4 pi/(XYZ) exp(-sigma^2 k^2/2)/k^2, k=0 -> 0 (CpuGreensFuncGSE.cpp:9-56);
+ *        sigma = 0 gives the bare Coulomb kernel.
+ *   G2 : exp(-sigma^2 k^2/2)/(XYZ) (CpuGreensFuncG2.cpp:9-60). */
+typedef enum gse_greens_kind { GSE_GREENS_GSE = 0, GSE_GREENS_G2 = 1 } gse_greens_kind_t;
+
+GSE_API const char* gse_status_string(gse_status_t status);
+GSE_API const char* gse_last_error(void);
+GSE_API int gse_version(void);
+
+/* Bin edge (grid cells) and LDS budget (bytes) used to tile spread and
+ * gather; 0 keeps the built-in choice.  Process-wide. */
+GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
+
+/* ------------------------------------------------------------------------
+ * CpuGaussCharge<AT,CT>  (CpuGaussCharge.h:11-49)
+ * ---------------------------------------------------------------------- */
+typedef struct gse_gauss_charge_s* gse_gauss_charge_t;
+
+/* CpuGaussCharge.h:30  CpuGaussCharge(sizeX, sizeY, sizeZ) */
+GSE_API gse_status_t gse_gauss_charge_create(gse_gauss_charge_t* out, gse_dtype_t at,
+                                             gse_dtype_t ct, int sizeX, int sizeY, int sizeZ);
+/* CpuGaussCharge.h:31  ~CpuGaussCharge() */
+GSE_API gse_status_t gse_gauss_charge_destroy(gse_gauss_charge_t h);
+GSE_API gse_status_t gse_gauss_charge_set_stream(gse_gauss_charge_t h, void* hip_stream);
+
+/* CpuGaussCharge.h:32-33  calcSupportSize (host only, no device work) */
+GSE_API gse_status_t gse_gauss_charge_calc_support_size(gse_gauss_charge_t h, double rcut,
+                                                        double boxx, double boxy, double boxz,
+                                                        int* nsup);
+
+/* CpuGaussCharge.h:42-45  spreadChargeToGrid(sigma, rcut, numCoord, xyzq,
+ * boxx, boxy, boxz, rho, normalize).  rho is overwritten (the reference
+ * clears it first).  Accumulation is int64 fixed point: the grid is
+ * bit-identical run to run. */
+GSE_API gse_status_t gse_gauss_charge_spread_charge_to_grid(gse_gauss_charge_t h, double sigma,
+                                                            double rcut, int numCoord,
+                                                            const void* xyzq, double boxx,
+                                                            double boxy, double boxz, void* rho,
+                                                            int normalize);
+
+/* CpuGaussCharge.h:46-48  spreadChargeOnGrid(sigma, rcut, box, rho, rhoS):
+ * separable X->Y->Z periodic Gaussian convolution with radius ceil(rcut/h)
+ * and the (2 pi sigma^2)^-3/2 h^3 prefactor on the Z taps. */
+GSE_API gse_status_t gse_gauss_charge_spread_charge_on_grid(gse_gauss_charge_t h, double sigma,
+                                                            double rcut, double boxx,
+                                                            double boxy, double boxz,
+                                                            const void* rho, void* rhoS);
+
+/* CpuGaussCharge.h:36-39  interpolateForce(sigma, rcut, numCoord, xyzq,
+ * box, phi, forceX, forceY, forceZ): sphere support r^2 < rcut^2, no
+ * ccelec factor (CpuGaussCharge.cpp:105-183). */
+GSE_API gse_status_t gse_gauss_charge_interpolate_force(gse_gauss_charge_t h, double sigma,
+                                                        double rcut, int numCoord,
+                                                        const void* xyzq, double boxx,
+                                                        double boxy, double boxz,
+                                                        const void* phi, void* forceX,
+                                                        void* forceY, void* forceZ);
+
+/* CpuGaussCharge.h:34-35  calcElectricFieldOnGrid */
+GSE_API gse_status_t gse_gauss_charge_calc_electric_field_on_grid(gse_gauss_charge_t h,
+                                                                  double boxx, double boxy,
+                                                                  double boxz, const void* phi,
+                                                                  void* Ex, void* Ey, void* Ez);
+
+/* CpuGaussCharge.h:40-41  calcDipoleSum; dipSum is a host array of 3
+ * doubles.  Synchronises the stream. */
+GSE_API gse_status_t gse_gauss_charge_calc_dipole_sum(gse_gauss_charge_t h, double boxx,
+                                                      double boxy, double boxz,
+                                                      const void* rho, double* dipSum);
+
+/* Build-defined gather over one cube walk (no reference counterpart; it
+ * extends interpolateForce).  Outputs may be NULL; `scale` multiplies all. */
+#define GSE_GATHER_SPHERE_FORCE 1 /* reference force, sphere support */
+#define GSE_GATHER_CUBE_FORCE 2   /* exact -dE/dr, cube support */
+#define GSE_GATHER_POTENTIAL 4    /* phi_i = (2 pi s^2)^-3/2 h^3 sum_cube w phi */
+#define GSE_GATHER_VIRIAL 8       /* sum_i q_i prefF sum_cube w phi d_a d_b */
+typedef struct gse_gather_desc {
+  int what;         /* mask of GSE_GATHER_* */
+  double scale;     /* e.g. ccelec */
+  void* forceX;     /* sphere force (AT) */
+  void* forceY;
+  void* forceZ;
+  void* cubeX;      /* cube force (AT) */
+  void* cubeY;
+  void* cubeZ;
+  void* potential;  /* AT */
+  double* virial;   /* device, 9 doubles (row-major 3x3), overwritten */
+} gse_gather_desc_t;
+GSE_API gse_status_t gse_gauss_charge_gather(gse_gauss_charge_t h, double sigma, double rcut,
+                                             int numCoord, const void* xyzq, double boxx,
+                                             double boxy, double boxz, const void* phi,
+                                             const gse_gather_desc_t* desc);
+
+/* Number of particles the last spread/gather rejected (non-finite or
+ * |x/h| >= 1e9).  Synchronises the stream. */
+GSE_API gse_status_t gse_gauss_charge_invalid_count(gse_gauss_charge_t h, int* count);
+
+/* ------------------------------------------------------------------------
+ * CpuConvolution<T>  (CpuConvolution.h:9-38)
+ * ---------------------------------------------------------------------- */
+typedef struct gse_convolution_s* gse_convolution_t;
+
+/* CpuConvolution.h:33 */
+GSE_API gse_status_t gse_convolution_create(gse_convolution_t* out, gse_dtype_t t, int sizeX,
+                                            int sizeY, int sizeZ);
+/* CpuConvolution.h:34 */
+GSE_API gse_status_t gse_convolution_destroy(gse_convolution_t h);
+GSE_API gse_status_t gse_convolution_set_stream(gse_convolution_t h, void* hip_stream);
+/* CpuConvolution.h:35-36  setupKernel(Rx, Ry, Rz, h_kernelX, h_kernelY,
+ * h_kernelZ): the taps are HOST arrays of 2R+1 values of type T. */
+GSE_API gse_status_t gse_convolution_setup_kernel(gse_convolution_t h, int kernelRadiusX,
+                                                  int kernelRadiusY, int kernelRadiusZ,
+                                                  const void* h_kernelX, const void* h_kernelY,
+                                                  const void* h_kernelZ);
+/* CpuConvolution.h:37  run(data, dataXYZ): the reference convolves along X
+ * only and copies the result (CpuConvolution.cpp:134-150); so does this. */
+GSE_API gse_status_t gse_convolution_run(gse_convolution_t h, const void* data, void* dataXYZ);
+/* The full separable X->Y->Z convolution with the three kernels. */
+GSE_API gse_status_t gse_convolution_run_xyz(gse_convolution_t h, const void* data,
+                                             void* dataXYZ);
+
+/* ------------------------------------------------------------------------
+ * CpuFFTSolver<T> with a CpuGreensFunc<T>  (CpuFFTSolver.h:9-43)
+ * ---------------------------------------------------------------------- */
+typedef struct gse_fft_solver_s* gse_fft_solver_t;
+
+/* CpuFFTSolver.h:39  CpuFFTSolver(greensFunc, sizeX, sizeY, sizeZ); the
+ * Green's function is chosen by kind. */
+GSE_API gse_status_t gse_fft_solver_create(gse_fft_solver_t* out, gse_dtype_t t,
+                                           gse_greens_kind_t kind, int sizeX, int sizeY,
+                                           int sizeZ);
+/* CpuFFTSolver.h:40 */
+GSE_API gse_status_t gse_fft_solver_destroy(gse_fft_solver_t h);
+GSE_API gse_status_t gse_fft_solver_set_stream(gse_fft_solver_t h, void* hip_stream);
+/* CpuFFTSolver.h:41-42  run(phi, rho, sigma, boxx, boxy, boxz):
+ * phi = c2r(G . r2c(rho)), both transforms unnormalised (FFTW semantics). */
+GSE_API gse_status_t gse_fft_solver_run(gse_fft_solver_t h, void* phi, const void* rho,
+                                        double sigma, double boxx, double boxy, double boxz);
+/* The two transforms of CpuFFTSolver.cpp:88 and :94 on their own, for a
+ * caller that applies its own Green's function between them.  c2r may
+ * overwrite rhoK. */
+GSE_API gse_status_t gse_fft_solver_r2c(gse_fft_solver_t h, const void* rho, void* rhoK);
+GSE_API gse_status_t gse_fft_solver_c2r(gse_fft_solver_t h, void* rhoK, void* phi);
+
+/* CpuGreensFunc.h:16-18  mul(nfftX, nfftY, nfftZ, box, sigma, rhoK) */
+GSE_API gse_status_t gse_greens_func_mul(gse_dtype_t t, gse_greens_kind_t kind, int nfftX,
+                                         int nfftY, int nfftZ, double boxx, double boxy,
+                                         double boxz, double sigma, void* rhoK,
+                                         void* hip_stream);
+
+/* ------------------------------------------------------------------------
+ * CpuGSE / CpuGSEk<AT,CT>  (CpuGSE.h:18-68, CpuGSEk.h:18-37)
+ * ---------------------------------------------------------------------- */
+typedef struct gse_gsek_s* gse_gsek_t;
+
+/* CpuGSEk.h:26-29  CpuGSEk(sizeX, sizeY, sizeZ, sigma, kappa, lambdaSigma,
+ * lambdaSigma1, boxx, boxy, boxz, ccelec); asserts of CpuGSE.cpp:33-44
+ * (0 <= kappa <= 0.5) become GSE_ERROR_INVALID_VALUE. */
+GSE_API gse_status_t gse_gsek_create(gse_gsek_t* out, gse_dtype_t at, gse_dtype_t ct,
+                                     int sizeX, int sizeY, int sizeZ, double sigma,
+                                     double kappa, double lambdaSigma, double lambdaSigma1,
+                                     double boxx, double boxy, double boxz, double ccelec);
+/* CpuGSEk.h:30 */
+GSE_API gse_status_t gse_gsek_destroy(gse_gsek_t h);
+GSE_API gse_status_t gse_gsek_set_stream(gse_gsek_t h, void* hip_stream);
+/* CpuGSE.h:58  setBoxSize */
+GSE_API gse_status_t gse_gsek_set_box_size(gse_gsek_t h, double boxx, double boxy, double boxz);
+/* CpuGSE.h:65-67  getSigma1, getSigma2, getLambdaSigma1 (host) */
+GSE_API gse_status_t gse_gsek_get_params(gse_gsek_t h, double* sigma1, double* sigma2,
+                                         double* lambdaSigma1);
+/* CpuGSEk.h:31  spreadCharge1(numCoord, xyzq): sigma1 spread into rhoS */
+GSE_API gse_status_t gse_gsek_spread_charge1(gse_gsek_t h, int numCoord, const void* xyzq);
+/* CpuGSEk.h:32  spreadCharge2(rhoOut): sigma2 on-grid spread of rhoS */
+GSE_API gse_status_t gse_gsek_spread_charge2(gse_gsek_t h, void* rhoOut);
+/* CpuGSEk.h:33  solvePoisson(): phiM = FFT solve of rhoS with G_GSE(sigma2) */
+GSE_API gse_status_t gse_gsek_solve_poisson(gse_gsek_t h);
+/* CpuGSEk.h:34  calculateEnergy(): 0.5*ccelec*h^3*sum(rhoS*phiM)
+ * (CpuGSE.cpp:86-104).  Synchronises; host result. */
+GSE_API gse_status_t gse_gsek_calculate_energy(gse_gsek_t h, double* energy);
+/* CpuGSE.h:61  interpolateForce(numCoord, xyzq, forceX, forceY, forceZ):
+ * sphere-support force times ccelec (CpuGSE.cpp:70-80). */
+GSE_API gse_status_t gse_gsek_interpolate_force(gse_gsek_t h, int numCoord, const void* xyzq,
+                                                void* forceX, void* forceY, void* forceZ);
+/* CpuGSE.h:63-64  getRhoS / getPhi: device pointers owned by the handle */
+GSE_API gse_status_t gse_gsek_get_rho_s(gse_gsek_t h, void** rhoS);
+GSE_API gse_status_t gse_gsek_get_phi(gse_gsek_t h, void** phi);
+/* Asynchronous device copy of rhoS (which = 0) or phiM (which = 1) into dst. */
+GSE_API gse_status_t gse_gsek_copy_grid(gse_gsek_t h, int which, void* dst);
+
+/* Build-defined fused step (no reference counterpart).
+ * mode 0 (K): spread sigma1 -> r2c -> G_GSE(sigma2) k-pass with FP64
+ *   energy/virial -> c2r -> gather; reproduces CpuGSEk.
+ * mode 1 (R): spread sigma1 -> on-grid sigma2 convolution (rcut2) -> r2c ->
+ *   G_GSE(0) k-pass -> c2r -> energy sum(rho1*phi) -> gather; the
+ *   composition of CpuGaussCharge + CpuFFTSolver as CpuGSEr drives it.
+ * Outputs may be NULL.  energy[0] (device) receives E; virial (device, 9
+ * doubles, mode K only) receives W_ab = -dE/d eps_ab. */
+typedef struct gse_step_desc {
+  int mode;
+  void* forceX;     /* reference sphere force x ccelec (AT) */
+  void* forceY;
+  void* forceZ;
+  void* potential;  /* phi_i = dE/dq_i (AT) */
+  void* cubeX;      /* exact force -dE/dr_i (AT) */
+  void* cubeY;
+  void* cubeZ;
+  double* energy;   /* device, 1 double */
+  double* virial;   /* device, 9 doubles */
+} gse_step_desc_t;
+GSE_API gse_status_t gse_gsek_step(gse_gsek_t h, int numCoord, const void* xyzq,
+                                   const gse_step_desc_t* desc);
+
+/* Build-defined adjoint sweep (mode K): recomputes spread -> k-pass ->
+ * gather from xyzq (no grid is kept from the forward) and writes, scaled
+ * by the upstream gradient *grad_energy (device scalar; NULL = 1):
+ *   dEdq[i] = dE/dq_i, dEdx/y/z[i] = dE/dr_i (cube support, exact),
+ * and virial (device, 9 doubles, unscaled) = -dE/d eps.  Any output NULL. */
+GSE_API gse_status_t gse_gsek_adjoint(gse_gsek_t h, int numCoord, const void* xyzq,
+                                      const double* grad_energy, void* dEdq, void* dEdx,
+                                      void* dEdy, void* dEdz, double* virial);
+
+/* Stage timing of gse_gsek_step: with profiling on, every step records HIP
+ * events on the handle's stream around its stages
+ *   0 bin, 1 spread, 2 fixed-point -> grid conversion, 3 r2c FFT,
+ *   4 k-pass, 5 c2r FFT, 6 gather
+ * (mode R adds its convolution to stage 3).  gse_gsek_stage_times
+ * synchronises, writes the mean milliseconds per stage over the recorded
+ * steps into ms[0..6], the number of steps into *steps, and clears them. */
+#define GSE_NUM_STAGES 7
+GSE_API gse_status_t gse_gsek_set_profiling(gse_gsek_t h, int enable);
+GSE_API gse_status_t gse_gsek_stage_times(gse_gsek_t h, double* ms, int* steps);
+
+/* Closed-form corrections in FP64 (device out[2]): out[0] self energy
+ * -ccelec sum q^2 / sqrt(2 pi sigma^2) (CpuEwaldDirect.cpp:124-128),
+ * out[1] neutralising-background energy -ccelec pi Q^2 sigma^2 / V. */
+GSE_API gse_status_t gse_gsek_corrections(gse_gsek_t h, int numCoord, const void* xyzq,
+                                          double* out);
+
+/* ------------------------------------------------------------------------
+ * Batched independent systems (build-defined)
+ * ---------------------------------------------------------------------- */
+typedef struct gse_batch_s* gse_batch_t;
+
+/* n systems; for system s: sizes[3s..3s+2] grid, boxes[3s..3s+2] box,
+ * counts[s] particles.  All systems share sigma, kappa, lambdas, ccelec and
+ * the (AT,CT) pair.  Host arrays. */
+GSE_API gse_status_t gse_batch_create(gse_batch_t* out, gse_dtype_t at, gse_dtype_t ct,
+                                      int nsys, const int* sizes, const double* boxes,
+                                      const int* counts, double sigma, double kappa,
+                                      double lambdaSigma, double lambdaSigma1, double ccelec);
+GSE_API gse_status_t gse_batch_destroy(gse_batch_t h);
+GSE_API gse_status_t gse_batch_set_stream(gse_batch_t h, void* hip_stream);
+/* xyzq: all systems' particles concatenated in system order (device).
+ * forces: AT arrays of the same concatenated length (sphere force x ccelec);
+ * potential may be NULL; energies: device, nsys doubles. */
+GSE_API gse_status_t gse_batch_step(gse_batch_t h, const void* xyzq, void* forceX, void* forceY,
+                                    void* forceZ, void* potential, double* energies);
+/* Capture one batched step in a HIP graph and replay it on later calls
+ * (the pointers given at capture time are reused). */
+GSE_API gse_status_t gse_batch_step_graph(gse_batch_t h, const void* xyzq, void* forceX,
+                                          void* forceY, void* forceZ, void* potential,
+                                          double* energies);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GSE_HIP_H */

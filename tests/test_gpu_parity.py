@@ -1,0 +1,350 @@
+"""HIP path vs the FP64 oracle, through the C-ABI (gse_amd/api.py -> ctypes).
+
+Tolerances (the bar for FP64 mode is 1e-10 relative):
+  * grids, forces, potentials: max|gpu - oracle| / max|oracle| <= 1e-10
+    (the spread's int64 fixed point and the FFT differ from the oracle only
+    by rounding, ~1e-14);
+  * energies: |gpu - oracle| / |oracle| <= 1e-10;
+  * FP32 mode: relative force RMS (cpu_gse.cpp:131-146 Erms) <= 1e-5.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from gse_amd import api, systems
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+F64 = torch.float64
+F32 = torch.float32
+DEV = "cuda"
+TOL = 1e-10
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    s = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / s) if s > 0 else float(np.max(np.abs(a - b)))
+
+
+def to_dev(a, dt=F64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device=DEV, dtype=dt).contiguous()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().double().cpu().numpy()
+
+
+def grid(dims, dt=F64):
+    X, Y, Z = dims
+    return torch.zeros((Z, Y, X), dtype=dt, device=DEV)
+
+
+CASES = [
+    # name, xyzq factory, dims, box, sigma(total)
+    ("pair_diag", lambda: systems.pair(1.0, 20.0, orientation="diagonal"), (20, 20, 20), (20.0,) * 3, 2.121320344),
+    ("pair_seed1", lambda: systems.pair(1.0, 20.0, seed=1), (20, 20, 20), (20.0,) * 3, 2.121320344),
+    ("water_3k", lambda: systems.water_box(1000, 31.0, seed=3), (32, 32, 32), (31.0,) * 3, 2.12),
+    ("aniso", lambda: systems.water_box(150, 18.0, seed=7) * np.array([1.0, 0.9, 1.2, 1.0]), (18, 16, 24), (18.0, 16.2, 21.6), 2.0),
+    ("wrap_M8", lambda: systems.random_alternating(12, 8.0, seed=4), (8, 8, 8), (8.0,) * 3, 2.121320344),
+    ("negative", lambda: systems.random_alternating(40, 16.0, seed=6) - np.array([8.0, 3.0, 15.0, 0.0]), (16, 16, 16), (16.0,) * 3, 2.0),
+]
+
+
+@pytest.fixture(scope="module", params=CASES, ids=[c[0] for c in CASES])
+def case(request):
+    name, make, dims, box, sigma = request.param
+    xyzq = make()
+    p = orc.Params(dims, box, sigma)
+    return name, xyzq, p
+
+
+# --------------------------------------------------------------- spread
+@pytest.mark.parametrize("normalize", [False, True])
+def test_spread_fp64(case, normalize):
+    name, xyzq, p = case
+    g = api.CpuGaussCharge(F64, F64, *p.dims)
+    rho = grid(p.dims)
+    g.spreadChargeToGrid(p.sigma1, p.rcut1, len(xyzq), to_dev(xyzq), *p.box, rho, normalize)
+    ref = orc.spread(p.dims, p.sigma1, p.rcut1, xyzq, p.box, normalize)
+    assert rel(host(rho), ref) <= TOL, name
+    # bit-identical on a second run (int64 accumulation)
+    rho2 = grid(p.dims)
+    g.spreadChargeToGrid(p.sigma1, p.rcut1, len(xyzq), to_dev(xyzq), *p.box, rho2, normalize)
+    assert torch.equal(rho, rho2)
+
+
+def test_spread_order_independent():
+    """Reversing the particle order leaves the grid bit-identical."""
+    xyzq = systems.water_box(500, 25.0, seed=12)
+    p = orc.Params(24, 25.0, 2.12)
+    g = api.CpuGaussCharge(F64, F64, *p.dims)
+    a, b = grid(p.dims), grid(p.dims)
+    g.spreadChargeToGrid(p.sigma1, p.rcut1, len(xyzq), to_dev(xyzq), *p.box, a)
+    g.spreadChargeToGrid(p.sigma1, p.rcut1, len(xyzq), to_dev(xyzq[::-1]), *p.box, b)
+    assert torch.equal(a, b)
+
+
+def test_spread_fp32_vs_oracle():
+    xyzq = systems.water_box(1000, 31.0, seed=3)
+    p = orc.Params(32, 31.0, 2.12)
+    g = api.CpuGaussCharge(F32, F32, *p.dims)
+    rho = grid(p.dims, F32)
+    g.spreadChargeToGrid(p.sigma1, p.rcut1, len(xyzq), to_dev(xyzq, F32), *p.box, rho)
+    ref = orc.spread(p.dims, p.sigma1, p.rcut1, xyzq.astype(np.float32).astype(np.float64), p.box)
+    assert rel(host(rho), ref) < 2e-6
+
+
+# --------------------------------------------------------------- gather
+def test_interpolate_force_fp64(case):
+    name, xyzq, p = case
+    X, Y, Z = p.dims
+    phi = np.random.default_rng(5).normal(size=(Z, Y, X))
+    g = api.CpuGaussCharge(F64, F64, *p.dims)
+    n = len(xyzq)
+    f = torch.zeros((3, n), dtype=F64, device=DEV)
+    g.interpolateForce(p.sigma1, p.rcut1, n, to_dev(xyzq), *p.box, to_dev(phi), f[0], f[1], f[2])
+    ref = orc.interpolate(p.dims, p.sigma1, p.rcut1, xyzq, p.box, phi)
+    assert rel(host(f).T, ref) <= TOL, name
+
+
+def test_gather_cube_potential_virial_fp64(case):
+    name, xyzq, p = case
+    X, Y, Z = p.dims
+    phi = np.random.default_rng(6).normal(size=(Z, Y, X))
+    g = api.CpuGaussCharge(F64, F64, *p.dims)
+    o = g.gather(p.sigma1, p.rcut1, to_dev(xyzq), p.box, to_dev(phi), 15)
+    pot, fc, vir = orc.gather_cube(p.dims, p.sigma1, p.rcut1, xyzq, p.box, phi)
+    fs = orc.interpolate(p.dims, p.sigma1, p.rcut1, xyzq, p.box, phi)
+    assert rel(host(o["force"]).T, fs) <= TOL
+    assert rel(host(o["force_cube"]).T, fc) <= TOL
+    assert rel(host(o["pot"]), pot) <= TOL
+    assert rel(host(o["virial"]).reshape(3, 3), vir) <= TOL
+
+
+# ---------------------------------------------------------- k-space
+@pytest.mark.parametrize("kind,sigma", [(0, 0.9487), (0, 0.0), (1, 0.9487)])
+@pytest.mark.parametrize("dims,box", [((20, 20, 20), (20.0,) * 3), ((18, 16, 24), (18.0, 16.2, 21.6)),
+                                      ((15, 9, 10), (15.0, 9.5, 10.5))])
+def test_fft_solver_fp64(kind, sigma, dims, box):
+    X, Y, Z = dims
+    rho = np.random.default_rng(8).normal(size=(Z, Y, X))
+    gf = api.CpuGreensFuncGSE() if kind == 0 else api.CpuGreensFuncG2()
+    s = api.CpuFFTSolver(gf, F64, *dims)
+    phi = grid(dims)
+    s.run(phi, to_dev(rho), sigma, *box)
+    ref = orc.fft_solve(kind, dims, sigma, box, rho)
+    assert rel(host(phi), ref) <= TOL
+
+
+def test_greens_mul_and_transforms():
+    dims, box = (12, 10, 9), (12.0, 10.5, 9.5)
+    X, Y, Z = dims
+    rho = np.random.default_rng(9).normal(size=(Z, Y, X))
+    s = api.CpuFFTSolver(api.CpuGreensFuncGSE(), F64, *dims)
+    spec = torch.zeros((Z, Y, X // 2 + 1), dtype=torch.complex128, device=DEV)
+    s.r2c(to_dev(rho), spec)
+    ref_spec = orc.fft_r2c(rho)
+    assert rel(host(torch.view_as_real(spec)), np.stack([ref_spec.real, ref_spec.imag], -1)) <= TOL
+    api.CpuGreensFuncGSE().mul(X, Y, Z, *box, 0.9, spec)
+    ref_mul = orc.greens_mul(0, dims, box, 0.9, ref_spec)
+    assert rel(host(torch.view_as_real(spec)), np.stack([ref_mul.real, ref_mul.imag], -1)) <= TOL
+    phi = grid(dims)
+    s.c2r(spec, phi)
+    assert rel(host(phi), orc.fft_c2r(ref_mul, X)) <= TOL
+
+
+# ------------------------------------------------------- convolution
+def test_spread_on_grid_and_convolution_fp64():
+    dims, box, sigma = (16, 12, 20), (16.5, 12.0, 19.0), 0.9487
+    X, Y, Z = dims
+    rcut = 3 * math.sqrt(2) * sigma
+    rho = np.random.default_rng(10).normal(size=(Z, Y, X))
+    g = api.CpuGaussCharge(F64, F64, *dims)
+    out = grid(dims)
+    g.spreadChargeOnGrid(sigma, rcut, *box, to_dev(rho), out)
+    assert rel(host(out), orc.spread_on_grid(dims, sigma, rcut, box, rho)) <= TOL
+    R = 5
+    k = np.exp(-np.arange(-R, R + 1) ** 2 / (2 * 2.12 ** 2))
+    c = api.CpuConvolution(F64, *dims)
+    c.setupKernel(R, R, R, k, k * 0.5, k * 2.0)
+    o1 = grid(dims)
+    c.run(to_dev(rho), o1)
+    assert rel(host(o1), orc.conv_run_x(dims, k, rho)) <= TOL
+    o3 = grid(dims)
+    c.runXYZ(to_dev(rho), o3)
+    assert rel(host(o3), orc.conv_run_xyz(dims, k, k * 0.5, k * 2.0, rho)) <= TOL
+
+
+def test_convolution_fp32_compensated():
+    dims = (32, 32, 32)
+    X, Y, Z = dims
+    rho = np.random.default_rng(11).normal(size=(Z, Y, X))
+    R = 5
+    k = np.exp(-np.arange(-R, R + 1) ** 2 / (2 * 2.12 ** 2))
+    c = api.CpuConvolution(F32, *dims)
+    c.setupKernel(R, R, R, k, k, k)
+    o = grid(dims, F32)
+    c.runXYZ(to_dev(rho, F32), o)
+    ref = orc.conv_run_xyz(dims, k, k, k, rho.astype(np.float32).astype(np.float64))
+    assert rel(host(o), ref) < 1e-6
+
+
+def test_efield_dipole_fp64():
+    dims, box = (10, 12, 8), (10.0, 12.5, 8.5)
+    X, Y, Z = dims
+    phi = np.random.default_rng(12).normal(size=(Z, Y, X))
+    g = api.CpuGaussCharge(F64, F64, *dims)
+    e = [grid(dims) for _ in range(3)]
+    g.calcElectricFieldOnGrid(*box, to_dev(phi), *e)
+    ref = orc.efield(dims, box, phi)
+    for a in range(3):
+        assert rel(host(e[a]), ref[a]) <= TOL
+    d = g.calcDipoleSum(*box, to_dev(phi))
+    assert rel(d, orc.dipole_sum(dims, box, phi)) <= TOL
+
+
+def test_support_size_matches_oracle():
+    g = api.CpuGaussCharge(F64, F64, 20, 20, 20)
+    assert g.calcSupportSize(5.6921, 20.0, 20.0, 20.0) == orc.support_size((20, 20, 20), 5.6921, (20.0,) * 3)
+
+
+# ------------------------------------------------------ full GSE step
+def test_gsek_step_fp64(case):
+    name, xyzq, p = case
+    r = orc.gse_step(p, xyzq, mode=0)
+    gs = api.CpuGSEk(F64, F64, *p.dims, p.sigma, 0.4, 3.0, 3.0, *p.box)
+    o = gs.step(to_dev(xyzq), mode=0, force=True, potential=True, cube=True, virial=True)
+    E = float(host(o["energy"])[0])
+    assert abs(E - r.energy) <= TOL * abs(r.energy), (name, E, r.energy)
+    assert rel(host(o["force"]).T, r.force) <= TOL
+    assert rel(host(o["force_cube"]).T, r.force_cube) <= TOL
+    assert rel(host(o["pot"]), r.pot) <= TOL
+    assert rel(host(o["virial"]).reshape(3, 3), r.virial) <= TOL
+
+
+def test_gsek_reference_call_sequence_fp64():
+    """The CpuGSEk calls in the order cpu_gse.cpp:719-735 makes them."""
+    xyzq = systems.pair(1.0, 20.0, orientation="diagonal")
+    p = orc.Params(20, 20.0, 2.121320344)
+    r = orc.gse_step(p, xyzq, mode=0, want_grids=True)
+    gs = api.CpuGSEk(F64, F64, 20, 20, 20, p.sigma, 0.4, 3.0, 3.0, 20.0, 20.0, 20.0)
+    assert abs(gs.getSigma1() - p.sigma1) < 1e-15 and abs(gs.getSigma2() - p.sigma2) < 1e-15
+    x = to_dev(xyzq)
+    gs.spreadCharge1(2, x)
+    gs.solvePoisson()
+    E = gs.calculateEnergy()
+    f = torch.zeros((3, 2), dtype=F64, device=DEV)
+    gs.interpolateForce(2, x, f[0], f[1], f[2])
+    assert abs(E - r.energy) <= TOL * abs(r.energy)
+    assert rel(host(f).T, r.force) <= TOL
+    assert rel(host(gs.getRhoS()), r.rho) <= TOL
+    assert rel(host(gs.getPhi()), r.phi) <= TOL
+
+
+def test_gsek_mode_r_fp64():
+    xyzq = systems.water_box(300, 24.0, seed=14)
+    p = orc.Params(24, 24.0, 2.12)
+    r = orc.gse_step(p, xyzq, mode=1)
+    gs = api.CpuGSEk(F64, F64, *p.dims, p.sigma, 0.4, 3.0, 3.0, *p.box)
+    o = gs.step(to_dev(xyzq), mode=1, force=True, potential=True, cube=True)
+    E = float(host(o["energy"])[0])
+    assert abs(E - r.energy) <= TOL * abs(r.energy)
+    assert rel(host(o["force"]).T, r.force) <= TOL
+    assert rel(host(o["pot"]), r.pot) <= TOL
+
+
+def test_gsek_fp32_water48k_force_rms():
+    """FP32 spread/FFT/gather with FP64 G table and reductions, 48,000
+    charges at M=80, against the FP64 oracle on the same inputs."""
+    xyzq, L, M, sigma = systems.make("water_48k_M80")
+    p = orc.Params(M, L, sigma)
+    r = orc.gse_step(p, xyzq, mode=0)
+    gs = api.CpuGSEk(F32, F32, M, M, M, sigma, 0.4, 3.0, 3.0, L, L, L)
+    o = gs.step(to_dev(xyzq, F32), mode=0, force=True, potential=True, cube=True)
+    erms = orc.erms(r.force, host(o["force"]).T)
+    print(f"FP32 water48k: force Erms = {erms:.3e}")
+    assert erms <= 1e-5
+    assert orc.erms(r.force_cube, host(o["force_cube"]).T) <= 1e-5
+    E = float(host(o["energy"])[0])
+    assert abs(E - r.energy) <= 1e-5 * abs(r.energy)
+    # (AT,CT) = (double, float): the reference's mixed instantiation
+    gm = api.CpuGSEk(F64, F32, M, M, M, sigma, 0.4, 3.0, 3.0, L, L, L)
+    om = gm.step(to_dev(xyzq, F32), mode=0, force=True, potential=False)
+    assert orc.erms(r.force, host(om["force"]).T) <= 1e-5
+
+
+def test_energy_parseval_and_real_space_agree_fp64():
+    xyzq = systems.water_box(800, 30.0, seed=15)
+    gs = api.CpuGSEk(F64, F64, 30, 30, 30, 2.12, 0.4, 3.0, 3.0, 30.0, 30.0, 30.0)
+    o = gs.step(to_dev(xyzq), force=False, potential=False)
+    Ek = float(host(o["energy"])[0])
+    assert abs(gs.calculateEnergy() - Ek) <= 1e-12 * abs(Ek)
+
+
+def test_corrections_closed_form():
+    xyzq = np.loadtxt if False else systems.random_alternating(11, 12.0, seed=3)
+    xyzq[:, 3] = np.random.default_rng(2).normal(size=11)
+    gs = api.CpuGSEk(F64, F64, 12, 12, 12, 2.0, 0.4, 3.0, 3.0, 12.0, 12.0, 12.0)
+    c = host(gs.corrections(to_dev(xyzq)))
+    es, en = orc.corrections(2.0, (12.0,) * 3, xyzq)
+    assert abs(c[0] - es) <= 1e-12 * abs(es) and abs(c[1] - en) <= 1e-12 * abs(en)
+
+
+# ------------------------------------------------------ edge cases
+def test_empty_system():
+    gs = api.CpuGSEk(F64, F64, 16, 16, 16, 2.0, 0.4, 3.0, 3.0, 16.0, 16.0, 16.0)
+    x = torch.zeros((0, 4), dtype=F64, device=DEV)
+    o = gs.step(x, force=True, potential=True, virial=True)
+    assert float(host(o["energy"])[0]) == 0.0
+    assert np.all(host(o["virial"]) == 0.0)
+
+
+def test_invalid_particles_are_rejected_and_counted():
+    xyzq = systems.random_alternating(20, 16.0, seed=8)
+    bad = xyzq.copy()
+    bad[3, 0] = np.nan
+    bad[7, 2] = np.inf
+    p = orc.Params(16, 16.0, 2.0)
+    g = api.CpuGaussCharge(F64, F64, 16, 16, 16)
+    rho = grid(p.dims)
+    g.spreadChargeToGrid(p.sigma1, p.rcut1, 20, to_dev(bad), *p.box, rho)
+    assert g.invalidCount() == 2
+    good = np.delete(xyzq, [3, 7], axis=0)
+    assert rel(host(rho), orc.spread(p.dims, p.sigma1, p.rcut1, good, p.box)) <= TOL
+
+
+def test_periodic_images_give_same_energy():
+    xyzq = systems.water_box(200, 20.0, seed=9)
+    shifted = xyzq.copy()
+    shifted[:, :3] += np.array([20.0, -40.0, 60.0])
+    gs = api.CpuGSEk(F64, F64, 20, 20, 20, 2.12, 0.4, 3.0, 3.0, 20.0, 20.0, 20.0)
+    e1 = float(host(gs.step(to_dev(xyzq), force=False, potential=False)["energy"])[0])
+    e2 = float(host(gs.step(to_dev(shifted), force=False, potential=False)["energy"])[0])
+    assert abs(e1 - e2) <= 1e-10 * abs(e1)
+
+
+def test_non_neutral_system():
+    xyzq = systems.random_alternating(31, 18.0, seed=10)  # net charge -1
+    p = orc.Params(18, 18.0, 2.0)
+    r = orc.gse_step(p, xyzq)
+    gs = api.CpuGSEk(F64, F64, 18, 18, 18, 2.0, 0.4, 3.0, 3.0, 18.0, 18.0, 18.0)
+    o = gs.step(to_dev(xyzq), force=True, potential=True)
+    assert abs(float(host(o["energy"])[0]) - r.energy) <= TOL * abs(r.energy)
+    assert rel(host(o["force"]).T, r.force) <= TOL
+
+
+def test_bad_arguments_raise():
+    with pytest.raises(api._lib.GSEError):
+        api.CpuGSEk(F64, F64, 16, 16, 16, 2.0, 0.7, 3.0, 3.0, 16.0, 16.0, 16.0)  # kappa > 0.5
+    with pytest.raises(api._lib.GSEError):
+        api.CpuGaussCharge(F32, F64, 8, 8, 8)  # (AT,CT) = (float,double) is not an instance
+    g = api.CpuGaussCharge(F64, F64, 8, 8, 8)
+    with pytest.raises(api._lib.GSEError):
+        g.spreadChargeToGrid(-1.0, 3.0, 0, torch.zeros((0, 4), dtype=F64, device=DEV), 8.0, 8.0, 8.0,
+                             grid((8, 8, 8)))
