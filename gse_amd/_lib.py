@@ -46,6 +46,8 @@ _SIGS = {
     "gse_last_error": [],
     "gse_version": [],
     "gse_set_tiling": [I, I],
+    "gse_set_tuning_param": [ctypes.c_char_p, I],
+    "gse_get_tuning_param": [ctypes.c_char_p, PI],
     "gse_gauss_charge_create": [PV, I, I, I, I, I],
     "gse_gauss_charge_destroy": [V],
     "gse_gauss_charge_set_stream": [V, V],

@@ -426,3 +426,24 @@ class CpuGSEk:
 
 def set_tiling(bin_edge=0, lds_budget=0):
     check(_lib.load().gse_set_tiling(int(bin_edge), int(lds_budget)))
+
+
+TUNING_KEYS = ("bin_edge", "brick_budget", "spread_block", "gather_block", "spread_stage",
+               "gather_stage", "spread_lds32", "spread_cols")
+
+
+def set_tuning(**kw):
+    """Set named launch-shape knobs (see include/gse_hip.h)."""
+    L = _lib.load()
+    for k, v in kw.items():
+        check(L.gse_set_tuning_param(k.encode(), int(v)))
+
+
+def get_tuning():
+    L = _lib.load()
+    out = {}
+    for k in TUNING_KEYS:
+        v = ctypes.c_int()
+        check(L.gse_get_tuning_param(k.encode(), ctypes.byref(v)))
+        out[k] = v.value
+    return out

@@ -18,6 +18,12 @@
 
 using namespace gse;
 
+namespace gse {
+extern Tuning g_tuning;
+Tuning g_tuning;
+Tuning& tuning() { return g_tuning; }
+}  // namespace gse
+
 namespace {
 
 thread_local std::string g_err;
@@ -45,9 +51,6 @@ gse_status_t fail(gse_status_t s, const char* fmt, ...) {
     gse_status_t s_ = (expr);          \
     if (s_ != GSE_SUCCESS) return s_;  \
   } while (0)
-
-int g_bin_edge = 8;
-size_t g_lds_budget = 64 * 1024;
 
 struct DevBuf {
   void* p = nullptr;
@@ -114,44 +117,18 @@ gse_status_t check_sigma_rcut(double sigma, double rcut) {
 struct Engine {
   bool ct_dbl = true, at_dbl = true;
   int X = 0, Y = 0, Z = 0;
-  DevBuf counts, atom_bin, atom_slot, perm, sorted, fp, grid64, vir6, part;
+  DevBuf counts[2], stats[2], ovf, perm, sorted, grid64, vir6, part;
+  int cur = 1;                  // buffer of the latest binning
+  long long clean[2] = {0, 0};  // leading counts (and the stats) known zero
   bool grid64_clean = false;
   Binning bin;
   Geom geom{};
   BinShape shape{};
-  int nbound = -1;  // particle count of the current binning
 
   void release() {
-    for (DevBuf* b : {&counts, &atom_bin, &atom_slot, &perm, &sorted, &fp, &grid64, &vir6, &part})
+    for (DevBuf* b : {&counts[0], &counts[1], &stats[0], &stats[1], &ovf, &perm, &sorted,
+                      &grid64, &vir6, &part})
       b->release();
-  }
-
-  gse_status_t prepare(int n, const Geom& g, hipStream_t s) {
-    geom = g;
-    shape = choose_bins(g, 8, g_lds_budget, g_bin_edge);
-    GSECHK(counts.ensure(sizeof(int) * (shape.nbins + 1)));
-    GSECHK(atom_bin.ensure(sizeof(int) * (size_t)(n > 0 ? n : 1)));
-    GSECHK(atom_slot.ensure(sizeof(int) * (size_t)(n > 0 ? n : 1)));
-    GSECHK(perm.ensure(sizeof(int) * (size_t)(n > 0 ? n : 1)));
-    GSECHK(sorted.ensure(4 * tsize(ct_dbl) * (size_t)(n > 0 ? n : 1)));
-    GSECHK(fp.ensure(sizeof(FixedParams)));
-    GSECHK(part.ensure(sizeof(double) * kRedBlocks * 8));
-    const size_t npts = (size_t)X * Y * Z;
-    if (grid64.bytes < npts * 8 || !grid64.p) {
-      GSECHK(grid64.ensure(npts * 8));
-      grid64_clean = false;
-    }
-    if (!grid64_clean) {
-      HIPCHK(launch_fill_u64(grid64.as<unsigned long long>(), npts, s));
-      grid64_clean = true;
-    }
-    bin.counts = counts.as<int>();
-    bin.atom_bin = atom_bin.as<int>();
-    bin.atom_slot = atom_slot.as<int>();
-    bin.perm = perm.as<int>();
-    bin.sorted = sorted.p;
-    bin.fp = fp.as<FixedParams>();
-    return GSE_SUCCESS;
   }
 
   // Bins the particles for a support of radius rcut and Gaussian sigma.
@@ -161,49 +138,105 @@ struct Engine {
     if (n > 0 && !xyzq) return fail(GSE_ERROR_INVALID_VALUE, "xyzq is NULL");
     GSECHK(check_box(bx, by, bz));
     GSECHK(check_sigma_rcut(sigma, rcut));
-    Geom g = make_geom(X, Y, Z, bx, by, bz, rcut);
-    GSECHK(prepare(n, g, s));
-    const int images = ((2 * g.nx + 1 + X - 1) / X) * ((2 * g.ny + 1 + Y - 1) / Y) *
-                       ((2 * g.nz + 1 + Z - 1) / Z);
-    const double pref = std::pow(2.0 * M_PI * sigma * sigma, -1.5) *
-                        spread_weight_bound(g, sigma, normalize);
-    HIPCHK(launch_bin(ct_dbl, g, shape, n, xyzq, bin, images, pref, s));
-    nbound = n;
+    const Geom g = make_geom(X, Y, Z, bx, by, bz, rcut);
+    geom = g;
+    shape = choose_bins(g, 8, (size_t)g_tuning.brick_budget, g_tuning.bin_edge);
+    const int nb = shape.nbins;
+    int cap_bin;
+    if (shape.direct) {
+      cap_bin = n > 0 ? n : 1;
+    } else {  // twice the mean occupancy plus a margin; excess goes to the overflow list
+      const long long c = 2LL * n / nb + 32;
+      cap_bin = (int)((c + 15) / 16 * 16);
+    }
+    const size_t slots = (size_t)nb * cap_bin;
+    GSECHK(sorted.ensure(4 * tsize(ct_dbl) * slots));
+    GSECHK(perm.ensure(sizeof(int) * slots));
+    GSECHK(ovf.ensure(sizeof(int) * (size_t)(n > 0 ? n : 1)));
+    GSECHK(part.ensure(sizeof(double) * kRedBlocks * 8));
+    for (int k = 0; k < 2; k++) {
+      if (counts[k].bytes < sizeof(int) * (size_t)nb || !counts[k].p) {
+        GSECHK(counts[k].ensure(sizeof(int) * (size_t)nb));
+        clean[k] = 0;
+      }
+      if (!stats[k].p) {
+        GSECHK(stats[k].ensure(sizeof(BinStats)));
+        clean[k] = 0;
+      }
+    }
+    const size_t npts = (size_t)X * Y * Z;
+    if (grid64.bytes < npts * 8 || !grid64.p) {
+      GSECHK(grid64.ensure(npts * 8));
+      grid64_clean = false;
+    }
+    if (!grid64_clean) {
+      HIPCHK(launch_fill_u64(grid64.as<unsigned long long>(), npts, s));
+      grid64_clean = true;
+    }
+    const int c = cur ^ 1;
+    if (clean[c] < nb) HIPCHK(launch_bin_clear(counts[c].as<int>(), nb, stats[c].as<BinStats>(), s));
+    clean[c] = 0;
+    cur = c;
+    bin.n = n;
+    bin.cap_bin = cap_bin;
+    bin.xyzq = xyzq;
+    bin.counts = counts[c].as<int>();
+    bin.counts_other = counts[c ^ 1].as<int>();
+    bin.stats = stats[c].as<BinStats>();
+    bin.stats_other = stats[c ^ 1].as<BinStats>();
+    bin.ovf = ovf.as<int>();
+    bin.perm = perm.as<int>();
+    bin.sorted = sorted.p;
+    bin.images = ((2 * g.nx + 1 + X - 1) / X) * ((2 * g.ny + 1 + Y - 1) / Y) *
+                 ((2 * g.nz + 1 + Z - 1) / Z);
+    bin.pref_bound = std::pow(2.0 * M_PI * sigma * sigma, -1.5) *
+                     spread_weight_bound(g, sigma, normalize);
+    HIPCHK(launch_bin(ct_dbl, g, shape, bin, s));
     return GSE_SUCCESS;
   }
 
-  gse_status_t spread_binned(double sigma, bool normalize, void* rho, hipStream_t s) {
+  // the consumer kernels clear the other buffer
+  void consumed() { clean[cur ^ 1] = shape.nbins; }
+
+  gse_status_t spread_binned(double sigma, bool normalize, hipStream_t s) {
     HIPCHK(launch_spread(ct_dbl, geom, shape, sigma, normalize, bin,
                          grid64.as<unsigned long long>(), s));
-    HIPCHK(launch_fixed_to_grid(ct_dbl, grid64.as<unsigned long long>(), rho,
-                                (size_t)X * Y * Z, bin.fp, true, s));
+    consumed();
+    return GSE_SUCCESS;
+  }
+
+  gse_status_t convert(void* rho, hipStream_t s) {
+    HIPCHK(launch_fixed_to_grid(ct_dbl, grid64.as<unsigned long long>(), rho, (size_t)X * Y * Z,
+                                bin, true, s));
     return GSE_SUCCESS;
   }
 
   gse_status_t gather_binned(double sigma, double rcut, int flags, const void* phi,
                              GatherOut& o, double* virial9, hipStream_t s) {
+    const int n = bin.n;
     if (flags & GF_VIRIAL) {
-      GSECHK(vir6.ensure(sizeof(double) * 6 * (size_t)(nbound > 0 ? nbound : 1)));
+      GSECHK(vir6.ensure(sizeof(double) * 6 * (size_t)(n > 0 ? n : 1)));
       o.vir6 = vir6.as<double>();
-      o.vir_stride = nbound > 0 ? nbound : 1;
-      if (nbound > 0)  // invalid particles leave their slots untouched
-        HIPCHK(launch_fill(true, vir6.p, 6 * (size_t)nbound, s));
+      o.vir_stride = n > 0 ? n : 1;
+      if (n > 0)  // invalid particles leave their slots untouched
+        HIPCHK(launch_fill(true, vir6.p, 6 * (size_t)n, s));
     }
-    HIPCHK(launch_gather(ct_dbl, at_dbl, geom, shape, sigma, rcut, flags, bin, nbound, phi, o, s));
-    if ((flags & GF_VIRIAL) && virial9) {
-      HIPCHK(launch_sum_columns(vir6.as<double>(), nbound, 6, part.as<double>(), s));
+    HIPCHK(launch_gather(ct_dbl, at_dbl, geom, shape, sigma, rcut, flags, bin, phi, o, s));
+    consumed();
+    if ((flags & GF_VIRIAL) && virial9 && n > 0) {
+      HIPCHK(launch_sum_columns(vir6.as<double>(), n, 6, part.as<double>(), s));
       HIPCHK(launch_add_sym6(part.as<double>(), kRedBlocks, virial9, s));
     }
     return GSE_SUCCESS;
   }
 
   gse_status_t invalid_count(hipStream_t s, int* out) {
-    if (!fp.p) {
+    if (!stats[cur].p) {
       *out = 0;
       return GSE_SUCCESS;
     }
-    FixedParams h;
-    HIPCHK(hipMemcpyAsync(&h, fp.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    BinStats h;
+    HIPCHK(hipMemcpyAsync(&h, stats[cur].p, sizeof(h), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     *out = h.invalid;
     return GSE_SUCCESS;
@@ -316,17 +349,30 @@ struct KSolver {
     return GSE_SUCCESS;
   }
   // phi = c2r(G * r2c(rho)); energy/virial (device) when requested
+  // energy / virial: device outputs (may be null); the k-space virial is
+  // written into virial[0..8].  With `defer` set, the final reduction is not
+  // launched but described in *defer (to run in the gather's spare workgroup).
   gse_status_t solve(int kind, double sigma, double bx, double by, double bz, const void* rho,
-                     void* phi, double* eout, bool vir, double ccelec, hipStream_t s,
-                     gse_gsek_s* marker = nullptr) {
+                     void* phi, double* energy, double* virial, double ccelec, hipStream_t s,
+                     gse_gsek_s* marker = nullptr, GatherOut* defer = nullptr) {
     GSECHK(ensure_table(kind, sigma, bx, by, bz, s));
     GSECHK(fft.exec(true, rho, spec.p, s));
     if (marker) GSECHK(stage_mark(marker));
-    HIPCHK(launch_kpass(dbl, kind, X, Y, Z, bx, by, bz, sigma, spec.p, table.p, vir && eout,
-                        part.as<double>(), s));
-    if (eout) {
+    HIPCHK(launch_kpass(dbl, kind, X, Y, Z, bx, by, bz, sigma, spec.p, table.p,
+                        virial != nullptr, part.as<double>(), s));
+    if (energy || virial) {
       const double vcell = (bx / X) * (by / Y) * (bz / Z);
-      HIPCHK(launch_kpass_finalize(part.as<double>(), 0.5 * ccelec * vcell, eout, vir, s));
+      const int ncomp = virial ? 7 : 1;
+      if (defer) {
+        defer->fin_partials = part.as<double>();
+        defer->fin_ncomp = ncomp;
+        defer->fin_pref = 0.5 * ccelec * vcell;
+        defer->fin_energy = energy;
+        defer->fin_virial = virial;
+      } else {
+        HIPCHK(launch_kpass_finalize(part.as<double>(), ncomp, 0.5 * ccelec * vcell, energy,
+                                     virial, s));
+      }
     }
     if (marker) GSECHK(stage_mark(marker));
     GSECHK(fft.exec(false, spec.p, phi, s));
@@ -437,10 +483,44 @@ const char* gse_last_error(void) { return g_err.c_str(); }
 int gse_version(void) { return 10000; }
 
 gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes) {
-  if (bin_edge < 0 || lds_budget_bytes < 0 || bin_edge > 64 || lds_budget_bytes > 160 * 1024)
-    return fail(GSE_ERROR_INVALID_VALUE, "bin_edge in [0,64], lds budget in [0,163840]");
-  g_bin_edge = bin_edge ? bin_edge : 8;
-  g_lds_budget = lds_budget_bytes ? (size_t)lds_budget_bytes : 64 * 1024;
+  if (bin_edge < 0 || lds_budget_bytes < 0 || bin_edge > 64 || lds_budget_bytes > 144 * 1024)
+    return fail(GSE_ERROR_INVALID_VALUE, "bin_edge in [0,64], lds budget in [0,147456]");
+  g_tuning.bin_edge = bin_edge ? bin_edge : 8;
+  g_tuning.brick_budget = lds_budget_bytes ? lds_budget_bytes : 64 * 1024;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_set_tuning_param(const char* name, int value) {
+  if (!name) return fail(GSE_ERROR_INVALID_VALUE, "name is NULL");
+  const std::string n(name);
+  Tuning& t = g_tuning;
+  if (n == "bin_edge" && value >= 1 && value <= 64) t.bin_edge = value;
+  else if (n == "brick_budget" && value >= 4096 && value <= 144 * 1024) t.brick_budget = value;
+  else if (n == "spread_block" && (value == 0 || value == 256 || value == 512 || value == 1024))
+    t.spread_block = value;
+  else if (n == "gather_block" && (value == 0 || value == 256 || value == 512)) t.gather_block = value;
+  else if (n == "spread_stage" && value >= 1024 && value <= 96 * 1024) t.spread_stage = value;
+  else if (n == "gather_stage" && value >= 1024 && value <= 96 * 1024) t.gather_stage = value;
+  else if (n == "spread_lds32" && (value == 0 || value == 1)) t.spread_lds32 = value;
+  else if (n == "spread_cols" && (value == 0 || value == 1)) t.spread_cols = value;
+  else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter or value out of range: %s=%d",
+                   name, value);
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_get_tuning_param(const char* name, int* value) {
+  if (!name || !value) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  const std::string n(name);
+  const Tuning& t = g_tuning;
+  if (n == "bin_edge") *value = t.bin_edge;
+  else if (n == "brick_budget") *value = t.brick_budget;
+  else if (n == "spread_block") *value = t.spread_block;
+  else if (n == "gather_block") *value = t.gather_block;
+  else if (n == "spread_stage") *value = t.spread_stage;
+  else if (n == "gather_stage") *value = t.gather_stage;
+  else if (n == "spread_lds32") *value = t.spread_lds32;
+  else if (n == "spread_cols") *value = t.spread_cols;
+  else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter: %s", name);
   return GSE_SUCCESS;
 }
 
@@ -514,7 +594,8 @@ gse_status_t gse_gauss_charge_spread_charge_to_grid(gse_gauss_charge_t h, double
                                                     int normalize) {
   if (!h || !rho) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
   GSECHK(h->eng.bin_particles(n, xyzq, bx, by, bz, sigma, rcut, normalize != 0, h->stream));
-  return h->eng.spread_binned(sigma, normalize != 0, rho, h->stream);
+  GSECHK(h->eng.spread_binned(sigma, normalize != 0, h->stream));
+  return h->eng.convert(rho, h->stream);
 }
 
 gse_status_t gse_gauss_charge_spread_charge_on_grid(gse_gauss_charge_t h, double sigma,
@@ -696,7 +777,7 @@ gse_status_t gse_fft_solver_run(gse_fft_solver_t h, void* phi, const void* rho, 
   GSECHK(check_box(bx, by, bz));
   if (!(sigma >= 0) || !std::isfinite(sigma))
     return fail(GSE_ERROR_INVALID_VALUE, "sigma must be >= 0");
-  return h->ks.solve(h->kind, sigma, bx, by, bz, rho, phi, nullptr, false, 1.0, h->stream);
+  return h->ks.solve(h->kind, sigma, bx, by, bz, rho, phi, nullptr, nullptr, 1.0, h->stream);
 }
 
 gse_status_t gse_fft_solver_r2c(gse_fft_solver_t h, const void* rho, void* rhoK) {
@@ -822,7 +903,8 @@ gse_status_t gse_gsek_spread_charge1(gse_gsek_t h, int n, const void* xyzq) {
   if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
   GSECHK(h->eng.bin_particles(n, xyzq, h->bx, h->by, h->bz, h->sigma1, h->rcut1(), false,
                               h->stream));
-  return h->eng.spread_binned(h->sigma1, false, h->rhoS.p, h->stream);
+  GSECHK(h->eng.spread_binned(h->sigma1, false, h->stream));
+  return h->eng.convert(h->rhoS.p, h->stream);
 }
 
 gse_status_t gse_gsek_spread_charge2(gse_gsek_t h, void* rhoOut) {
@@ -837,7 +919,7 @@ gse_status_t gse_gsek_spread_charge2(gse_gsek_t h, void* rhoOut) {
 gse_status_t gse_gsek_solve_poisson(gse_gsek_t h) {
   if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
   return h->ks.solve(GSE_GREENS_GSE, h->sigma2, h->bx, h->by, h->bz, h->rhoS.p, h->phiM.p,
-                     nullptr, false, h->ccelec, h->stream);
+                     nullptr, nullptr, h->ccelec, h->stream);
 }
 
 gse_status_t gse_gsek_calculate_energy(gse_gsek_t h, double* energy) {
@@ -907,16 +989,14 @@ static gse_status_t gsek_run(gse_gsek_t h, int n, const void* xyzq, int mode, in
   GSECHK(mark(h));
   GSECHK(h->eng.bin_particles(n, xyzq, h->bx, h->by, h->bz, h->sigma1, rc1, false, s));
   GSECHK(mark(h));
-  HIPCHK(launch_spread(h->ct_dbl, h->eng.geom, h->eng.shape, h->sigma1, false, h->eng.bin,
-                       h->eng.grid64.as<unsigned long long>(), s));
+  GSECHK(h->eng.spread_binned(h->sigma1, false, s));
   GSECHK(mark(h));
-  HIPCHK(launch_fixed_to_grid(h->ct_dbl, h->eng.grid64.as<unsigned long long>(), h->rhoS.p,
-                              (size_t)h->X * h->Y * h->Z, h->eng.bin.fp, true, s));
+  GSECHK(h->eng.convert(h->rhoS.p, s));
   GSECHK(mark(h));
-  double* eout = h->dout.as<double>();
+  double* eout = energy ? energy : h->dout.as<double>();
   if (mode == 0) {
     GSECHK(h->ks.solve(GSE_GREENS_GSE, h->sigma2, h->bx, h->by, h->bz, h->rhoS.p, h->phiM.p, eout,
-                       virial != nullptr, h->ccelec, s, h->profiling ? h : nullptr));
+                       virial, h->ccelec, s, h->profiling ? h : nullptr, flags ? &o : nullptr));
   } else {
     int R[3];
     GSECHK(make_ongrid_taps(h->ct_dbl, h->X, h->Y, h->Z, h->sigma2, h->rcut2(), h->bx, h->by,
@@ -925,16 +1005,13 @@ static gse_status_t gsek_run(gse_gsek_t h, int n, const void* xyzq, int mode, in
     GSECHK(conv3(h->ct_dbl, h->X, h->Y, h->Z, R, h->taps, h->rhoS.p, h->rhoR.p, h->tmp,
                  h->tmp2, s));
     GSECHK(h->ks.solve(GSE_GREENS_GSE, 0.0, h->bx, h->by, h->bz, h->rhoR.p, h->phiM.p, nullptr,
-                       false, h->ccelec, s, h->profiling ? h : nullptr));
+                       nullptr, h->ccelec, s, h->profiling ? h : nullptr));
     const double vcell = (h->bx / h->X) * (h->by / h->Y) * (h->bz / h->Z);
     HIPCHK(launch_dot(h->ct_dbl, h->rhoS.p, h->phiM.p, (size_t)h->X * h->Y * h->Z,
                       h->part.as<double>(), s));
     HIPCHK(launch_reduce_partials(h->part.as<double>(), kRedBlocks, 1, 1,
                                   0.5 * h->ccelec * vcell, eout, s));
   }
-  if (energy) HIPCHK(hipMemcpyAsync(energy, eout, sizeof(double), hipMemcpyDeviceToDevice, s));
-  if (virial && mode == 0)
-    HIPCHK(hipMemcpyAsync(virial, eout + 1, 9 * sizeof(double), hipMemcpyDeviceToDevice, s));
   if (flags) GSECHK(h->eng.gather_binned(h->sigma1, rc1, flags, h->phiM.p, o,
                                          (virial && mode == 0) ? virial : nullptr, s));
   GSECHK(mark(h));

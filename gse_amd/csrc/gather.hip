@@ -5,11 +5,12 @@
 // moments) in one pass over the cube.
 //
 // One workgroup per bin (the binning of the spread).  The bin's potential
-// brick of (B+2n)^3 values is staged once into LDS with wrapped, x-coalesced
-// loads; each wave then takes one particle at a time: lanes compute the 3(2n+1)
-// one-dimensional offsets and Gaussian factors, lanes own (tx,ty) columns of
-// the cube and walk tz with the z factors held wave-uniform, and the partial
-// sums are reduced across the wave with a fixed butterfly.
+// brick of (B+2n)^3 values and its particles are staged into LDS with
+// coalesced loads; one wave per particle computes the 3(2n+1) offsets and
+// Gaussian factors; then one wave per particle walks the cube, lanes on
+// (tx,ty) columns walking tz with the z factors in registers, and the partial
+// sums are reduced across the wave with a fixed butterfly.  kOverflowWG extra
+// workgroups gather the overflow particles from global memory.
 #include <math.h>
 
 #include "gse_device.h"
@@ -36,73 +37,212 @@ struct GatherArgs {
   double rc2;
   double prefF;  // (2 pi)^-3/2 h^3 sigma^-5   (CpuGaussCharge.cpp:128)
   double prefP;  // (2 pi sigma^2)^-3/2 h^3
+  int cap;       // particles staged per chunk
+  int cap_bin;   // slots per bin
+  int nwg_bins;
 };
 
+__device__ __forceinline__ float fmac(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmac(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Writes one particle's outputs from its (wave-reduced) sums.
+template <typename AT, int FL>
+__device__ __forceinline__ void gather_store(const GatherArgs& a, const GatherOut& o, int oi,
+                                             double qv, double ds, const AT (&f)[13]) {
+  const double fq = qv * a.prefF * o.fscale * ds;
+  const double vq = qv * a.prefF * o.vscale;
+  if ((FL & GF_SPHERE) && o.fx) {
+    ((AT*)o.fx)[oi] = (AT)(fq * (double)f[0]);
+    ((AT*)o.fy)[oi] = (AT)(fq * (double)f[1]);
+    ((AT*)o.fz)[oi] = (AT)(fq * (double)f[2]);
+  }
+  if ((FL & GF_CUBE) && o.cx) {
+    ((AT*)o.cx)[oi] = (AT)(fq * (double)f[3]);
+    ((AT*)o.cy)[oi] = (AT)(fq * (double)f[4]);
+    ((AT*)o.cz)[oi] = (AT)(fq * (double)f[5]);
+  }
+  if ((FL & GF_POT) && o.pot) ((AT*)o.pot)[oi] = (AT)(a.prefP * o.pscale * ds * (double)f[6]);
+  if ((FL & GF_VIRIAL) && o.vir6) {
+    const size_t st = (size_t)o.vir_stride;
+    for (int c = 0; c < 6; c++) o.vir6[c * st + oi] = vq * (double)f[7 + c];
+  }
+}
+
+// One particle gathered straight from global memory, one wave: used for the
+// overflow list and for supports too wide for the LDS path.
+template <typename CT, typename AT, int FL>
+__device__ void gather_one_global(const GatherArgs& a, const GatherOut& o, const V4<CT>& p,
+                                  int oi, double ds, const CT* __restrict__ phi) {
+  const Geom& g = a.g;
+  const int lane = threadIdx.x & 63;
+  const int NX = 2 * g.nx + 1, NY = 2 * g.ny + 1, NZ = 2 * g.nz + 1;
+  const CT inv2s2 = (CT)a.inv2s2;
+  const CT rc2 = (CT)a.rc2;
+  const double x = (double)p.x, y = (double)p.y, z = (double)p.z;
+  const int cx = (int)round(x * g.ihx) - g.nx;
+  const int cy = (int)round(y * g.ihy) - g.ny;
+  const int cz = (int)round(z * g.ihz) - g.nz;
+  AT f[13];
+  for (int c = 0; c < 13; c++) f[c] = 0;
+  for (int idx = lane; idx < NX * NY * NZ; idx += 64) {
+    const int tx = idx % NX, t2 = idx / NX, ty = t2 % NY, tz = t2 / NY;
+    const CT dx = (CT)sub_rn(x, mul_rn((double)(cx + tx), g.hx));
+    const CT dy = (CT)sub_rn(y, mul_rn((double)(cy + ty), g.hy));
+    const CT dz = (CT)sub_rn(z, mul_rn((double)(cz + tz), g.hz));
+    const CT r2 = add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz));
+    const int gx = wrapi(cx + tx, g.X), gy = wrapi(cy + ty, g.Y), gz = wrapi(cz + tz, g.Z);
+    const CT ph = phi[gx + (size_t)g.X * (gy + (size_t)g.Y * gz)];
+    const CT wt = gexp(mul_rn(mul_rn(-dx, dx), inv2s2)) * gexp(mul_rn(mul_rn(-dy, dy), inv2s2)) *
+                  gexp(mul_rn(mul_rn(-dz, dz), inv2s2));
+    const AT s = (AT)(wt * ph);
+    if (r2 < rc2) {
+      f[0] += s * (AT)dx;
+      f[1] += s * (AT)dy;
+      f[2] += s * (AT)dz;
+    }
+    f[3] += s * (AT)dx;
+    f[4] += s * (AT)dy;
+    f[5] += s * (AT)dz;
+    f[6] += s;
+    f[7] += s * (AT)dx * (AT)dx;
+    f[8] += s * (AT)dx * (AT)dy;
+    f[9] += s * (AT)dx * (AT)dz;
+    f[10] += s * (AT)dy * (AT)dy;
+    f[11] += s * (AT)dy * (AT)dz;
+    f[12] += s * (AT)dz * (AT)dz;
+  }
+  for (int c = 0; c < 13; c++) f[c] = wave_sum(f[c]);
+  if (lane == 0) gather_store<AT, FL>(a, o, oi, (double)p.w, ds, f);
+}
+
+// LDS layout of k_gather: staged 4-vectors (32-byte aligned) at 0, then the
+// factor pairs (w, d) of type CT, q (double), brick offsets and original
+// indices (int), then the potential brick (CT).
+struct GatherLds {
+  size_t oX, oW, oQ, oB, oP, oBrick, total;
+};
+__host__ __device__ inline GatherLds gather_lds(int vol, int cap, int nw, int ct_bytes) {
+  GatherLds l;
+  size_t o = 0;
+  l.oX = o;
+  o += (size_t)cap * 4 * ct_bytes;
+  l.oW = o;
+  o += (size_t)cap * nw * 2 * ct_bytes;
+  o = (o + 7) & ~(size_t)7;
+  l.oQ = o;
+  o += (size_t)cap * 8;
+  l.oB = o;
+  o += (size_t)cap * 4;
+  l.oP = o;
+  o += (size_t)cap * 4;
+  o = (o + 15) & ~(size_t)15;
+  l.oBrick = o;
+  o += (size_t)vol * ct_bytes;
+  l.total = o;
+  return l;
+}
+
 // FL: GatherFlags bit mask; NZC: 2nz+1 known at compile time (0 = runtime)
-template <typename CT, typename AT, int NZC, int FL>
-__global__ void __launch_bounds__(256)
-    k_gather(GatherArgs a, const V4<CT>* __restrict__ sorted, const int* __restrict__ perm,
-             const int* __restrict__ starts, const CT* __restrict__ phi, GatherOut o) {
-  extern __shared__ char smem_raw[];
-  CT* brick = (CT*)smem_raw;
+template <typename CT, typename AT, int NZC, int FL, int BS>
+__global__ void __launch_bounds__(BS)
+    k_gather(GatherArgs a, const V4<CT>* __restrict__ xyzq, const V4<CT>* __restrict__ sorted,
+             const int* __restrict__ perm, const int* __restrict__ counts,
+             int* __restrict__ counts_other, const BinStats* __restrict__ stats,
+             BinStats* __restrict__ stats_other, const int* __restrict__ ovf,
+             const CT* __restrict__ phi, GatherOut o) {
+  extern __shared__ __align__(32) double smem_raw[];
   const Geom& g = a.g;
   const BinShape& b = a.b;
-  const int bin = xcd_remap(blockIdx.x, gridDim.x);
-  if (bin >= b.nbins) return;
-  const int s0 = starts[bin], s1 = starts[bin + 1];
-  if (s0 == s1) return;
-  const int bxi = bin % b.nbx, byi = (bin / b.nbx) % b.nby, bzi = bin / (b.nbx * b.nby);
-  const int Lx = b.Lx, Ly = b.Ly, LxLy = b.Lx * b.Ly;
-  const int vol = LxLy * b.Lz;
-  {
-    const int gx0 = bxi * b.Bx, gy0 = byi * b.By, gz0 = bzi * b.Bz;
-    for (int j = threadIdx.x; j < vol; j += blockDim.x) {
-      const int lx = j % Lx;
-      const int t = j / Lx;
-      const int ly = t % Ly;
-      const int lz = t / Ly;
-      int gx = gx0 + lx, gy = gy0 + ly, gz = gz0 + lz;
-      while (gx >= g.X) gx -= g.X;
-      while (gy >= g.Y) gy -= g.Y;
-      while (gz >= g.Z) gz -= g.Z;
-      brick[j] = phi[gx + (size_t)g.X * (gy + (size_t)g.Y * gz)];
-    }
+  const double ds = o.dscale ? *o.dscale : 1.0;
+  if ((int)blockIdx.x == a.nwg_bins + kOverflowWG) {  // spare workgroup: k-pass finalize
+    kpass_finalize_block(o.fin_partials, kKpassBlocks, o.fin_ncomp, o.fin_pref, o.fin_energy,
+                         o.fin_virial);
+    return;
   }
-  __syncthreads();
+  if ((int)blockIdx.x >= a.nwg_bins) {  // overflow particles from global memory
+    if (blockIdx.x == (unsigned)a.nwg_bins && threadIdx.x == 0) {
+      stats_other->qmax_bits = 0ull;
+      stats_other->invalid = 0;
+      stats_other->overflow = 0;
+    }
+    const int novf = stats->overflow;
+    const int gw = (blockIdx.x - a.nwg_bins) * (BS / 64) + (threadIdx.x >> 6);
+    for (int k = __builtin_amdgcn_readfirstlane(gw); k < novf; k += kOverflowWG * (BS / 64)) {
+      const int oi = ovf[k];
+      gather_one_global<CT, AT, FL>(a, o, xyzq[oi], oi, ds, phi);
+    }
+    return;
+  }
+  const int bin = xcd_remap(blockIdx.x, a.nwg_bins);
+  if (bin >= b.nbins) return;
+  const int cnt = min(counts[bin], a.cap_bin);
+  if (threadIdx.x == 0) counts_other[bin] = 0;
+  if (cnt == 0) return;
+  const int s0 = bin * a.cap_bin, s1 = s0 + cnt;
+  const int bxi = bin % b.nbx, byi = (bin / b.nbx) % b.nby, bzi = bin / (b.nbx * b.nby);
+  const int Lx = b.Lx, Ly = b.Ly, Lz = b.Lz, LxLy = b.Lx * b.Ly;
+  const int NX = 2 * g.nx + 1, NY = 2 * g.ny + 1;
+  const int NZ = NZC > 0 ? NZC : 2 * g.nz + 1;
+  const int NW = NX + NY + NZ;
+  const int cap = a.cap;
+  const GatherLds lay = gather_lds(LxLy * Lz, cap, NW, sizeof(CT));
+  char* lds = (char*)smem_raw;
+  V4<CT>* sX = (V4<CT>*)(lds + lay.oX);
+  CT* sW = (CT*)(lds + lay.oW);  // [cap][NW][2] (w, d)
+  double* sQ = (double*)(lds + lay.oQ);
+  int* sB = (int*)(lds + lay.oB);
+  int* sP = (int*)(lds + lay.oP);
+  CT* brick = (CT*)(lds + lay.oBrick);
 
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int nw = blockDim.x >> 6;
-  const int NX = 2 * g.nx + 1, NY = 2 * g.ny + 1;
-  const int NZ = NZC > 0 ? NZC : 2 * g.nz + 1;
+  constexpr int nwv = BS / 64;
+  {  // stage the potential brick: rows of Lx lanes, no per-element division
+    const int rows_per = 64 / Lx;
+    const int rl = lane / Lx;
+    const int lx = lane - rl * Lx;
+    int gx = bxi * b.Bx + lx;
+    while (gx >= g.X) gx -= g.X;
+    const int rows = Ly * Lz;
+    const int rstep = nwv * rows_per;
+    int r = wid * rows_per + rl;
+    int ly = r % Ly, lz = r / Ly;
+    for (; r < rows; r += rstep) {
+      if (rl < rows_per) {
+        int gy = byi * b.By + ly, gz = bzi * b.Bz + lz;
+        while (gy >= g.Y) gy -= g.Y;
+        while (gz >= g.Z) gz -= g.Z;
+        brick[lx + Lx * (ly + Ly * lz)] = phi[gx + (size_t)g.X * (gy + (size_t)g.Y * gz)];
+      }
+      ly += rstep;
+      while (ly >= Ly) {
+        ly -= Ly;
+        lz++;
+      }
+    }
+  }
+
   const int NXY = NX * NY;
   const int KI = (NXY + 63) >> 6;
   const CT rc2 = (CT)a.rc2;
   const CT inv2s2 = (CT)a.inv2s2;
-
-  // lane roles for the 1D factors
-  int axis, jj;
+  double hsel, ihsel;
+  int nsel, jj, axis;
   if (lane < NX) {
-    axis = 0; jj = lane;
+    axis = 0; jj = lane; hsel = g.hx; ihsel = g.ihx; nsel = g.nx;
   } else if (lane < NX + NY) {
-    axis = 1; jj = lane - NX;
-  } else if (lane < NX + NY + NZ) {
-    axis = 2; jj = lane - NX - NY;
+    axis = 1; jj = lane - NX; hsel = g.hy; ihsel = g.ihy; nsel = g.ny;
   } else {
-    axis = -1; jj = 0;
+    axis = lane < NW ? 2 : -1; jj = lane - NX - NY; hsel = g.hz; ihsel = g.ihz; nsel = g.nz;
   }
-  const double hsel = axis == 0 ? g.hx : axis == 1 ? g.hy : g.hz;
-  const double ihsel = axis == 0 ? g.ihx : axis == 1 ? g.ihy : g.ihz;
-  const int nsel = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
-
   int k_tx[kMaxK], k_ty[kMaxK], k_off[kMaxK];
   bool k_ok[kMaxK];
 #pragma unroll
   for (int k = 0; k < kMaxK; k++) {
     const int idx = lane + 64 * k;
     k_ok[k] = (k < KI) && idx < NXY;
-    k_tx[k] = k_ok[k] ? idx % NX : 0;
     k_ty[k] = k_ok[k] ? idx / NX : 0;
+    k_tx[k] = k_ok[k] ? idx - k_ty[k] * NX : 0;
     k_off[k] = k_tx[k] + Lx * k_ty[k];
   }
 
@@ -114,47 +254,64 @@ __global__ void __launch_bounds__(256)
   constexpr bool NEED_B1 = CUB || VIR;
   constexpr int NZA = NZC > 0 ? NZC : 1;
 
-  for (int i = s0 + wid; i < s1; i += nw) {
-    const V4<CT> p = sorted[i];
-    const double x = (double)p.x, y = (double)p.y, z = (double)p.z;
-    const double ps = axis == 0 ? x : axis == 1 ? y : z;
-    // centre cell (int)round(x*inv_h) (CpuGaussCharge.cpp:139-141); this
-    // lane's grid index t = c - n + j and offset d = x - t*h (:146-155)
-    const int c = (int)round(ps * ihsel);
-    const double dd = sub_rn(ps, mul_rn((double)(c - nsel + jj), hsel));
-    const CT d = (CT)dd;
-    const CT w = axis >= 0 ? gexp(mul_rn(mul_rn(-d, d), inv2s2)) : (CT)0;
-
-    const int cxo = (int)round(x * g.ihx) - g.nx;
-    const int cyo = (int)round(y * g.ihy) - g.ny;
-    const int czo = (int)round(z * g.ihz) - g.nz;
-    const int ox = wrapi(cxo, g.X) - bxi * b.Bx;
-    const int oy = wrapi(cyo, g.Y) - byi * b.By;
-    const int oz = wrapi(czo, g.Z) - bzi * b.Bz;
-    const int base = ox + Lx * (oy + Ly * oz);
-
-    CT wzr[NZA], dzr[NZA];
-    if (NZC > 0) {
-#pragma unroll
-      for (int t = 0; t < NZA; t++) {
-        wzr[t] = readlane(w, NX + NY + t);
-        dzr[t] = readlane(d, NX + NY + t);
+  for (int c0 = s0; c0 < s1; c0 += cap) {
+    const int na = min(cap, s1 - c0);
+    // phase 0: particles and original indices into LDS, one coalesced pass
+    for (int t = threadIdx.x; t < na; t += BS) {
+      sX[t] = sorted[c0 + t];
+      sP[t] = perm[c0 + t];
+    }
+    __syncthreads();
+    // phase 1: one wave per particle; lane j holds the reference gather's
+    // offset d = x - t*h (CpuGaussCharge.cpp:146-155, t = round(x/h) - n + j)
+    // and factor exp(-d^2/2s^2)
+    for (int ai = wid; ai < na; ai += nwv) {
+      const V4<CT> p = sX[ai];
+      const double ps = axis == 0 ? (double)p.x : axis == 1 ? (double)p.y : (double)p.z;
+      if (axis >= 0) {
+        const int c = (int)round(ps * ihsel);
+        const CT d = (CT)sub_rn(ps, mul_rn((double)(c - nsel + jj), hsel));
+        sW[2 * ((size_t)ai * NW + lane)] = gexp(mul_rn(mul_rn(-d, d), inv2s2));
+        sW[2 * ((size_t)ai * NW + lane) + 1] = d;
+      }
+      if (lane == 0) {
+        const double x = (double)p.x, y = (double)p.y, z = (double)p.z;
+        const int ox = wrapi((int)round(x * g.ihx) - g.nx, g.X) - bxi * b.Bx;
+        const int oy = wrapi((int)round(y * g.ihy) - g.ny, g.Y) - byi * b.By;
+        const int oz = wrapi((int)round(z * g.ihz) - g.nz, g.Z) - bzi * b.Bz;
+        sB[ai] = ox + Lx * (oy + Ly * oz);
+        sQ[ai] = (double)p.w;
       }
     }
-
-    AT fsx = 0, fsy = 0, fsz = 0, fcx = 0, fcy = 0, fcz = 0, pp = 0;
-    AT vxx = 0, vxy = 0, vxz = 0, vyy = 0, vyz = 0, vzz = 0;
+    __syncthreads();
+    // phase 2: one wave per particle
+    for (int ai = wid; ai < na; ai += nwv) {
+      const CT* w = sW + (size_t)ai * NW * 2;
+      const int base = sB[ai];
+      CT wzr[NZA], dzr[NZA], dz2[NZA];
+      if (NZC > 0) {
 #pragma unroll
-    for (int k = 0; k < kMaxK; k++) {
-      if (k >= KI) break;
-      const CT wx = shfl(w, k_tx[k]);
-      const CT dx = shfl(d, k_tx[k]);
-      const CT wy = shfl(w, NX + k_ty[k]);
-      const CT dy = shfl(d, NX + k_ty[k]);
-      const CT dxy2 = add_rn(mul_rn(dx, dx), mul_rn(dy, dy));
-      const int a0 = base + k_off[k];
-      AT B0 = 0, B1 = 0, B2 = 0, A0 = 0, A1 = 0;
-      if (k_ok[k]) {
+        for (int t = 0; t < NZA; t++) {
+          wzr[t] = w[2 * (NX + NY + t)];
+          dzr[t] = w[2 * (NX + NY + t) + 1];
+          dz2[t] = mul_rn(dzr[t], dzr[t]);
+        }
+      }
+      AT f[13];
+#pragma unroll
+      for (int c = 0; c < 13; c++) f[c] = 0;
+#pragma unroll
+      for (int k = 0; k < kMaxK; k++) {
+        if (k >= KI) break;
+        if (!k_ok[k]) continue;
+        const CT wx = w[2 * k_tx[k]];
+        const CT dx = w[2 * k_tx[k] + 1];
+        const CT wy = w[2 * (NX + k_ty[k])];
+        const CT dy = w[2 * (NX + k_ty[k]) + 1];
+        // r2 = (dx*dx + dy*dy) + dz*dz, each product rounded (:157)
+        const CT dxy2 = add_rn(mul_rn(dx, dx), mul_rn(dy, dy));
+        const int a0 = base + k_off[k];
+        AT B0 = 0, B1 = 0, B2 = 0, A0 = 0, A1 = 0;
         if (NZC > 0) {
 #pragma unroll
           for (int t = 0; t < NZA; t++) {
@@ -162,225 +319,155 @@ __global__ void __launch_bounds__(256)
             const AT wph = (AT)(wzr[t] * ph);
             const AT dzt = (AT)dzr[t];
             if (NEED_B0) B0 += wph;
-            if (NEED_B1) B1 += wph * dzt;
-            if (VIR) B2 += wph * dzt * dzt;
+            if (NEED_B1) B1 = fmac(wph, dzt, B1);
+            if (VIR) B2 = fmac(wph, (AT)dz2[t], B2);
             if (SPH) {
-              const CT r2 = add_rn(dxy2, mul_rn(dzr[t], dzr[t]));
-              if (r2 < rc2) {
-                A0 += wph;
-                A1 += wph * dzt;
-              }
+              const AT ws = add_rn(dxy2, dz2[t]) < rc2 ? wph : (AT)0;
+              A0 += ws;
+              A1 = fmac(ws, dzt, A1);
             }
           }
         } else {
           for (int t = 0; t < NZ; t++) {
-            const CT wzt = readlane(w, NX + NY + t);
-            const CT dzc = readlane(d, NX + NY + t);
+            const CT wzt = w[2 * (NX + NY + t)];
+            const CT dzc = w[2 * (NX + NY + t) + 1];
+            const CT dzc2 = mul_rn(dzc, dzc);
             const CT ph = brick[a0 + t * LxLy];
             const AT wph = (AT)(wzt * ph);
             const AT dzt = (AT)dzc;
             if (NEED_B0) B0 += wph;
-            if (NEED_B1) B1 += wph * dzt;
-            if (VIR) B2 += wph * dzt * dzt;
+            if (NEED_B1) B1 = fmac(wph, dzt, B1);
+            if (VIR) B2 = fmac(wph, (AT)dzc2, B2);
             if (SPH) {
-              const CT r2 = add_rn(dxy2, mul_rn(dzc, dzc));
-              if (r2 < rc2) {
-                A0 += wph;
-                A1 += wph * dzt;
-              }
+              const AT ws = add_rn(dxy2, dzc2) < rc2 ? wph : (AT)0;
+              A0 += ws;
+              A1 = fmac(ws, dzt, A1);
             }
           }
         }
         const AT wxy = (AT)wx * (AT)wy;
         const AT ax = (AT)dx, ay = (AT)dy;
         if (SPH) {
-          fsx += wxy * ax * A0;
-          fsy += wxy * ay * A0;
-          fsz += wxy * A1;
+          const AT t0 = wxy * A0;
+          f[0] = fmac(t0, ax, f[0]);
+          f[1] = fmac(t0, ay, f[1]);
+          f[2] = fmac(wxy, A1, f[2]);
         }
-        if (CUB) {
-          fcx += wxy * ax * B0;
-          fcy += wxy * ay * B0;
-          fcz += wxy * B1;
+        if (NEED_B0) {
+          const AT t0 = wxy * B0;
+          if (CUB) {
+            f[3] = fmac(t0, ax, f[3]);
+            f[4] = fmac(t0, ay, f[4]);
+            f[5] = fmac(wxy, B1, f[5]);
+          }
+          if (POT) f[6] += t0;
+          if (VIR) {
+            f[7] = fmac(t0 * ax, ax, f[7]);
+            f[8] = fmac(t0 * ax, ay, f[8]);
+            f[9] = fmac(wxy * ax, B1, f[9]);
+            f[10] = fmac(t0 * ay, ay, f[10]);
+            f[11] = fmac(wxy * ay, B1, f[11]);
+            f[12] = fmac(wxy, B2, f[12]);
+          }
         }
-        if (POT) pp += wxy * B0;
-        if (VIR) {
-          vxx += wxy * ax * ax * B0;
-          vxy += wxy * ax * ay * B0;
-          vxz += wxy * ax * B1;
-          vyy += wxy * ay * ay * B0;
-          vyz += wxy * ay * B1;
-          vzz += wxy * B2;
-        }
       }
-    }
-    if (SPH) {
-      fsx = wave_sum(fsx);
-      fsy = wave_sum(fsy);
-      fsz = wave_sum(fsz);
-    }
-    if (CUB) {
-      fcx = wave_sum(fcx);
-      fcy = wave_sum(fcy);
-      fcz = wave_sum(fcz);
-    }
-    if (POT) pp = wave_sum(pp);
-    if (VIR) {
-      vxx = wave_sum(vxx);
-      vxy = wave_sum(vxy);
-      vxz = wave_sum(vxz);
-      vyy = wave_sum(vyy);
-      vyz = wave_sum(vyz);
-      vzz = wave_sum(vzz);
-    }
-    if (lane == 0) {
-      const int oi = perm[i];
-      const double ds = o.dscale ? *o.dscale : 1.0;
-      const double fq = (double)p.w * a.prefF * o.fscale * ds;
-      const double vq = (double)p.w * a.prefF * o.vscale;
-      AT* const ofx = (AT*)o.fx;
-      AT* const ofy = (AT*)o.fy;
-      AT* const ofz = (AT*)o.fz;
-      if (SPH && ofx) {
-        ofx[oi] = (AT)(fq * (double)fsx);
-        ofy[oi] = (AT)(fq * (double)fsy);
-        ofz[oi] = (AT)(fq * (double)fsz);
+#pragma unroll
+      for (int c = 0; c < 13; c++) {
+        const bool used = (c < 3 && SPH) || (c >= 3 && c < 6 && CUB) || (c == 6 && POT) ||
+                          (c >= 7 && VIR);
+        if (used) f[c] = wave_sum(f[c]);
       }
-      if (CUB && o.cx) {
-        ((AT*)o.cx)[oi] = (AT)(fq * (double)fcx);
-        ((AT*)o.cy)[oi] = (AT)(fq * (double)fcy);
-        ((AT*)o.cz)[oi] = (AT)(fq * (double)fcz);
-      }
-      if (POT && o.pot) ((AT*)o.pot)[oi] = (AT)(a.prefP * o.pscale * ds * (double)pp);
-      if (VIR && o.vir6) {
-        double* v = o.vir6;
-        const size_t stride = (size_t)o.vir_stride;
-        v[0 * stride + oi] = vq * (double)vxx;
-        v[1 * stride + oi] = vq * (double)vxy;
-        v[2 * stride + oi] = vq * (double)vxz;
-        v[3 * stride + oi] = vq * (double)vyy;
-        v[4 * stride + oi] = vq * (double)vyz;
-        v[5 * stride + oi] = vq * (double)vzz;
-      }
+      if (lane == 0) gather_store<AT, FL>(a, o, sP[ai], sQ[ai], ds, f);
     }
+    __syncthreads();
   }
 }
 
-// Fallback for supports too wide for the lane layout or the LDS brick: one
-// wave per particle, every cube point evaluated with its own factors and the
-// potential read from global memory.
-template <typename CT, typename AT>
+// Supports too wide for the LDS path: every binned particle (one bin of
+// capacity n) from global memory, one wave per particle.
+template <typename CT, typename AT, int FL>
 __global__ void __launch_bounds__(256)
-    k_gather_global(GatherArgs a, int flags, const V4<CT>* __restrict__ sorted,
-                    const int* __restrict__ perm, const int* __restrict__ starts,
-                    const CT* __restrict__ phi, GatherOut o) {
-  const Geom& g = a.g;
-  const int n = starts[a.b.nbins];
-  const int lane = threadIdx.x & 63;
-  const int gw = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int nwave = (gridDim.x * blockDim.x) >> 6;
-  const int NX = 2 * g.nx + 1, NY = 2 * g.ny + 1, NZ = 2 * g.nz + 1;
-  const CT inv2s2 = (CT)a.inv2s2;
-  const CT rc2 = (CT)a.rc2;
-  for (int i = gw; i < n; i += nwave) {
-    const V4<CT> p = sorted[i];
-    const double x = (double)p.x, y = (double)p.y, z = (double)p.z;
-    const int cx = (int)round(x * g.ihx) - g.nx;
-    const int cy = (int)round(y * g.ihy) - g.ny;
-    const int cz = (int)round(z * g.ihz) - g.nz;
-    AT f[13];
-    for (int c = 0; c < 13; c++) f[c] = 0;
-    for (int idx = lane; idx < NX * NY * NZ; idx += 64) {
-      const int tx = idx % NX, t2 = idx / NX, ty = t2 % NY, tz = t2 / NY;
-      const CT dx = (CT)sub_rn(x, mul_rn((double)(cx + tx), g.hx));
-      const CT dy = (CT)sub_rn(y, mul_rn((double)(cy + ty), g.hy));
-      const CT dz = (CT)sub_rn(z, mul_rn((double)(cz + tz), g.hz));
-      const CT r2 = add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz));
-      const int gx = wrapi(cx + tx, g.X), gy = wrapi(cy + ty, g.Y), gz = wrapi(cz + tz, g.Z);
-      const CT ph = phi[gx + (size_t)g.X * (gy + (size_t)g.Y * gz)];
-      const CT wt = gexp(mul_rn(mul_rn(-dx, dx), inv2s2)) * gexp(mul_rn(mul_rn(-dy, dy), inv2s2)) *
-                    gexp(mul_rn(mul_rn(-dz, dz), inv2s2));
-      const AT s = (AT)(wt * ph);
-      if (r2 < rc2) {
-        f[0] += s * (AT)dx;
-        f[1] += s * (AT)dy;
-        f[2] += s * (AT)dz;
-      }
-      f[3] += s * (AT)dx;
-      f[4] += s * (AT)dy;
-      f[5] += s * (AT)dz;
-      f[6] += s;
-      f[7] += s * (AT)dx * (AT)dx;
-      f[8] += s * (AT)dx * (AT)dy;
-      f[9] += s * (AT)dx * (AT)dz;
-      f[10] += s * (AT)dy * (AT)dy;
-      f[11] += s * (AT)dy * (AT)dz;
-      f[12] += s * (AT)dz * (AT)dz;
-    }
-    for (int c = 0; c < 13; c++) f[c] = wave_sum(f[c]);
-    if (lane == 0) {
-      const int oi = perm[i];
-      const double ds = o.dscale ? *o.dscale : 1.0;
-      const double fq = (double)p.w * a.prefF * o.fscale * ds;
-      const double vq = (double)p.w * a.prefF * o.vscale;
-      if ((flags & GF_SPHERE) && o.fx) {
-        ((AT*)o.fx)[oi] = (AT)(fq * (double)f[0]);
-        ((AT*)o.fy)[oi] = (AT)(fq * (double)f[1]);
-        ((AT*)o.fz)[oi] = (AT)(fq * (double)f[2]);
-      }
-      if ((flags & GF_CUBE) && o.cx) {
-        ((AT*)o.cx)[oi] = (AT)(fq * (double)f[3]);
-        ((AT*)o.cy)[oi] = (AT)(fq * (double)f[4]);
-        ((AT*)o.cz)[oi] = (AT)(fq * (double)f[5]);
-      }
-      if ((flags & GF_POT) && o.pot) ((AT*)o.pot)[oi] = (AT)(a.prefP * o.pscale * ds * (double)f[6]);
-      if ((flags & GF_VIRIAL) && o.vir6) {
-        const size_t st = (size_t)o.vir_stride;
-        for (int c = 0; c < 6; c++) o.vir6[c * st + oi] = vq * (double)f[7 + c];
-      }
-    }
+    k_gather_direct(GatherArgs a, const V4<CT>* __restrict__ sorted, const int* __restrict__ perm,
+                    const int* __restrict__ counts, int* __restrict__ counts_other,
+                    BinStats* __restrict__ stats_other, const CT* __restrict__ phi, GatherOut o) {
+  if (blockIdx.x == gridDim.x - 1) {  // spare workgroup: k-pass finalize
+    if (o.fin_partials)
+      kpass_finalize_block(o.fin_partials, kKpassBlocks, o.fin_ncomp, o.fin_pref, o.fin_energy,
+                           o.fin_virial);
+    return;
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    counts_other[0] = 0;
+    stats_other->qmax_bits = 0ull;
+    stats_other->invalid = 0;
+    stats_other->overflow = 0;
+  }
+  const double ds = o.dscale ? *o.dscale : 1.0;
+  const int n = min(counts[0], a.cap_bin);
+  // every block but the spare one shares the particles
+  const int gw = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int nwave = ((gridDim.x - 1) * blockDim.x) >> 6;
+  for (int i = gw; i < n; i += nwave) gather_one_global<CT, AT, FL>(a, o, sorted[i], perm[i], ds, phi);
 }
 
 // ---------------------------------------------------------------------------
 template <typename CT, typename AT, int FL>
-static hipError_t gather_nz(const GatherArgs& a, const Binning& w, const void* phi,
-                            const GatherOut& o, size_t lds, unsigned nwg, hipStream_t s) {
+static hipError_t gather_nz(GatherArgs a, const Binning& w, const void* phi, const GatherOut& o,
+                            hipStream_t s) {
+  const V4<CT>* xq = (const V4<CT>*)w.xyzq;
   const V4<CT>* srt = (const V4<CT>*)w.sorted;
   const CT* ph = (const CT*)phi;
-  switch (2 * a.g.nz + 1) {
-#define GSE_G_CASE(NZV)                                                                     \
-  case NZV:                                                                                 \
-    k_gather<CT, AT, NZV, FL><<<nwg, 256, lds, s>>>(a, srt, w.perm, w.counts, ph, o);      \
-    break;
-    GSE_G_CASE(9)
-    GSE_G_CASE(11)
-    GSE_G_CASE(13)
-    GSE_G_CASE(15)
-    GSE_G_CASE(17)
-#undef GSE_G_CASE
-    default:
-      k_gather<CT, AT, 0, FL><<<nwg, 256, lds, s>>>(a, srt, w.perm, w.counts, ph, o);
+  if (a.b.direct) {
+    a.nwg_bins = 1;
+    k_gather_direct<CT, AT, FL><<<2049, 256, 0, s>>>(a, srt, w.perm, w.counts, w.counts_other,
+                                                     w.stats_other, ph, o);
+    return hipGetLastError();
   }
+  const int NW = 2 * (a.g.nx + a.g.ny + a.g.nz) + 3;
+  const int vol = a.b.Lx * a.b.Ly * a.b.Lz;
+  const Tuning& tu = tuning();
+  const size_t lds_total = (size_t)vol * sizeof(CT) + (size_t)tu.gather_stage;
+  int cap = 256;
+  while (cap > 1 && gather_lds(vol, cap, NW, sizeof(CT)).total > lds_total) cap--;
+  a.cap = cap;
+  const size_t lds = gather_lds(vol, cap, NW, sizeof(CT)).total;
+  if (lds > 150 * 1024) return hipErrorInvalidConfiguration;
+  a.nwg_bins = ((a.b.nbins + 7) / 8) * 8;
+  const unsigned nwg = (unsigned)(a.nwg_bins + kOverflowWG + (o.fin_partials ? 1 : 0));
+  const int bs = tu.gather_block ? tu.gather_block : (sizeof(CT) == 4 ? 512 : 256);
+#define GSE_G_ARGS a, xq, srt, w.perm, w.counts, w.counts_other, w.stats, w.stats_other, w.ovf, ph, o
+#define GSE_G_BS(NZV)                                                      \
+  if (bs == 512)                                                           \
+    k_gather<CT, AT, NZV, FL, 512><<<nwg, 512, lds, s>>>(GSE_G_ARGS);     \
+  else                                                                     \
+    k_gather<CT, AT, NZV, FL, 256><<<nwg, 256, lds, s>>>(GSE_G_ARGS);
+  switch (2 * a.g.nz + 1) {
+    case 11: GSE_G_BS(11) break;
+    case 13: GSE_G_BS(13) break;
+    case 15: GSE_G_BS(15) break;
+    case 17: GSE_G_BS(17) break;
+    default: GSE_G_BS(0) break;
+  }
+#undef GSE_G_BS
+#undef GSE_G_ARGS
   return hipGetLastError();
 }
 
 template <typename CT, typename AT>
 static hipError_t gather_fl(int flags, const GatherArgs& a, const Binning& w, const void* phi,
-                            const GatherOut& o, size_t lds, unsigned nwg, hipStream_t s) {
+                            const GatherOut& o, hipStream_t s) {
   // smallest instantiated superset of the requested outputs
   const int S = GF_SPHERE, C = GF_CUBE, P = GF_POT, V = GF_VIRIAL;
-  if ((flags & ~S) == 0) return gather_nz<CT, AT, S>(a, w, phi, o, lds, nwg, s);
-  if ((flags & ~(S | P)) == 0) return gather_nz<CT, AT, S | P>(a, w, phi, o, lds, nwg, s);
-  if ((flags & ~(C | P)) == 0) return gather_nz<CT, AT, C | P>(a, w, phi, o, lds, nwg, s);
-  if ((flags & ~(C | P | V)) == 0) return gather_nz<CT, AT, C | P | V>(a, w, phi, o, lds, nwg, s);
-  return gather_nz<CT, AT, S | C | P | V>(a, w, phi, o, lds, nwg, s);
+  if ((flags & ~S) == 0) return gather_nz<CT, AT, S>(a, w, phi, o, s);
+  if ((flags & ~(S | P)) == 0) return gather_nz<CT, AT, S | P>(a, w, phi, o, s);
+  if ((flags & ~(C | P)) == 0) return gather_nz<CT, AT, C | P>(a, w, phi, o, s);
+  if ((flags & ~(C | P | V)) == 0) return gather_nz<CT, AT, C | P | V>(a, w, phi, o, s);
+  return gather_nz<CT, AT, S | C | P | V>(a, w, phi, o, s);
 }
 
 hipError_t launch_gather(bool ct_double, bool at_double, const Geom& g, const BinShape& b,
-                         double sigma, double rcut, int flags, const Binning& w, int n,
-                         const void* phi, const GatherOut& o, hipStream_t s) {
-  (void)n;
+                         double sigma, double rcut, int flags, const Binning& w, const void* phi,
+                         const GatherOut& o, hipStream_t s) {
   GatherArgs a;
   a.g = g;
   a.b = b;
@@ -388,25 +475,12 @@ hipError_t launch_gather(bool ct_double, bool at_double, const Geom& g, const Bi
   a.rc2 = rcut * rcut;
   a.prefF = pow(2.0 * M_PI, -3.0 / 2.0) * g.hx * g.hy * g.hz * pow(sigma, -5.0);
   a.prefP = pow(2.0 * M_PI * sigma * sigma, -3.0 / 2.0) * g.hx * g.hy * g.hz;
-  const int NX = 2 * g.nx + 1, NY = 2 * g.ny + 1, NZ = 2 * g.nz + 1;
-  const bool lane_ok = NX + NY + NZ <= 64 && (NX * NY + 63) / 64 <= kMaxK;
-  if (b.direct || !lane_ok) {
-    if (ct_double)
-      k_gather_global<double, double><<<2048, 256, 0, s>>>(a, flags, (const V4<double>*)w.sorted,
-                                                           w.perm, w.counts, (const double*)phi, o);
-    else if (at_double)
-      k_gather_global<float, double><<<2048, 256, 0, s>>>(a, flags, (const V4<float>*)w.sorted,
-                                                          w.perm, w.counts, (const float*)phi, o);
-    else
-      k_gather_global<float, float><<<2048, 256, 0, s>>>(a, flags, (const V4<float>*)w.sorted,
-                                                         w.perm, w.counts, (const float*)phi, o);
-    return hipGetLastError();
-  }
-  const size_t lds = b.brick_bytes(ct_double ? 8 : 4);
-  const unsigned nwg = (unsigned)(((b.nbins + 7) / 8) * 8);
-  if (ct_double) return gather_fl<double, double>(flags, a, w, phi, o, lds, nwg, s);
-  if (at_double) return gather_fl<float, double>(flags, a, w, phi, o, lds, nwg, s);
-  return gather_fl<float, float>(flags, a, w, phi, o, lds, nwg, s);
+  a.cap = 0;
+  a.cap_bin = w.cap_bin;
+  a.nwg_bins = 0;
+  if (ct_double) return gather_fl<double, double>(flags, a, w, phi, o, s);
+  if (at_double) return gather_fl<float, double>(flags, a, w, phi, o, s);
+  return gather_fl<float, float>(flags, a, w, phi, o, s);
 }
 
 }  // namespace gse

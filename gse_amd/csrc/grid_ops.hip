@@ -151,25 +151,10 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// out[0] = pref * sum E, out[1+3a+b] = -delta_ab E - pref * sum K_ab
-__global__ void k_kpass_finalize(const double* __restrict__ partials, int nblocks, int ncomp,
-                                 double pref, double* __restrict__ out) {
-  __shared__ double sh[7];
-  if (threadIdx.x < ncomp) {
-    double s = 0.0;
-    for (int b = 0; b < nblocks; b++) s += partials[b * ncomp + threadIdx.x];
-    sh[threadIdx.x] = s * pref;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const double E = sh[0];
-    out[0] = E;
-    if (ncomp == 7) {
-      const double K[6] = {sh[1], sh[2], sh[3], sh[4], sh[5], sh[6]};
-      const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
-      for (int i = 0; i < 9; i++) out[1 + i] = -K[map[i]] - ((i == 0 || i == 4 || i == 8) ? E : 0.0);
-    }
-  }
+__global__ void __launch_bounds__(256)
+    k_kpass_finalize(const double* __restrict__ partials, int nblocks, int ncomp, double pref,
+                     double* __restrict__ energy, double* __restrict__ virial) {
+  kpass_finalize_block(partials, nblocks, ncomp, pref, energy, virial);
 }
 
 template <typename T>
@@ -190,13 +175,13 @@ __global__ void __launch_bounds__(256)
   block_sum_store<6>(acc, partials);
 }
 
-__global__ void k_reduce_partials(const double* __restrict__ partials, int nblocks, int ncomp,
-                                  int stride, double scale, double* __restrict__ out) {
-  const int c = threadIdx.x;
-  if (c >= ncomp) return;
-  double s = 0.0;
-  for (int b = 0; b < nblocks; b++) s += partials[b * stride + c];
-  out[c] = s * scale;
+template <int NC>
+__global__ void __launch_bounds__(256)
+    k_reduce_partials(const double* __restrict__ partials, int nblocks, int ncomp, int stride,
+                      double scale, double* __restrict__ out) {
+  double t[NC];
+  reduce_partials_block<NC>(partials, nblocks, stride, t);
+  if (threadIdx.x < ncomp && threadIdx.x < NC) out[threadIdx.x] = t[threadIdx.x] * scale;
 }
 
 template <typename T>
@@ -324,9 +309,9 @@ hipError_t launch_kpass(bool dbl, int kind, int X, int Y, int Z, double bx, doub
   return hipGetLastError();
 }
 
-hipError_t launch_kpass_finalize(const double* partials, double pref, double* out, bool vir,
-                                 hipStream_t s) {
-  k_kpass_finalize<<<1, 64, 0, s>>>(partials, kKpassBlocks, vir ? 7 : 1, pref, out);
+hipError_t launch_kpass_finalize(const double* partials, int ncomp, double pref, double* energy,
+                                 double* virial, hipStream_t s) {
+  k_kpass_finalize<<<1, 256, 0, s>>>(partials, kKpassBlocks, ncomp, pref, energy, virial);
   return hipGetLastError();
 }
 
@@ -341,45 +326,45 @@ hipError_t launch_dot(bool dbl, const void* a, const void* b, size_t n, double* 
 
 hipError_t launch_reduce_partials(const double* partials, int nblocks, int ncomp, int stride,
                                   double scale, double* out, hipStream_t s) {
-  k_reduce_partials<<<1, 64, 0, s>>>(partials, nblocks, ncomp, stride, scale, out);
+  if (ncomp <= 1)
+    k_reduce_partials<1><<<1, 256, 0, s>>>(partials, nblocks, ncomp, stride, scale, out);
+  else if (ncomp <= 3)
+    k_reduce_partials<3><<<1, 256, 0, s>>>(partials, nblocks, ncomp, stride, scale, out);
+  else
+    k_reduce_partials<6><<<1, 256, 0, s>>>(partials, nblocks, ncomp, stride, scale, out);
   return hipGetLastError();
 }
 
-__global__ void k_add_sym6(const double* __restrict__ partials, int nblocks, double* vir9) {
-  __shared__ double sh[6];
-  if (threadIdx.x < 6) {
-    double s = 0.0;
-    for (int b = 0; b < nblocks; b++) s += partials[b * 6 + threadIdx.x];
-    sh[threadIdx.x] = s;
-  }
-  __syncthreads();
+__global__ void __launch_bounds__(256)
+    k_add_sym6(const double* __restrict__ partials, int nblocks, double* vir9) {
+  double t[6];
+  reduce_partials_block<6>(partials, nblocks, 6, t);
   if (threadIdx.x < 9) {
     const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
-    vir9[threadIdx.x] += sh[map[threadIdx.x]];
+    vir9[threadIdx.x] += t[map[threadIdx.x]];
   }
 }
 
 hipError_t launch_add_sym6(const double* partials, int nblocks, double* vir9, hipStream_t s) {
-  k_add_sym6<<<1, 64, 0, s>>>(partials, nblocks, vir9);
+  k_add_sym6<<<1, 256, 0, s>>>(partials, nblocks, vir9);
   return hipGetLastError();
 }
 
-__global__ void k_corrections(const double* __restrict__ partials, int nblocks, double sigma,
-                              double vol, double ccelec, double* out) {
-  if (threadIdx.x != 0) return;
-  double q = 0.0, q2 = 0.0;
-  for (int b = 0; b < nblocks; b++) {
-    q += partials[2 * b];
-    q2 += partials[2 * b + 1];
+__global__ void __launch_bounds__(256)
+    k_corrections(const double* __restrict__ partials, int nblocks, double sigma, double vol,
+                  double ccelec, double* out) {
+  double t[2];
+  reduce_partials_block<2>(partials, nblocks, 2, t);
+  if (threadIdx.x == 0) {
+    out[0] = -ccelec * t[1] / sqrt(2.0 * M_PI * sigma * sigma);
+    out[1] = -ccelec * M_PI * t[0] * t[0] * sigma * sigma / vol;
   }
-  out[0] = -ccelec * q2 / sqrt(2.0 * M_PI * sigma * sigma);
-  out[1] = -ccelec * M_PI * q * q * sigma * sigma / vol;
 }
 
 hipError_t launch_corrections_finalize(const double* partials, int nblocks, double sigma,
                                        double bx, double by, double bz, double ccelec,
                                        double* out, hipStream_t s) {
-  k_corrections<<<1, 64, 0, s>>>(partials, nblocks, sigma, bx * by * bz, ccelec, out);
+  k_corrections<<<1, 256, 0, s>>>(partials, nblocks, sigma, bx * by * bz, ccelec, out);
   return hipGetLastError();
 }
 

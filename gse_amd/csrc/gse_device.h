@@ -105,3 +105,53 @@ struct KSum {
 };
 
 }  // namespace gse
+
+namespace gse {
+
+// Sum of partials[b*stride + c] over b < nblocks for c < NC, by one block in
+// a fixed order (strided per-thread sums, butterfly, waves combined in index
+// order).  Every thread receives the NC totals.  Needs blockDim.x <= 1024.
+template <int NC>
+__device__ __forceinline__ void reduce_partials_block(const double* __restrict__ partials,
+                                                      int nblocks, int stride, double (&tot)[NC]) {
+  __shared__ double sh_red[NC][16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partials[b * stride + c];
+    s = wave_sum(s);
+    if (lane == 0) sh_red[c][wid] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    double s = 0.0;
+    for (int w = 0; w < nw; w++) s += sh_red[c][w];
+    tot[c] = s;
+  }
+}
+
+// Energy / k-space virial from the k-pass partials ([nblocks][ncomp], ncomp 1
+// or 7): energy[0] = pref*E, virial[3a+b] = -delta_ab E - pref*K_ab.
+__device__ __forceinline__ void kpass_finalize_block(const double* __restrict__ partials,
+                                                     int nblocks, int ncomp, double pref,
+                                                     double* energy, double* virial) {
+  const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+  if (ncomp == 7) {
+    double t[7];
+    reduce_partials_block<7>(partials, nblocks, 7, t);
+    const double E = t[0] * pref;
+    if (threadIdx.x == 0 && energy) energy[0] = E;
+    if (virial && threadIdx.x < 9) {
+      const int i = threadIdx.x;
+      virial[i] = -t[1 + map[i]] * pref - ((i == 0 || i == 4 || i == 8) ? E : 0.0);
+    }
+  } else {
+    double t[1];
+    reduce_partials_block<1>(partials, nblocks, ncomp, t);
+    if (threadIdx.x == 0 && energy) energy[0] = t[0] * pref;
+  }
+}
+
+}  // namespace gse

@@ -34,31 +34,88 @@ struct BinShape {
 
 BinShape choose_bins(const Geom& g, size_t elem_bytes, size_t lds_budget, int pref_edge);
 
-// Device-side fixed-point parameters of one spread
-struct FixedParams {
-  double scale;      // 2^e
-  double inv_scale;  // 2^-e
-  unsigned long long qmax_bits;  // bits of max |q| (as double)
-  int invalid;       // number of rejected particles (non-finite / out of range)
-  int pad;
+// Launch-shape knobs (process-wide; the autotuner sets them per workload).
+struct Tuning {
+  int bin_edge = 8;            // preferred bin edge in cells
+  int brick_budget = 64 * 1024;  // LDS bytes allowed for an int64 brick
+  int spread_block = 0;        // threads per spread workgroup (0: 512 FP32 / 256 FP64)
+  int gather_block = 0;        // threads per gather workgroup (0: 512 FP32 / 256 FP64)
+  int spread_stage = 16 * 1024;  // LDS bytes for staged particles beside the brick
+  int gather_stage = 16 * 1024;
+  int spread_lds32 = 1;        // FP32 grids: 32-bit accumulators with a per-bin scale
+  int spread_cols = 0;         // column-owner spread in registers (else LDS-atomic brick)
+};
+Tuning& tuning();
+
+// Per-binning statistics, accumulated by the binning kernel and read by its
+// consumers (spread, conversion, gather).  Two copies alternate between
+// consecutive binnings; the consumers of one clear the other.
+struct BinStats {
+  unsigned long long qmax_bits;  // bits of max |q| (as a positive double)
+  int invalid;                   // particles rejected (non-finite / out of range)
+  int overflow;                  // particles that did not fit their bin
 };
 
-// Binning scratch owned by a handle
+// Particles binned into fixed-capacity bins: bin b owns slots
+// [b*cap_bin, (b+1)*cap_bin) of sorted/perm; counts[b] may exceed cap_bin,
+// the excess goes to the overflow list (indices into xyzq).
 struct Binning {
-  int n_cap = 0;
-  int bins_cap = 0;
-  int* counts = nullptr;     // [bins_cap + 1], exclusive starts after scan
-  int* atom_bin = nullptr;   // [n_cap]
-  int* atom_slot = nullptr;  // [n_cap]
-  int* perm = nullptr;       // [n_cap] sorted slot -> original index
-  void* sorted = nullptr;    // [n_cap] 4-vectors of the compute type
-  FixedParams* fp = nullptr;
+  int n = 0;                 // particles in the call
+  int cap_bin = 0;
+  const void* xyzq = nullptr;  // the caller's particles
+  int* counts = nullptr;       // [nbins] (current buffer)
+  int* counts_other = nullptr; // [nbins] cleared by this binning's consumers
+  BinStats* stats = nullptr;
+  BinStats* stats_other = nullptr;
+  int* ovf = nullptr;          // [n]
+  int* perm = nullptr;         // [nbins*cap_bin]
+  void* sorted = nullptr;      // [nbins*cap_bin] 4-vectors of the compute type
+  int images = 1;              // cube images per grid point (2n+1 > M wraps)
+  double pref_bound = 0.0;     // bound of |q|-free contribution: pref * max weight
 };
+
+// Fixed-point exponent e (scale 2^e) of one spread: every contribution
+// |q pref w w w| stays below 2^51 and any grid point's sum below 2^62.
+__host__ __device__ inline int fixed_exponent(unsigned long long qbits, int nvalid, int images,
+                                              double pref_bound) {
+  union {
+    unsigned long long u;
+    double d;
+  } cv;
+  cv.u = qbits;
+  const double one = cv.d * pref_bound;
+  if (!(one > 0.0)) return 60;
+  const double all = one * (double)(nvalid > 0 ? nvalid : 1) * (double)images;
+  int k1, k2;
+  frexp(one, &k1);  // one < 2^k1
+  frexp(all, &k2);
+  int e = 51 - k1 < 62 - k2 ? 51 - k1 : 62 - k2;
+  if (e < -60) e = -60;
+  if (e > 200) e = 200;
+  return e;
+}
+
+// Exponent of a 32-bit brick holding the sum of `cnt` contributions, each at
+// most qmax * pref_bound: the sum stays below 2^30.
+__host__ __device__ inline int fixed_exponent_bin(unsigned long long qbits, int cnt,
+                                                  double pref_bound) {
+  union {
+    unsigned long long u;
+    double d;
+  } cv;
+  cv.u = qbits;
+  const double all = cv.d * pref_bound * (double)(cnt > 0 ? cnt : 1);
+  if (!(all > 0.0)) return 30;
+  int k;
+  frexp(all, &k);  // all < 2^k
+  return 30 - k;
+}
 
 // ---- kernel launchers (defined in the .hip units) ------------------------
 // ct_double: particle/grid compute type is double (else float)
-hipError_t launch_bin(bool ct_double, const Geom& g, const BinShape& b, int n,
-                      const void* xyzq, Binning& w, int images, double pref,
+// Clears counts[0..nbins) and the statistics of one buffer.
+hipError_t launch_bin_clear(int* counts, int nbins, BinStats* stats, hipStream_t s);
+hipError_t launch_bin(bool ct_double, const Geom& g, const BinShape& b, Binning& w,
                       hipStream_t s);
 
 hipError_t launch_spread(bool ct_double, const Geom& g, const BinShape& b,
@@ -70,7 +127,7 @@ hipError_t launch_spread(bool ct_double, const Geom& g, const BinShape& b,
 double spread_weight_bound(const Geom& g, double sigma, bool normalize);
 
 hipError_t launch_fixed_to_grid(bool ct_double, const unsigned long long* grid64,
-                                void* out, size_t count, const FixedParams* fp,
+                                void* out, size_t count, const Binning& w,
                                 bool clear, hipStream_t s);
 
 hipError_t launch_fill_u64(unsigned long long* p, size_t count, hipStream_t s);
@@ -98,12 +155,22 @@ struct GatherOut {
   double pscale = 1.0;     // multiplies the potential
   double vscale = 1.0;     // multiplies the virial moments
   const double* dscale = nullptr;  // optional device scalar on forces and potential
+  // optional k-pass finalize run by one spare workgroup of the gather
+  const double* fin_partials = nullptr;
+  int fin_ncomp = 0;
+  double fin_pref = 0.0;
+  double* fin_energy = nullptr;
+  double* fin_virial = nullptr;
 };
 
 hipError_t launch_gather(bool ct_double, bool at_double, const Geom& g,
                          const BinShape& b, double sigma, double rcut,
-                         int flags, const Binning& w, int n, const void* phi,
+                         int flags, const Binning& w, const void* phi,
                          const GatherOut& o, hipStream_t s);
+
+// Number of extra workgroups the spread and gather launch for overflow
+// particles (processed without LDS staging).
+constexpr int kOverflowWG = 8;
 
 // k-space
 hipError_t launch_greens_table(bool dbl, int kind, int X, int Y, int Z,
@@ -112,15 +179,19 @@ hipError_t launch_greens_table(bool dbl, int kind, int X, int Y, int Z,
 
 // In-place multiply of the half spectrum by the table, FP64 partial sums of
 // the energy (and the 6 k-space virial moments when vir is set) per block.
+// In-place multiply of the half spectrum by the table; FP64 partial sums of
+// the energy (and, when vir is set, the six k-space virial moments) per
+// block into partials[kKpassBlocks][vir ? 7 : 1].
 constexpr int kKpassBlocks = 512;
 hipError_t launch_kpass(bool dbl, int kind, int X, int Y, int Z, double bx,
                         double by, double bz, double sigma, void* spec,
                         const void* table, bool vir, double* partials,
                         hipStream_t s);
-// partials [kKpassBlocks][7] -> out[0] = E, out[1..9] = W (3x3) with
-// prefactor 0.5*ccelec*vcell; adds `extra9` (device, may be null) into W.
-hipError_t launch_kpass_finalize(const double* partials, double pref, double* out,
-                                 bool vir, hipStream_t s);
+// energy[0] = pref*E (may be null); virial[0..8] = W^k (ncomp 7 only) with
+// pref = 0.5*ccelec*vcell.  The same reduction can run in a spare workgroup
+// of the following gather (GatherOut::fin), saving a launch.
+hipError_t launch_kpass_finalize(const double* partials, int ncomp, double pref,
+                                 double* energy, double* virial, hipStream_t s);
 
 // real-space reductions
 constexpr int kRedBlocks = 512;

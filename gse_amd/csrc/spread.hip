@@ -1,21 +1,23 @@
 // Cell-binned Gaussian charge spreading with fixed-point int64 accumulation
 // (CpuGaussCharge::spreadChargeToGrid, CpuGaussCharge.cpp:235-384).
 //
-// Pipeline (all on one stream):
-//   k_bin_reset    zero bin counters and the fixed-point statistics
-//   k_bin_count    cube origin o = wrap(round(x*inv_h) - n) per particle,
-//                  bin of o, slot in the bin, max|q|
-//   k_bin_scan     exclusive scan of the counters (one workgroup) and the
-//                  fixed-point scale 2^e chosen from N, max|q| and pref
-//   k_bin_scatter  particles copied into bin order (sorted 4-vectors + perm)
+//   k_bin          one pass over the particles: cube origin
+//                  o = wrap(round(x*inv_h) - n), bin of o, slot in the bin by
+//                  atomic count; the particle is copied into its bin's fixed
+//                  slots (or onto the overflow list); max|q| and rejected
+//                  particles are counted.
 //   k_spread       one workgroup per bin: the (B+2n)^3 brick lives in LDS as
-//                  int64; every wave takes one particle at a time, lanes own
-//                  (tx,ty) columns of its cube and walk tz, adding
-//                  round(q*pref*wx*wy*wz*2^e) with ds_add_u64; the brick is
-//                  then added into the global int64 grid with 64-bit atomics.
-//   k_fixed_to_grid  int64 -> T with 2^-e, and the int64 grid re-zeroed.
+//                  int64.  The bin's particles are staged into LDS, one wave per
+//                  particle computes its 3(2n+1) factors, then one wave per
+//                  particle walks its cube with lanes on (tx,ty) columns and
+//                  adds round(q*pref*wx*wy*wz*2^e) with ds_add_u64.  The brick
+//                  is added into the global int64 grid with 64-bit atomics.
+//                  kOverflowWG extra workgroups add the overflow particles
+//                  straight into the global grid.
+//   k_fixed_to_grid  int64 -> T with 2^-e, the int64 grid re-zeroed.
 // Integer addition is associative, so the grid is bit-identical run to run
-// whatever order the particles and workgroups run in.
+// whatever order particles and workgroups run in.  The scale 2^e follows from
+// max|q|, the particle count and the support (fixed_exponent).
 #include <math.h>
 
 #include "gse_device.h"
@@ -36,23 +38,25 @@ __device__ __forceinline__ int xcd_remap(int wg, int nwg) {
   return (wg & 7) * chunk + (wg >> 3);
 }
 
-__global__ void k_bin_reset(int* counts, int n1, FixedParams* fp) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n1; i += gridDim.x * blockDim.x)
+__global__ void k_bin_clear(int* counts, int nbins, BinStats* stats) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nbins; i += gridDim.x * blockDim.x)
     counts[i] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    fp->qmax_bits = 0ull;
-    fp->invalid = 0;
+    stats->qmax_bits = 0ull;
+    stats->invalid = 0;
+    stats->overflow = 0;
   }
 }
 
 template <typename CT>
-__global__ void k_bin_count(Geom g, BinShape b, int n, const V4<CT>* __restrict__ xyzq,
-                            int* __restrict__ counts, int* __restrict__ atom_bin,
-                            int* __restrict__ atom_slot, FixedParams* fp) {
+__global__ void __launch_bounds__(256)
+    k_bin(Geom g, BinShape b, int n, int cap_bin, const V4<CT>* __restrict__ xyzq,
+          int* __restrict__ counts, BinStats* __restrict__ stats, int* __restrict__ ovf,
+          int* __restrict__ perm, V4<CT>* __restrict__ sorted) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double aq = 0.0;
   if (i < n) {
-    V4<CT> p = xyzq[i];
+    const V4<CT> p = xyzq[i];
     // x*inv_h in FP64 as CpuGaussCharge.cpp:311-313 (CT=double); float
     // coordinates are promoted so cube selection matches the FP64 oracle
     const double fx = (double)p.x * g.ihx;
@@ -66,117 +70,38 @@ __global__ void k_bin_count(Geom g, BinShape b, int n, const V4<CT>* __restrict_
       const int oy = wrapi((int)round(fy) - g.ny, g.Y);
       const int oz = wrapi((int)round(fz) - g.nz, g.Z);
       const int bin = ox / b.Bx + b.nbx * (oy / b.By + b.nby * (oz / b.Bz));
-      atom_bin[i] = bin;
-      atom_slot[i] = atomicAdd(&counts[bin], 1);
+      const int slot = atomicAdd(&counts[bin], 1);
+      if (slot < cap_bin) {
+        const size_t pos = (size_t)bin * cap_bin + slot;
+        sorted[pos] = p;
+        perm[pos] = i;
+      } else {
+        ovf[atomicAdd(&stats->overflow, 1)] = i;
+      }
       aq = fabs((double)p.w);
     } else {
-      atom_bin[i] = -1;
-      atomicAdd(&fp->invalid, 1);
+      atomicAdd(&stats->invalid, 1);
     }
   }
+  // one max per workgroup reaches memory
+  __shared__ double wmax[4];
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) aq = fmax(aq, __shfl_xor(aq, m, kWave));
-  if ((threadIdx.x & (kWave - 1)) == 0 && aq > 0.0)
-    atomicMax(&fp->qmax_bits, (unsigned long long)__double_as_longlong(aq));
-}
-
-// One workgroup of 1024 threads: exclusive scan in place, counts[nb] = total.
-__global__ void __launch_bounds__(1024) k_bin_scan(int* counts, int nb, int n, int images,
-                                                   double pref, FixedParams* fp) {
-  __shared__ int wsum[16];
-  __shared__ int carry;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  if (tid == 0) carry = 0;
+  if ((threadIdx.x & (kWave - 1)) == 0) wmax[threadIdx.x >> 6] = aq;
   __syncthreads();
-  for (int base = 0; base < nb; base += 1024) {
-    const int i = base + tid;
-    int v = i < nb ? counts[i] : 0;
-    // inclusive wave scan
-    int x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      int y = __shfl_up(x, o, kWave);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) wsum[wid] = x;
-    __syncthreads();
-    if (wid == 0) {
-      int s = lane < 16 ? wsum[lane] : 0;
-#pragma unroll
-      for (int o = 1; o < 16; o <<= 1) {
-        int y = __shfl_up(s, o, kWave);
-        if (lane >= o) s += y;
-      }
-      if (lane < 16) wsum[lane] = s;
-    }
-    __syncthreads();
-    const int excl = carry + (wid ? wsum[wid - 1] : 0) + x - v;
-    if (i < nb) counts[i] = excl;
-    __syncthreads();
-    if (tid == 1023) carry = excl + v;
-    __syncthreads();
-  }
-  if (tid == 0) {
-    counts[nb] = carry;
-    // Fixed-point scale: every contribution |q pref w w w| <= qmax*pref must
-    // stay below 2^51 (exact rounding trick) and the sum over all particles
-    // and cube images reaching one point below 2^62.
-    const double qmax = __longlong_as_double((long long)fp->qmax_bits);
-    const double one = qmax * pref;
-    const double all = one * (double)(carry > 0 ? carry : 1) * (double)images;
-    int e = 60;
-    if (one > 0.0) {
-      int k1, k2;
-      frexp(one, &k1);  // one < 2^k1
-      frexp(all, &k2);
-      e = min(51 - k1, 62 - k2);
-    }
-    e = max(-60, min(e, 200));
-    fp->scale = ldexp(1.0, e);
-    fp->inv_scale = ldexp(1.0, -e);
+  if (threadIdx.x == 0) {
+    const double m = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+    if (m > 0.0) atomicMax(&stats->qmax_bits, (unsigned long long)__double_as_longlong(m));
   }
 }
 
-template <typename CT>
-__global__ void k_bin_scatter(int n, const V4<CT>* __restrict__ xyzq, const int* __restrict__ starts,
-                              const int* __restrict__ atom_bin, const int* __restrict__ atom_slot,
-                              int* __restrict__ perm, V4<CT>* __restrict__ sorted) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int bin = atom_bin[i];
-  if (bin < 0) return;
-  const int pos = starts[bin] + atom_slot[i];
-  perm[pos] = i;
-  sorted[pos] = xyzq[i];
-}
-
-// Per-lane role in the weight phase: lanes [0,NX) x-weights, [NX,NX+NY)
-// y-weights, [NX+NY, NX+NY+NZ) z-weights.
 struct LaneRole {
   int axis;  // 0,1,2 or -1
   int j;
 };
 
-__device__ __forceinline__ LaneRole lane_role(int lane, int NX, int NY, int NZ) {
-  LaneRole r;
-  if (lane < NX) {
-    r.axis = 0;
-    r.j = lane;
-  } else if (lane < NX + NY) {
-    r.axis = 1;
-    r.j = lane - NX;
-  } else if (lane < NX + NY + NZ) {
-    r.axis = 2;
-    r.j = lane - NX - NY;
-  } else {
-    r.axis = -1;
-    r.j = 0;
-  }
-  return r;
-}
-
-// Weight of this lane's (axis, j) entry: exp(-t*t*inv_2sigmasq) with
-// t = (c*h - x) + j*h (CpuGaussCharge.cpp:314-329); offsets in FP64.
+// Weight of (axis, j): exp(-t*t*inv_2sigmasq) with t = (c*h - x) + j*h
+// (CpuGaussCharge.cpp:314-329); offsets in FP64, the exponential in CT.
 template <typename CT>
 __device__ __forceinline__ CT lane_weight(const LaneRole& r, int cx, int cy, int cz, double x,
                                           double y, double z, const Geom& g, double inv2s2) {
@@ -197,170 +122,490 @@ __device__ __forceinline__ CT lane_weight(const LaneRole& r, int cx, int cy, int
 
 constexpr int kMaxK = 5;  // ceil((2n+1)^2/64) for n <= 8
 
-template <typename CT, int NZC>
-__global__ void __launch_bounds__(256)
-    k_spread(Geom g, BinShape b, double inv2s2, double pref, int normalize, double pref1,
-             const V4<CT>* __restrict__ sorted, const int* __restrict__ starts,
-             const FixedParams* __restrict__ fp, unsigned long long* __restrict__ grid64) {
-  extern __shared__ unsigned long long brick[];
-  const int bin = xcd_remap(blockIdx.x, gridDim.x);
+// Adds one particle's cube straight into the global int64 grid (overflow
+// particles and supports too wide for the LDS path); one wave per particle.
+template <typename CT>
+__device__ void spread_one_global(const Geom& g, double inv2s2, double pref, double S,
+                                  int normalize, double pref1, const V4<CT>& p,
+                                  unsigned long long* __restrict__ grid64) {
+  const int lane = threadIdx.x & 63;
+  const int NX = 2 * g.nx + 1, NY = 2 * g.ny + 1, NZ = 2 * g.nz + 1;
+  const double x = (double)p.x, y = (double)p.y, z = (double)p.z;
+  const int cx = (int)round(x * g.ihx) - g.nx;
+  const int cy = (int)round(y * g.ihy) - g.ny;
+  const int cz = (int)round(z * g.ihz) - g.nz;
+  const double qps = ((double)p.w * pref) * S;
+  CT sx = 1, sy = 1, sz = 1;
+  if (normalize) {  // per-axis sums in j order (CpuGaussCharge.cpp:330-362)
+    LaneRole r;
+    sx = sy = sz = 0;
+    r.axis = 0;
+    for (int j = 0; j < NX; j++) { r.j = j; sx += lane_weight<CT>(r, cx, cy, cz, x, y, z, g, inv2s2); }
+    r.axis = 1;
+    for (int j = 0; j < NY; j++) { r.j = j; sy += lane_weight<CT>(r, cx, cy, cz, x, y, z, g, inv2s2); }
+    r.axis = 2;
+    for (int j = 0; j < NZ; j++) { r.j = j; sz += lane_weight<CT>(r, cx, cy, cz, x, y, z, g, inv2s2); }
+  }
+  const int wx0 = wrapi(cx, g.X), wy0 = wrapi(cy, g.Y), wz0 = wrapi(cz, g.Z);
+  for (int idx = lane; idx < NX * NY * NZ; idx += 64) {
+    const int tx = idx % NX, t2 = idx / NX, ty = t2 % NY, tz = t2 / NY;
+    LaneRole r;
+    r.axis = 0; r.j = tx;
+    CT wx = lane_weight<CT>(r, cx, cy, cz, x, y, z, g, inv2s2);
+    r.axis = 1; r.j = ty;
+    CT wy = lane_weight<CT>(r, cx, cy, cz, x, y, z, g, inv2s2);
+    r.axis = 2; r.j = tz;
+    CT wzv = lane_weight<CT>(r, cx, cy, cz, x, y, z, g, inv2s2);
+    if (normalize) {
+      wx = wx * ((CT)pref1 / sx);
+      wy = wy * ((CT)pref1 / sy);
+      wzv = wzv * ((CT)pref1 / sz);
+    }
+    const double v = ((qps * (double)wx) * (double)wy) * (double)wzv;
+    const int gx = (wx0 + tx) % g.X, gy = (wy0 + ty) % g.Y, gz = (wz0 + tz) % g.Z;
+    atomicAdd(&grid64[gx + (size_t)g.X * (gy + (size_t)g.Y * gz)],
+              (unsigned long long)fixed_round(v));
+  }
+}
+
+struct SpreadArgs {
+  Geom g;
+  BinShape b;
+  double inv2s2, pref, pref1;
+  int normalize;
+  int cap;      // particles staged in LDS per chunk
+  int cap_bin;  // slots per bin
+  int n;
+  int images;
+  double pref_bound;
+  int nwg_bins;
+};
+
+// LDS layout of k_spread: int64 brick at 0, then per staged particle (cap
+// of them) the 4-vector (CT, 32-byte aligned block), its 3(2n+1) factors
+// (double), q*pref*2^e (double) and its local brick offset (int).
+struct SpreadLds {
+  size_t oX, oW, oQ, oB, total;
+};
+__host__ __device__ inline SpreadLds spread_lds(int vol, int cap, int nw, int ct_bytes) {
+  SpreadLds l;
+  size_t o = ((size_t)vol * 8 + 31) & ~(size_t)31;
+  l.oX = o;
+  o += (size_t)cap * 4 * ct_bytes;
+  l.oW = o;
+  o += (size_t)cap * nw * 8;
+  l.oQ = o;
+  o += (size_t)cap * 8;
+  l.oB = o;
+  o += (size_t)cap * 4;
+  l.total = o;
+  return l;
+}
+
+// ACC: LDS accumulator, unsigned long long (int64 at the global scale 2^e) or
+// unsigned (int32 at a per-bin scale 2^eb, eb <= e, from the bin's particle
+// count, shifted exactly to 2^e when the brick is flushed).
+template <typename CT, int NZC, int BS, typename ACC>
+__global__ void __launch_bounds__(BS)
+    k_spread(SpreadArgs a, const V4<CT>* __restrict__ xyzq, const V4<CT>* __restrict__ sorted,
+             const int* __restrict__ counts, int* __restrict__ counts_other,
+             const BinStats* __restrict__ stats, BinStats* __restrict__ stats_other,
+             const int* __restrict__ ovf, unsigned long long* __restrict__ grid64) {
+  extern __shared__ __align__(32) unsigned char spread_smem[];
+  ACC* const brick = (ACC*)spread_smem;
+  const Geom& g = a.g;
+  const BinShape& b = a.b;
+  const BinStats st = *stats;
+  const int eg = fixed_exponent(st.qmax_bits, a.n - st.invalid, a.images, a.pref_bound);
+  const double S = ldexp(1.0, eg);
+  if ((int)blockIdx.x >= a.nwg_bins) {  // overflow particles, global atomics
+    if (blockIdx.x == (unsigned)a.nwg_bins && threadIdx.x == 0) {
+      stats_other->qmax_bits = 0ull;
+      stats_other->invalid = 0;
+      stats_other->overflow = 0;
+    }
+    const int wg = blockIdx.x - a.nwg_bins;
+    const int gw = wg * (BS / 64) + (threadIdx.x >> 6);
+    for (int k = __builtin_amdgcn_readfirstlane(gw); k < st.overflow; k += kOverflowWG * (BS / 64))
+      spread_one_global<CT>(g, a.inv2s2, a.pref, S, a.normalize, a.pref1, xyzq[ovf[k]], grid64);
+    return;
+  }
+  const int bin = xcd_remap(blockIdx.x, a.nwg_bins);
   if (bin >= b.nbins) return;
-  const int s0 = starts[bin], s1 = starts[bin + 1];
-  if (s0 == s1) return;
+  const int cnt = min(counts[bin], a.cap_bin);
+  if (threadIdx.x == 0) counts_other[bin] = 0;
+  if (cnt == 0) return;
+  // int32 bricks: per-bin exponent so that the bin's brick sums stay below 2^30
+  int shift = 0;
+  double Sb = S;
+  if (sizeof(ACC) == 4) {
+    const int eb = fixed_exponent_bin(st.qmax_bits, cnt, a.pref_bound);
+    shift = eg > eb ? eg - eb : 0;
+    Sb = ldexp(S, -shift);
+  }
+  const int s0 = bin * a.cap_bin, s1 = s0 + cnt;
   const int bxi = bin % b.nbx, byi = (bin / b.nbx) % b.nby, bzi = bin / (b.nbx * b.nby);
-  const int Lx = b.Lx, Ly = b.Ly, LxLy = b.Lx * b.Ly;
-  const int vol = LxLy * b.Lz;
-  for (int j = threadIdx.x; j < vol; j += blockDim.x) brick[j] = 0ull;
-  __syncthreads();
+  const int Lx = b.Lx, Ly = b.Ly, Lz = b.Lz, LxLy = b.Lx * b.Ly;
+  const int vol = LxLy * Lz;
+  const int NX = 2 * g.nx + 1, NY = 2 * g.ny + 1;
+  const int NZ = NZC > 0 ? NZC : 2 * g.nz + 1;
+  const int NW = NX + NY + NZ;
+  const int cap = a.cap;
+  const SpreadLds lay = spread_lds(vol * (int)sizeof(ACC) / 8 + 1, cap, NW, sizeof(CT));
+  char* lds = (char*)brick;
+  V4<CT>* sX = (V4<CT>*)(lds + lay.oX);  // [cap]
+  double* sW = (double*)(lds + lay.oW);  // [cap][NW]
+  double* sQ = (double*)(lds + lay.oQ);  // [cap]
+  int* sB = (int*)(lds + lay.oB);        // [cap]
+  for (int j = threadIdx.x; j < vol; j += BS) brick[j] = (ACC)0;
 
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int nw = blockDim.x >> 6;
-  const int NX = 2 * g.nx + 1, NY = 2 * g.ny + 1;
-  const int NZ = NZC > 0 ? NZC : 2 * g.nz + 1;
+  constexpr int nwv = BS / 64;
   const int NXY = NX * NY;
   const int KI = (NXY + 63) >> 6;
-  const double S = fp->scale;
-  const LaneRole role = lane_role(lane, NX, NY, NZ);
-
+  LaneRole role;
+  if (lane < NX) {
+    role.axis = 0; role.j = lane;
+  } else if (lane < NX + NY) {
+    role.axis = 1; role.j = lane - NX;
+  } else if (lane < NW) {
+    role.axis = 2; role.j = lane - NX - NY;
+  } else {
+    role.axis = -1; role.j = 0;
+  }
   int k_tx[kMaxK], k_ty[kMaxK], k_off[kMaxK];
   bool k_ok[kMaxK];
 #pragma unroll
   for (int k = 0; k < kMaxK; k++) {
     const int idx = lane + 64 * k;
     k_ok[k] = (k < KI) && idx < NXY;
-    k_tx[k] = k_ok[k] ? idx % NX : 0;
     k_ty[k] = k_ok[k] ? idx / NX : 0;
+    k_tx[k] = k_ok[k] ? idx - k_ty[k] * NX : 0;
     k_off[k] = k_tx[k] + Lx * k_ty[k];
   }
 
-  for (int i = s0 + wid; i < s1; i += nw) {
-    const V4<CT> p = sorted[i];
-    const double x = (double)p.x, y = (double)p.y, z = (double)p.z;
-    const int cx = (int)round(x * g.ihx) - g.nx;
-    const int cy = (int)round(y * g.ihy) - g.ny;
-    const int cz = (int)round(z * g.ihz) - g.nz;
-    CT w = lane_weight<CT>(role, cx, cy, cz, x, y, z, g, inv2s2);
-    if (normalize) {  // CpuGaussCharge.cpp:330-362, sums in j order
-      CT sx = 0, sy = 0, sz = 0;
-      for (int j = 0; j < NX; j++) sx += readlane(w, j);
-      for (int j = 0; j < NY; j++) sy += readlane(w, NX + j);
-      for (int j = 0; j < NZ; j++) sz += readlane(w, NX + NY + j);
-      const CT nrm = role.axis == 0 ? (CT)pref1 / sx : role.axis == 1 ? (CT)pref1 / sy : (CT)pref1 / sz;
-      w = w * nrm;
+  for (int c0 = s0; c0 < s1; c0 += cap) {
+    const int na = min(cap, s1 - c0);
+    // phase 0: the chunk's particles into LDS in one coalesced pass
+    for (int t = threadIdx.x; t < na; t += BS) sX[t] = sorted[c0 + t];
+    __syncthreads();
+    // phase 1: one wave per particle, lane j computes factor j
+    for (int ai = wid; ai < na; ai += nwv) {
+      const V4<CT> p = sX[ai];
+      const double x = (double)p.x, y = (double)p.y, z = (double)p.z;
+      const int cx = (int)round(x * g.ihx) - g.nx;
+      const int cy = (int)round(y * g.ihy) - g.ny;
+      const int cz = (int)round(z * g.ihz) - g.nz;
+      if (role.axis >= 0)
+        sW[(size_t)ai * NW + lane] = (double)lane_weight<CT>(role, cx, cy, cz, x, y, z, g, a.inv2s2);
+      if (lane == 0) {
+        const int ox = wrapi(cx, g.X) - bxi * b.Bx;  // in [0, Bx)
+        const int oy = wrapi(cy, g.Y) - byi * b.By;
+        const int oz = wrapi(cz, g.Z) - bzi * b.Bz;
+        sB[ai] = ox + Lx * (oy + Ly * oz);
+        sQ[ai] = ((double)p.w * a.pref) * Sb;
+      }
     }
-    const int ox = wrapi(cx, g.X) - bxi * b.Bx;  // in [0, Bx)
-    const int oy = wrapi(cy, g.Y) - byi * b.By;
-    const int oz = wrapi(cz, g.Z) - bzi * b.Bz;
-    const int base = ox + Lx * (oy + Ly * oz);
-    const double qps = ((double)p.w * pref) * S;
-
-    double wz[NZC > 0 ? NZC : 1];
-    if (NZC > 0) {
-#pragma unroll
-      for (int t = 0; t < (NZC > 0 ? NZC : 1); t++) wz[t] = (double)readlane(w, NX + NY + t);
+    __syncthreads();
+    if (a.normalize) {  // CpuGaussCharge.cpp:330-362: per-axis sums in j order, in CT
+      for (int e = threadIdx.x; e < na * 3; e += BS) {
+        const int ai = e / 3, ax = e - 3 * ai;
+        const int j0 = ax == 0 ? 0 : ax == 1 ? NX : NX + NY;
+        const int nj = ax == 0 ? NX : ax == 1 ? NY : NZ;
+        double* w = sW + (size_t)ai * NW + j0;
+        CT sum = 0;
+        for (int j = 0; j < nj; j++) sum += (CT)w[j];
+        const CT nrm = (CT)a.pref1 / sum;
+        for (int j = 0; j < nj; j++) w[j] = (double)((CT)w[j] * nrm);
+      }
+      __syncthreads();
     }
+    // phase 2: one wave per particle, lanes on (tx,ty) columns, walk tz
+    for (int ai = wid; ai < na; ai += nwv) {
+      const double* w = sW + (size_t)ai * NW;
+      const int base = sB[ai];
+      const double qps = sQ[ai];
+      double wz[NZC > 0 ? NZC : 1];
+      if (NZC > 0) {
 #pragma unroll
-    for (int k = 0; k < kMaxK; k++) {
-      if (k >= KI) break;
-      const double wxv = (double)shfl(w, k_tx[k]);
-      const double wyv = (double)shfl(w, NX + k_ty[k]);
-      const double qxy = (qps * wxv) * wyv;
-      const int a0 = base + k_off[k];
-      if (k_ok[k]) {
-        if (NZC > 0) {
+        for (int t = 0; t < (NZC > 0 ? NZC : 1); t++) wz[t] = w[NX + NY + t];
+      }
 #pragma unroll
-          for (int t = 0; t < (NZC > 0 ? NZC : 1); t++)
-            atomicAdd(&brick[a0 + t * LxLy], (unsigned long long)fixed_round(qxy * wz[t]));
-        } else {
-          for (int t = 0; t < NZ; t++) {
-            const double wzt = (double)readlane(w, NX + NY + t);
-            atomicAdd(&brick[a0 + t * LxLy], (unsigned long long)fixed_round(qxy * wzt));
+      for (int k = 0; k < kMaxK; k++) {
+        if (k >= KI) break;
+        if (k_ok[k]) {
+          // ((q pref 2^e) wx) wy: the reference's product order, scaled exactly
+          const double qxy = (qps * w[k_tx[k]]) * w[NX + k_ty[k]];
+          const int a0 = base + k_off[k];
+          if (NZC > 0) {
+#pragma unroll
+            for (int t = 0; t < (NZC > 0 ? NZC : 1); t++)
+              atomicAdd(&brick[a0 + t * LxLy], (ACC)fixed_round(qxy * wz[t]));
+          } else {
+            for (int t = 0; t < NZ; t++)
+              atomicAdd(&brick[a0 + t * LxLy], (ACC)fixed_round(qxy * w[NX + NY + t]));
           }
         }
       }
     }
+    __syncthreads();
   }
-  __syncthreads();
 
+  // flush: rows of Lx lanes (rows_per = 64/Lx per wave), no per-element
+  // division; (ly, lz) advance by the constant row stride
   const int X = g.X, Y = g.Y, Z = g.Z;
-  const int gx0 = bxi * b.Bx, gy0 = byi * b.By, gz0 = bzi * b.Bz;
-  for (int j = threadIdx.x; j < vol; j += blockDim.x) {
-    const unsigned long long v = brick[j];
-    if (v == 0ull) continue;
-    const int lx = j % Lx;
-    const int t = j / Lx;
-    const int ly = t % Ly;
-    const int lz = t / Ly;
-    int gx = gx0 + lx, gy = gy0 + ly, gz = gz0 + lz;
-    while (gx >= X) gx -= X;
-    while (gy >= Y) gy -= Y;
-    while (gz >= Z) gz -= Z;
-    atomicAdd(&grid64[gx + (size_t)X * (gy + (size_t)Y * gz)], v);
+  const int rows_per = 64 / Lx;
+  const int rl = lane / Lx;
+  const int lx = lane - rl * Lx;
+  int gx = bxi * b.Bx + lx;
+  while (gx >= X) gx -= X;
+  const int rows = Ly * Lz;
+  const int rstep = nwv * rows_per;
+  int r = wid * rows_per + rl;
+  int ly = r % Ly, lz = r / Ly;
+  for (; r < rows; r += rstep) {
+    if (rl < rows_per) {
+      const ACC raw = brick[lx + Lx * (ly + Ly * lz)];
+      // sign-extend int32 bricks and move them to the global scale 2^e
+      const unsigned long long v =
+          sizeof(ACC) == 4 ? (unsigned long long)((long long)(int)raw << shift)
+                           : (unsigned long long)raw;
+      if (v != 0ull) {
+        int gy = byi * b.By + ly, gz = bzi * b.Bz + lz;
+        while (gy >= Y) gy -= Y;
+        while (gz >= Z) gz -= Z;
+        atomicAdd(&grid64[gx + (size_t)X * (gy + (size_t)Y * gz)], v);
+      }
+    }
+    ly += rstep;
+    while (ly >= Ly) {
+      ly -= Ly;
+      lz++;
+    }
   }
 }
 
-// Fallback when even a B=1 brick does not fit in LDS: the same particle
-// loop adding straight into the global int64 grid.
-template <typename CT>
-__global__ void __launch_bounds__(256)
-    k_spread_direct(Geom g, double inv2s2, double pref, int normalize, double pref1,
-                    const int* __restrict__ starts, int nbins, const V4<CT>* __restrict__ sorted,
-                    const FixedParams* __restrict__ fp, unsigned long long* __restrict__ grid64) {
-  const int n = starts[nbins];
-  const int lane = threadIdx.x & 63;
-  const int gw = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int nwave = (gridDim.x * blockDim.x) >> 6;
-  const int NX = 2 * g.nx + 1, NY = 2 * g.ny + 1, NZ = 2 * g.nz + 1;
-  const double S = fp->scale;
-  const LaneRole role = lane_role(lane, NX, NY, NZ);
-  const bool fits = NX + NY + NZ <= 64;
-  for (int i = gw; i < n; i += nwave) {
-    const V4<CT> p = sorted[i];
-    const double x = (double)p.x, y = (double)p.y, z = (double)p.z;
-    const int cx = (int)round(x * g.ihx) - g.nx;
-    const int cy = (int)round(y * g.ihy) - g.ny;
-    const int cz = (int)round(z * g.ihz) - g.nz;
-    const double qps = ((double)p.w * pref) * S;
-    CT w = fits ? lane_weight<CT>(role, cx, cy, cz, x, y, z, g, inv2s2) : (CT)0;
-    CT sx = 1, sy = 1, sz = 1;
-    if (normalize && fits) {
-      sx = sy = sz = 0;
-      for (int j = 0; j < NX; j++) sx += readlane(w, j);
-      for (int j = 0; j < NY; j++) sy += readlane(w, NX + j);
-      for (int j = 0; j < NZ; j++) sz += readlane(w, NX + NY + j);
+// Column-owner spread: thread j owns brick column (x, y) = (j % Lx, j / Lx)
+// and keeps its Lz = Bz + 2n values in registers, as int32 (FP32 grids, per-bin
+// scale) or int64 (global scale).  All threads walk all staged particles of
+// the bin; a thread inside a particle's (x, y) footprint adds
+// round(qxy * wz[t] * 2^e) for the particle's 2n+1 z planes.  fma(a, b, M)
+// with M = 1.5*2^52 rounds the exact product to an integer held in the low
+// mantissa word, so one FMA and one integer add make one update; the
+// particle's z origin is wave-uniform, so the switch keeps every register
+// index static.  No LDS atomics; the column is then added into the global
+// int64 grid.
+constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52
+constexpr long long kMagicBits = 0x4338000000000000LL;
+
+template <typename ACC, int NZC, int BZC>
+struct ColAcc {
+  ACC v[BZC + NZC - 1];
+};
+
+template <typename ACC, int NZC, int OZ, int BZC>
+__device__ __forceinline__ void col_add(ColAcc<ACC, NZC, BZC>& c, double qxy, const double* wz) {
+#pragma unroll
+  for (int t = 0; t < NZC; t++) {
+    const double d = __builtin_fma(qxy, wz[t], kMagic);
+    if (sizeof(ACC) == 4)
+      c.v[OZ + t] += (ACC)__double2loint(d);
+    else
+      c.v[OZ + t] += (ACC)(__double_as_longlong(d) - kMagicBits);
+  }
+}
+
+template <typename CT, int NZC, int BZC, int BS, typename ACC>
+__global__ void __launch_bounds__(BS)
+    k_spread_cols(SpreadArgs a, const V4<CT>* __restrict__ xyzq, const V4<CT>* __restrict__ sorted,
+                  const int* __restrict__ counts, int* __restrict__ counts_other,
+                  const BinStats* __restrict__ stats, BinStats* __restrict__ stats_other,
+                  const int* __restrict__ ovf, unsigned long long* __restrict__ grid64) {
+  extern __shared__ __align__(32) unsigned char cols_smem[];
+  const Geom& g = a.g;
+  const BinShape& b = a.b;
+  const BinStats st = *stats;
+  const int eg = fixed_exponent(st.qmax_bits, a.n - st.invalid, a.images, a.pref_bound);
+  const double S = ldexp(1.0, eg);
+  if ((int)blockIdx.x >= a.nwg_bins) {  // overflow particles, global atomics
+    if (blockIdx.x == (unsigned)a.nwg_bins && threadIdx.x == 0) {
+      stats_other->qmax_bits = 0ull;
+      stats_other->invalid = 0;
+      stats_other->overflow = 0;
     }
-    const int wx0 = wrapi(cx, g.X), wy0 = wrapi(cy, g.Y), wz0 = wrapi(cz, g.Z);
-    for (int idx = lane; idx < NX * NY * NZ; idx += 64) {
-      const int tx = idx % NX, t2 = idx / NX, ty = t2 % NY, tz = t2 / NY;
-      CT wx, wy, wzv;
-      if (fits) {
-        wx = shfl(w, tx);
-        wy = shfl(w, NX + ty);
-        wzv = shfl(w, NX + NY + tz);
-      } else {  // very wide supports: evaluate the weights per point
-        LaneRole r;
-        r.axis = 0; r.j = tx; wx = lane_weight<CT>(r, cx, cy, cz, x, y, z, g, inv2s2);
-        r.axis = 1; r.j = ty; wy = lane_weight<CT>(r, cx, cy, cz, x, y, z, g, inv2s2);
-        r.axis = 2; r.j = tz; wzv = lane_weight<CT>(r, cx, cy, cz, x, y, z, g, inv2s2);
+    const int wg = blockIdx.x - a.nwg_bins;
+    const int gw = wg * (BS / 64) + (threadIdx.x >> 6);
+    for (int k = __builtin_amdgcn_readfirstlane(gw); k < st.overflow; k += kOverflowWG * (BS / 64))
+      spread_one_global<CT>(g, a.inv2s2, a.pref, S, a.normalize, a.pref1, xyzq[ovf[k]], grid64);
+    return;
+  }
+  const int bin = xcd_remap(blockIdx.x, a.nwg_bins);
+  if (bin >= b.nbins) return;
+  const int cnt = min(counts[bin], a.cap_bin);
+  if (threadIdx.x == 0) counts_other[bin] = 0;
+  if (cnt == 0) return;
+  int shift = 0;
+  double Sb = S;
+  if (sizeof(ACC) == 4) {  // int32 columns: per-bin exponent keeps each sum below 2^30
+    const int eb = fixed_exponent_bin(st.qmax_bits, cnt, a.pref_bound);
+    shift = eg > eb ? eg - eb : 0;
+    Sb = ldexp(S, -shift);
+  }
+  const int s0 = bin * a.cap_bin, s1 = s0 + cnt;
+  const int bxi = bin % b.nbx, byi = (bin / b.nbx) % b.nby, bzi = bin / (b.nbx * b.nby);
+  const int Lx = b.Lx, Ly = b.Ly;
+  const int NX = 2 * g.nx + 1, NY = 2 * g.ny + 1;
+  constexpr int NW_MAX = 64;
+  const int NW = NX + NY + NZC;
+  const int cap = a.cap;
+  const SpreadLds lay = spread_lds(0, cap, NW, sizeof(CT));
+  V4<CT>* sX = (V4<CT>*)(cols_smem + lay.oX);
+  double* sW = (double*)(cols_smem + lay.oW);
+  double* sQ = (double*)(cols_smem + lay.oQ);
+  int* sO = (int*)(cols_smem + lay.oB);  // packed local origin ox | oy<<10 | oz<<20
+  (void)NW_MAX;
+
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  constexpr int nwv = BS / 64;
+  LaneRole role;
+  if (lane < NX) {
+    role.axis = 0; role.j = lane;
+  } else if (lane < NX + NY) {
+    role.axis = 1; role.j = lane - NX;
+  } else if (lane < NW) {
+    role.axis = 2; role.j = lane - NX - NY;
+  } else {
+    role.axis = -1; role.j = 0;
+  }
+  const int col = threadIdx.x;
+  const bool has_col = col < Lx * Ly;
+  const int cyl = has_col ? col / Lx : 0;
+  const int cxl = has_col ? col - cyl * Lx : 0;
+  ColAcc<ACC, NZC, BZC> acc;
+#pragma unroll
+  for (int z = 0; z < BZC + NZC - 1; z++) acc.v[z] = 0;
+
+  for (int c0 = s0; c0 < s1; c0 += cap) {
+    const int na = min(cap, s1 - c0);
+    for (int t = threadIdx.x; t < na; t += BS) sX[t] = sorted[c0 + t];
+    __syncthreads();
+    for (int ai = wid; ai < na; ai += nwv) {
+      const V4<CT> p = sX[ai];
+      const double x = (double)p.x, y = (double)p.y, z = (double)p.z;
+      const int cx = (int)round(x * g.ihx) - g.nx;
+      const int cy = (int)round(y * g.ihy) - g.ny;
+      const int cz = (int)round(z * g.ihz) - g.nz;
+      if (role.axis >= 0)
+        sW[(size_t)ai * NW + lane] = (double)lane_weight<CT>(role, cx, cy, cz, x, y, z, g, a.inv2s2);
+      if (lane == 0) {
+        const int ox = wrapi(cx, g.X) - bxi * b.Bx;  // in [0, Bx)
+        const int oy = wrapi(cy, g.Y) - byi * b.By;
+        const int oz = wrapi(cz, g.Z) - bzi * b.Bz;  // in [0, Bz) <= BZC
+        sO[ai] = ox | (oy << 10) | (oz << 20);
+        sQ[ai] = ((double)p.w * a.pref) * Sb;
       }
-      if (normalize) {
-        wx = wx * ((CT)pref1 / sx);
-        wy = wy * ((CT)pref1 / sy);
-        wzv = wzv * ((CT)pref1 / sz);
+    }
+    __syncthreads();
+    if (a.normalize) {  // CpuGaussCharge.cpp:330-362: per-axis sums in j order, in CT
+      for (int e = threadIdx.x; e < na * 3; e += BS) {
+        const int ai = e / 3, ax = e - 3 * ai;
+        const int j0 = ax == 0 ? 0 : ax == 1 ? NX : NX + NY;
+        const int nj = ax == 0 ? NX : ax == 1 ? NY : NZC;
+        double* w = sW + (size_t)ai * NW + j0;
+        CT sum = 0;
+        for (int j = 0; j < nj; j++) sum += (CT)w[j];
+        const CT nrm = (CT)a.pref1 / sum;
+        for (int j = 0; j < nj; j++) w[j] = (double)((CT)w[j] * nrm);
       }
-      const double v = ((qps * (double)wx) * (double)wy) * (double)wzv;
-      const int gx = (wx0 + tx) % g.X, gy = (wy0 + ty) % g.Y, gz = (wz0 + tz) % g.Z;
-      atomicAdd(&grid64[gx + (size_t)g.X * (gy + (size_t)g.Y * gz)],
-                (unsigned long long)fixed_round(v));
+      __syncthreads();
+    }
+    for (int ai = 0; ai < na; ai++) {
+      const int o = sO[ai];
+      const int tx = cxl - (o & 1023);
+      const int ty = cyl - ((o >> 10) & 1023);
+      if (has_col && (unsigned)tx < (unsigned)NX && (unsigned)ty < (unsigned)NY) {
+        const double* w = sW + (size_t)ai * NW;
+        // ((q pref 2^e) wx) wy: the reference's product order, scaled exactly
+        const double qxy = (sQ[ai] * w[tx]) * w[NX + ty];
+        double wz[NZC];
+#pragma unroll
+        for (int t = 0; t < NZC; t++) wz[t] = w[NX + NY + t];
+        switch (o >> 20) {
+#define GSE_COL_CASE(OZ) \
+  case OZ:               \
+    if (OZ < BZC) col_add<ACC, NZC, (OZ < BZC ? OZ : 0), BZC>(acc, qxy, wz); \
+    break;
+          GSE_COL_CASE(0)
+          GSE_COL_CASE(1)
+          GSE_COL_CASE(2)
+          GSE_COL_CASE(3)
+          GSE_COL_CASE(4)
+          GSE_COL_CASE(5)
+          GSE_COL_CASE(6)
+          GSE_COL_CASE(7)
+#undef GSE_COL_CASE
+          default:
+            break;
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  if (has_col) {
+    const int X = g.X, Y = g.Y, Z = g.Z;
+    int gx = bxi * b.Bx + cxl, gy = byi * b.By + cyl;
+    while (gx >= X) gx -= X;
+    while (gy >= Y) gy -= Y;
+    int gz = bzi * b.Bz;
+    unsigned long long* colp = grid64 + gx + (size_t)X * gy;
+    const size_t plane = (size_t)X * Y;
+    const int Lz = b.Bz + NZC - 1;
+#pragma unroll
+    for (int z = 0; z < BZC + NZC - 1; z++) {
+      if (z < Lz) {
+        const unsigned long long v =
+            sizeof(ACC) == 4 ? (unsigned long long)((long long)(int)acc.v[z] << shift)
+                             : (unsigned long long)acc.v[z];
+        int zz = gz + z;
+        while (zz >= Z) zz -= Z;
+        if (v != 0ull) atomicAdd(&colp[(size_t)zz * plane], v);
+      }
     }
   }
+}
+
+// Supports too wide for the LDS path: all binned particles (one bin of
+// capacity n) through the global path, one wave per particle.
+template <typename CT>
+__global__ void __launch_bounds__(256)
+    k_spread_direct(SpreadArgs a, const V4<CT>* __restrict__ sorted, const int* __restrict__ counts,
+                    int* __restrict__ counts_other, const BinStats* __restrict__ stats,
+                    BinStats* __restrict__ stats_other, unsigned long long* __restrict__ grid64) {
+  const BinStats st = *stats;
+  const double S = ldexp(1.0, fixed_exponent(st.qmax_bits, a.n - st.invalid, a.images,
+                                             a.pref_bound));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    counts_other[0] = 0;
+    stats_other->qmax_bits = 0ull;
+    stats_other->invalid = 0;
+    stats_other->overflow = 0;
+  }
+  const int n = min(counts[0], a.cap_bin);
+  const int gw = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int nwave = (gridDim.x * blockDim.x) >> 6;
+  for (int i = gw; i < n; i += nwave)
+    spread_one_global<CT>(a.g, a.inv2s2, a.pref, S, a.normalize, a.pref1, sorted[i], grid64);
 }
 
 template <typename T>
 __global__ void k_fixed_to_grid(unsigned long long* __restrict__ g64, T* __restrict__ out,
-                                size_t count, const FixedParams* __restrict__ fp, int clear) {
-  const double inv = fp->inv_scale;
+                                size_t count, const BinStats* __restrict__ stats, int n,
+                                int images, double pref_bound, int clear) {
+  const BinStats st = *stats;
+  const double inv = ldexp(1.0, -fixed_exponent(st.qmax_bits, n - st.invalid, images, pref_bound));
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count;
        i += (size_t)gridDim.x * blockDim.x) {
     const long long v = (long long)g64[i];
@@ -409,31 +654,36 @@ Geom make_geom(int X, int Y, int Z, double bx, double by, double bz, double rcut
   return g;
 }
 
+// The LDS path needs the 3(2n+1) factors in one wave, at most kMaxK column
+// passes, brick rows no longer than a wave and a brick within the budget.
 BinShape choose_bins(const Geom& g, size_t elem, size_t budget, int pref_edge) {
   BinShape b;
+  const int NX = 2 * g.nx + 1, NY = 2 * g.ny + 1, NZ = 2 * g.nz + 1;
+  const bool lanes_ok = NX + NY + NZ <= 64 && (NX * NY + 63) / 64 <= kMaxK;
   int B[3] = {pref_edge, pref_edge, pref_edge};
   const int M[3] = {g.X, g.Y, g.Z};
   const int n[3] = {g.nx, g.ny, g.nz};
-  for (int a = 0; a < 3; a++)
+  for (int a = 0; a < 3; a++) {
     if (B[a] > M[a]) B[a] = M[a];
+    if (B[a] + 2 * n[a] > 64) B[a] = 64 - 2 * n[a];
+    if (B[a] < 1) B[a] = 0;
+  }
+  bool fits = lanes_ok && B[0] > 0 && B[1] > 0 && B[2] > 0;
   // shrink the largest edge until the brick fits
-  for (;;) {
+  while (fits) {
     size_t bytes = (size_t)(B[0] + 2 * n[0]) * (B[1] + 2 * n[1]) * (B[2] + 2 * n[2]) * elem;
     if (bytes <= budget) break;
     int a = 0;
     for (int k = 1; k < 3; k++)
       if (B[k] > B[a]) a = k;
     if (B[a] == 1) {
-      B[0] = B[1] = B[2] = 0;  // does not fit at all
+      fits = false;
       break;
     }
     B[a] = (B[a] + 1) / 2;
   }
-  b.Bx = B[0];
-  b.By = B[1];
-  b.Bz = B[2];
   b.direct = 0;
-  if (B[0] == 0) {  // one bin holding every particle
+  if (!fits) {  // one bin holding every particle, global-memory kernels
     b.Bx = g.X;
     b.By = g.Y;
     b.Bz = g.Z;
@@ -443,6 +693,9 @@ BinShape choose_bins(const Geom& g, size_t elem, size_t budget, int pref_edge) {
     b.direct = 1;
     return b;
   }
+  b.Bx = B[0];
+  b.By = B[1];
+  b.Bz = B[2];
   b.nbx = (g.X + B[0] - 1) / B[0];
   b.nby = (g.Y + B[1] - 1) / B[1];
   b.nbz = (g.Z + B[2] - 1) / B[2];
@@ -453,28 +706,21 @@ BinShape choose_bins(const Geom& g, size_t elem, size_t budget, int pref_edge) {
   return b;
 }
 
-hipError_t launch_bin(bool dbl, const Geom& g, const BinShape& b, int n, const void* xyzq,
-                      Binning& w, int images, double pref, hipStream_t s) {
-  k_bin_reset<<<grid_for(b.nbins + 1, 256), 256, 0, s>>>(w.counts, b.nbins + 1, w.fp);
-  if (n > 0) {
-    const unsigned grid = (unsigned)((n + 255) / 256);
+hipError_t launch_bin_clear(int* counts, int nbins, BinStats* stats, hipStream_t s) {
+  k_bin_clear<<<grid_for(nbins, 256), 256, 0, s>>>(counts, nbins, stats);
+  return hipGetLastError();
+}
+
+hipError_t launch_bin(bool dbl, const Geom& g, const BinShape& b, Binning& w, hipStream_t s) {
+  if (w.n > 0) {
+    const unsigned grid = (unsigned)((w.n + 255) / 256);
     if (dbl)
-      k_bin_count<double><<<grid, 256, 0, s>>>(g, b, n, (const V4<double>*)xyzq, w.counts,
-                                               w.atom_bin, w.atom_slot, w.fp);
+      k_bin<double><<<grid, 256, 0, s>>>(g, b, w.n, w.cap_bin, (const V4<double>*)w.xyzq,
+                                         w.counts, w.stats, w.ovf, w.perm,
+                                         (V4<double>*)w.sorted);
     else
-      k_bin_count<float><<<grid, 256, 0, s>>>(g, b, n, (const V4<float>*)xyzq, w.counts,
-                                              w.atom_bin, w.atom_slot, w.fp);
-  }
-  k_bin_scan<<<1, 1024, 0, s>>>(w.counts, b.nbins, n, images, pref, w.fp);
-  if (n > 0) {
-    const unsigned grid = (unsigned)((n + 255) / 256);
-    if (dbl)
-      k_bin_scatter<double><<<grid, 256, 0, s>>>(n, (const V4<double>*)xyzq, w.counts,
-                                                 w.atom_bin, w.atom_slot, w.perm,
-                                                 (V4<double>*)w.sorted);
-    else
-      k_bin_scatter<float><<<grid, 256, 0, s>>>(n, (const V4<float>*)xyzq, w.counts, w.atom_bin,
-                                                w.atom_slot, w.perm, (V4<float>*)w.sorted);
+      k_bin<float><<<grid, 256, 0, s>>>(g, b, w.n, w.cap_bin, (const V4<float>*)w.xyzq, w.counts,
+                                        w.stats, w.ovf, w.perm, (V4<float>*)w.sorted);
   }
   return hipGetLastError();
 }
@@ -482,38 +728,109 @@ hipError_t launch_bin(bool dbl, const Geom& g, const BinShape& b, int n, const v
 template <typename CT>
 static hipError_t spread_t(const Geom& g, const BinShape& b, double sigma, bool normalize,
                            Binning& w, unsigned long long* grid64, hipStream_t s) {
-  const double inv2s2 = 1.0 / (2.0 * sigma * sigma);
-  const double pref = pow(2.0 * M_PI * sigma * sigma, -3.0 / 2.0);
-  const double pref1 = sqrt(2.0 * M_PI * sigma * sigma);
+  SpreadArgs a;
+  a.g = g;
+  a.b = b;
+  a.inv2s2 = 1.0 / (2.0 * sigma * sigma);
+  a.pref = pow(2.0 * M_PI * sigma * sigma, -3.0 / 2.0);
+  a.pref1 = sqrt(2.0 * M_PI * sigma * sigma);
+  a.normalize = normalize ? 1 : 0;
+  a.cap_bin = w.cap_bin;
+  a.n = w.n;
+  a.images = w.images;
+  a.pref_bound = w.pref_bound;
   const int NX = 2 * g.nx + 1, NY = 2 * g.ny + 1, NZ = 2 * g.nz + 1;
-  const bool lane_ok = NX + NY + NZ <= 64 && (NX * NY + 63) / 64 <= kMaxK;
-  if (b.direct || !lane_ok) {
-    k_spread_direct<CT><<<2048, 256, 0, s>>>(g, inv2s2, pref, normalize ? 1 : 0, pref1,
-                                             w.counts, b.nbins, (const V4<CT>*)w.sorted, w.fp,
-                                             grid64);
+  if (b.direct) {
+    a.cap = 0;
+    a.nwg_bins = 1;
+    k_spread_direct<CT><<<2048, 256, 0, s>>>(a, (const V4<CT>*)w.sorted, w.counts, w.counts_other,
+                                             w.stats, w.stats_other, grid64);
     return hipGetLastError();
   }
-  const size_t lds = b.brick_bytes(8);
-  const unsigned nwg = (unsigned)(((b.nbins + 7) / 8) * 8);
-  const V4<CT>* srt = (const V4<CT>*)w.sorted;
-  const int nz = NZ;
-#define GSE_SPREAD_CASE(NZV)                                                                 \
-  case NZV:                                                                                  \
-    k_spread<CT, NZV><<<nwg, 256, lds, s>>>(g, b, inv2s2, pref, normalize ? 1 : 0, pref1,   \
-                                             srt, w.counts, w.fp, grid64);                   \
-    break;
-  switch (nz) {
-    GSE_SPREAD_CASE(7)
-    GSE_SPREAD_CASE(9)
-    GSE_SPREAD_CASE(11)
-    GSE_SPREAD_CASE(13)
-    GSE_SPREAD_CASE(15)
-    GSE_SPREAD_CASE(17)
-    default:
-      k_spread<CT, 0><<<nwg, 256, lds, s>>>(g, b, inv2s2, pref, normalize ? 1 : 0, pref1, srt,
-                                            w.counts, w.fp, grid64);
+  const Tuning& tu = tuning();
+  if (tu.spread_cols && b.Bz <= 8 && b.Lx * b.Ly <= 1024 && (NZ == 11 || NZ == 13 || NZ == 15 ||
+                                                            NZ == 17)) {
+    const bool acc32 = sizeof(CT) == 4 && tu.spread_lds32;
+    int cap = tu.spread_stage / (int)(4 * sizeof(CT) + 8 * (NX + NY + NZ) + 8 + 4);
+    if (cap > 256) cap = 256;
+    if (cap < 8) cap = 8;
+    a.cap = cap;
+    const size_t lds = spread_lds(0, cap, NX + NY + NZ, sizeof(CT)).total;
+    a.nwg_bins = ((b.nbins + 7) / 8) * 8;
+    const unsigned nwg = (unsigned)(a.nwg_bins + kOverflowWG);
+    const int cols = b.Lx * b.Ly;
+    const int bs = cols <= 256 ? 256 : cols <= 512 ? 512 : 1024;
+    const V4<CT>* xq = (const V4<CT>*)w.xyzq;
+    const V4<CT>* srt = (const V4<CT>*)w.sorted;
+#define GSE_COLS_LAUNCH(NZV, BSV, ACCT)                                                       \
+  k_spread_cols<CT, NZV, 8, BSV, ACCT><<<nwg, BSV, lds, s>>>(a, xq, srt, w.counts,            \
+                                                             w.counts_other, w.stats,         \
+                                                             w.stats_other, w.ovf, grid64)
+#define GSE_COLS_BS(NZV, ACCT)            \
+  if (bs == 256)                          \
+    GSE_COLS_LAUNCH(NZV, 256, ACCT);      \
+  else if (bs == 512)                     \
+    GSE_COLS_LAUNCH(NZV, 512, ACCT);      \
+  else                                    \
+    GSE_COLS_LAUNCH(NZV, 1024, ACCT);
+#define GSE_COLS_NZ(ACCT)                      \
+  switch (NZ) {                                \
+    case 11: GSE_COLS_BS(11, ACCT) break;      \
+    case 13: GSE_COLS_BS(13, ACCT) break;      \
+    case 15: GSE_COLS_BS(15, ACCT) break;      \
+    default: GSE_COLS_BS(17, ACCT) break;      \
   }
-#undef GSE_SPREAD_CASE
+    if (acc32) {
+      GSE_COLS_NZ(unsigned)
+    } else {
+      GSE_COLS_NZ(unsigned long long)
+    }
+#undef GSE_COLS_NZ
+#undef GSE_COLS_BS
+#undef GSE_COLS_LAUNCH
+    return hipGetLastError();
+  }
+  // stage as many particles as fit beside the brick in the workgroup's LDS
+  const bool acc32 = sizeof(CT) == 4 && tu.spread_lds32;
+  const int vol = b.Lx * b.Ly * b.Lz;
+  const int vol_eff = vol * (acc32 ? 4 : 8) / 8 + 1;  // brick size in 8-byte units
+  const size_t lds_total = (size_t)vol_eff * 8 + (size_t)tu.spread_stage;
+  int cap = 256;
+  while (cap > 1 && spread_lds(vol_eff, cap, NX + NY + NZ, sizeof(CT)).total > lds_total) cap--;
+  a.cap = cap;
+  const size_t lds = spread_lds(vol_eff, cap, NX + NY + NZ, sizeof(CT)).total;
+  if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
+  a.nwg_bins = ((b.nbins + 7) / 8) * 8;
+  const unsigned nwg = (unsigned)(a.nwg_bins + kOverflowWG);
+  const V4<CT>* xq = (const V4<CT>*)w.xyzq;
+  const V4<CT>* srt = (const V4<CT>*)w.sorted;
+  const int bs = tu.spread_block ? tu.spread_block : (sizeof(CT) == 4 ? 512 : 256);
+#define GSE_SPREAD_LAUNCH(NZV, BSV, ACCT)                                                    \
+  k_spread<CT, NZV, BSV, ACCT><<<nwg, BSV, lds, s>>>(a, xq, srt, w.counts, w.counts_other,  \
+                                                     w.stats, w.stats_other, w.ovf, grid64)
+#define GSE_SPREAD_BS(NZV, ACCT)                      \
+  if (bs == 512)                                      \
+    GSE_SPREAD_LAUNCH(NZV, 512, ACCT);                \
+  else if (bs == 1024)                                \
+    GSE_SPREAD_LAUNCH(NZV, 1024, ACCT);               \
+  else                                                \
+    GSE_SPREAD_LAUNCH(NZV, 256, ACCT);
+#define GSE_SPREAD_NZ(ACCT)                           \
+  switch (NZ) {                                       \
+    case 11: GSE_SPREAD_BS(11, ACCT) break;           \
+    case 13: GSE_SPREAD_BS(13, ACCT) break;           \
+    case 15: GSE_SPREAD_BS(15, ACCT) break;           \
+    case 17: GSE_SPREAD_BS(17, ACCT) break;           \
+    default: GSE_SPREAD_BS(0, ACCT) break;            \
+  }
+  if (acc32) {
+    GSE_SPREAD_NZ(unsigned)
+  } else {
+    GSE_SPREAD_NZ(unsigned long long)
+  }
+#undef GSE_SPREAD_NZ
+#undef GSE_SPREAD_BS
+#undef GSE_SPREAD_LAUNCH
   return hipGetLastError();
 }
 
@@ -536,14 +853,16 @@ double spread_weight_bound(const Geom& g, double sigma, bool normalize) {
 }
 
 hipError_t launch_fixed_to_grid(bool dbl, const unsigned long long* grid64, void* out,
-                                size_t count, const FixedParams* fp, bool clear, hipStream_t s) {
+                                size_t count, const Binning& w, bool clear, hipStream_t s) {
   const unsigned grid = grid_for(count, 256, 8192);
   if (dbl)
     k_fixed_to_grid<double><<<grid, 256, 0, s>>>((unsigned long long*)grid64, (double*)out,
-                                                 count, fp, clear ? 1 : 0);
+                                                 count, w.stats, w.n, w.images, w.pref_bound,
+                                                 clear ? 1 : 0);
   else
     k_fixed_to_grid<float><<<grid, 256, 0, s>>>((unsigned long long*)grid64, (float*)out, count,
-                                                fp, clear ? 1 : 0);
+                                                w.stats, w.n, w.images, w.pref_bound,
+                                                clear ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -57,6 +57,18 @@ GSE_API int gse_version(void);
  * gather; 0 keeps the built-in choice.  Process-wide. */
 GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
 
+/* Named launch-shape knobs, process-wide (the autotuner's interface):
+ *   bin_edge      preferred bin edge in grid cells            (1..64, 8)
+ *   brick_budget  LDS bytes allowed for an int64 brick        (64 KiB)
+ *   spread_block  threads per spread workgroup, 0 = automatic (0,256,512,1024)
+ *   gather_block  threads per gather workgroup, 0 = automatic (0,256,512)
+ *   spread_stage  LDS bytes for staged particles in the spread (16 KiB)
+ *   gather_stage  LDS bytes for staged particles in the gather (16 KiB)
+ *   spread_lds32  FP32 grids: 32-bit accumulators, per-bin scale (0/1, 1)
+ *   spread_cols   column-owner register spread (0/1, 1); 0 = LDS-atomic brick */
+GSE_API gse_status_t gse_set_tuning_param(const char* name, int value);
+GSE_API gse_status_t gse_get_tuning_param(const char* name, int* value);
+
 /* ------------------------------------------------------------------------
  * CpuGaussCharge<AT,CT>  (CpuGaussCharge.h:11-49)
  * ---------------------------------------------------------------------- */
