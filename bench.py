@@ -25,6 +25,15 @@ sys.path.insert(0, ROOT)
 METRIC = "GSE energy+force steps/s (48k atoms, M=80) + HBM roofline frac; batched systems/s"
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch measured by two rocprofv3 --pmc passes (FETCH_SIZE,
+# WRITE_SIZE), scaled by the same counters read for a 256 MiB device copy
+# (FETCH x2.000, WRITE x1.000); the commands and per-kernel table are in
+# profiles/k_gather_k_spread_water48k_M80_fp32_hbm_traffic.md.
+MEASURED_TRAFFIC = {
+    ("water_48k_M80", "fp32", "gather"): 8582531,
+    ("water_48k_M80", "fp32", "spread"): 56767795,
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -100,7 +109,8 @@ def main():
     dom = max(ours, key=ours.get)
     achieved = sb[dom] / (ours[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": MEASURED_TRAFFIC.get((args.workload, args.precision, dom)),
             "bytes_per_launch": sb[dom], "avg_ms": round(ours[dom], 5)}
 
     # accuracy against the FP64 oracle and the CPU baseline (oracle timing)
