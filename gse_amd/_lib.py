@@ -48,6 +48,10 @@ _SIGS = {
     "gse_set_tiling": [I, I],
     "gse_set_tuning_param": [ctypes.c_char_p, I],
     "gse_get_tuning_param": [ctypes.c_char_p, PI],
+    "gse_device_malloc": [PV, ctypes.c_size_t],
+    "gse_device_free": [V],
+    "gse_memcpy_to_device": [V, V, ctypes.c_size_t],
+    "gse_memcpy_to_host": [V, V, ctypes.c_size_t],
     "gse_gauss_charge_create": [PV, I, I, I, I, I],
     "gse_gauss_charge_destroy": [V],
     "gse_gauss_charge_set_stream": [V, V],

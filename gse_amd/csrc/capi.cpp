@@ -524,6 +524,33 @@ gse_status_t gse_get_tuning_param(const char* name, int* value) {
   return GSE_SUCCESS;
 }
 
+// ------------------------------- device memory -----------------------------
+gse_status_t gse_device_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return fail(GSE_ERROR_INVALID_VALUE, "ptr is NULL");
+  *ptr = nullptr;
+  HIPCHK(hipMalloc(ptr, bytes ? bytes : 1));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_device_free(void* ptr) {
+  if (ptr) HIPCHK(hipFree(ptr));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_memcpy_to_device(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return GSE_SUCCESS;
+  if (!dst || !src) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_memcpy_to_host(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return GSE_SUCCESS;
+  if (!dst || !src) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return GSE_SUCCESS;
+}
+
 // ------------------------------- CpuGaussCharge ----------------------------
 gse_status_t gse_gauss_charge_create(gse_gauss_charge_t* out, gse_dtype_t at, gse_dtype_t ct,
                                      int X, int Y, int Z) {

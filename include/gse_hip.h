@@ -69,6 +69,15 @@ GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
 GSE_API gse_status_t gse_set_tuning_param(const char* name, int value);
 GSE_API gse_status_t gse_get_tuning_param(const char* name, int* value);
 
+/* Device memory for callers that hold host buffers (the C++ drop-in classes
+ * of include/gse/gse.hpp use these; no reference counterpart: the reference
+ * CPU classes own host memory, CpuGrid.h:21-24).  The copies are synchronous
+ * and order after all work on the null stream. */
+GSE_API gse_status_t gse_device_malloc(void** ptr, size_t bytes);
+GSE_API gse_status_t gse_device_free(void* ptr);
+GSE_API gse_status_t gse_memcpy_to_device(void* dst, const void* src, size_t bytes);
+GSE_API gse_status_t gse_memcpy_to_host(void* dst, const void* src, size_t bytes);
+
 /* ------------------------------------------------------------------------
  * CpuGaussCharge<AT,CT>  (CpuGaussCharge.h:11-49)
  * ---------------------------------------------------------------------- */
