@@ -71,6 +71,37 @@ def time_steps(fn, steps, warmup, torch):
     return (time.perf_counter() - t0) / steps
 
 
+def bench_batched(api, systems, torch, dev, nsys=256):
+    """256 independent water boxes (600..8000 charges, M in {24, 32, 40}),
+    FP32: the batched step (autotuned, replayed from a HIP graph) against
+    looping the plain single-system step over the same systems."""
+    bs = systems.batched_systems(nsys, seed=4)
+    xs = [torch.from_numpy(x).to(device=dev, dtype=torch.float32).contiguous() for x, _, _ in bs]
+    xc = torch.cat(xs).contiguous()
+    batch = api.GSEBatch(torch.float32, torch.float32,
+                         [((M, M, M), (L, L, L), len(x)) for x, L, M in bs], 2.12)
+    cfg, tuned_ms = batch.autotune(xc, reps=3)
+    ob = {}
+    t_graph = time_steps(lambda: batch.step(xc, force=True, graph=True, out=ob), 50, 5, torch)
+    t_eager = time_steps(lambda: batch.step(xc, force=True, out=ob), 50, 5, torch)
+    handles = [api.CpuGSEk(torch.float32, torch.float32, M, M, M, 2.12, 0.4, 3.0, 3.0, L, L, L)
+               for _, L, M in bs]
+    outs = [{} for _ in bs]
+
+    def loop():
+        for h, x, o in zip(handles, xs, outs):
+            h.step(x, force=True, potential=False, out=o)
+
+    t_loop = time_steps(loop, 10, 2, torch)
+    return {"systems": nsys, "charges": int(xc.shape[0]),
+            "systems_per_s_graph": round(nsys / t_graph, 1),
+            "systems_per_s_eager": round(nsys / t_eager, 1),
+            "systems_per_s_looped": round(nsys / t_loop, 1),
+            "speedup_graph_vs_looped": round(t_loop / t_graph, 3),
+            "ms_per_batched_step": round(t_graph * 1e3, 4),
+            "autotuned": cfg}
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -168,6 +199,7 @@ def main():
         # conv + gather roofline as the survey defines it (mode R includes the conv)
         extras["stage_roofline_frac"] = {
             k: round(sb[k] / (ours[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 5) for k in sb}
+        extras["batched"] = bench_batched(api, systems, torch, dev)
         line["extras"] = extras
 
     print(json.dumps(line), flush=True)

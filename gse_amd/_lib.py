@@ -99,6 +99,9 @@ _SIGS = {
     "gse_batch_set_stream": [V, V],
     "gse_batch_step": [V, V, V, V, V, V, V],
     "gse_batch_step_graph": [V, V, V, V, V, V, V],
+    "gse_batch_set_config": [V, I, I, I],
+    "gse_batch_get_config": [V, PI],
+    "gse_batch_autotune": [V, V, I, PD],
 }
 _RET = {"gse_status_string": ctypes.c_char_p, "gse_last_error": ctypes.c_char_p,
         "gse_version": I}

@@ -447,3 +447,85 @@ def get_tuning():
         check(L.gse_get_tuning_param(k.encode(), ctypes.byref(v)))
         out[k] = v.value
     return out
+
+
+class GSEBatch:
+    """Many independent GSE-k systems stepped together (include/gse_hip.h,
+    gse_batch_*): one launch per stage for all systems, optionally replayed
+    from a HIP graph.  systems: list of (M or (X, Y, Z), L or (bx, by, bz),
+    n_particles); particles of all systems are concatenated in list order."""
+
+    def __init__(self, at, ct, systems, sigma, kappa=0.4, lambdaSigma=3.0, lambdaSigma1=3.0,
+                 ccelec=332.0716):
+        L = _lib.load()
+        self._L = L
+        self.at, self.ct = at, ct
+        n = len(systems)
+        sizes = (ctypes.c_int * (3 * n))()
+        boxes = (ctypes.c_double * (3 * n))()
+        counts = (ctypes.c_int * n)()
+        self.counts = []
+        for k, (M, box, cnt) in enumerate(systems):
+            d = _dims(M)
+            b = _box(box)
+            for c in range(3):
+                sizes[3 * k + c] = d[c]
+                boxes[3 * k + c] = b[c]
+            counts[k] = int(cnt)
+            self.counts.append(int(cnt))
+        self.nsys = n
+        self.ntot = sum(self.counts)
+        h = ctypes.c_void_p()
+        check(L.gse_batch_create(ctypes.byref(h), _code(at), _code(ct), n, sizes, boxes, counts,
+                                 float(sigma), float(kappa), float(lambdaSigma),
+                                 float(lambdaSigma1), float(ccelec)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.gse_batch_destroy(self._h)
+            self._h = None
+
+    def _s(self):
+        check(self._L.gse_batch_set_stream(self._h, _stream()))
+
+    def step(self, xyzq, force=True, potential=False, graph=False, out=None):
+        """One batched step; returns dict with force (3, N), pot (N,) and
+        energy (nsys,) float64, all on the device."""
+        _check_xyzq(xyzq, self.ct)
+        if xyzq.shape[0] != self.ntot:
+            raise ValueError(f"xyzq must hold {self.ntot} particles")
+        o = {} if out is None else out
+        dev = xyzq.device
+        if force and "force" not in o:
+            o["force"] = torch.empty((3, self.ntot), dtype=self.at, device=dev)
+        if potential and "pot" not in o:
+            o["pot"] = torch.empty(self.ntot, dtype=self.at, device=dev)
+        if "energy" not in o:
+            o["energy"] = torch.empty(self.nsys, dtype=torch.float64, device=dev)
+        f = o.get("force") if force else None
+        args = (_ptr(xyzq), _ptr(f[0]) if f is not None else None,
+                _ptr(f[1]) if f is not None else None, _ptr(f[2]) if f is not None else None,
+                _ptr(o.get("pot")) if potential else None, _ptr(o["energy"]))
+        self._s()
+        fn = self._L.gse_batch_step_graph if graph else self._L.gse_batch_step
+        check(fn(self._h, *args))
+        return o
+
+    def set_config(self, bin_edge=8, spread_block=0, gather_block=0):
+        check(self._L.gse_batch_set_config(self._h, int(bin_edge), int(spread_block),
+                                           int(gather_block)))
+
+    def get_config(self):
+        c = (ctypes.c_int * 3)()
+        check(self._L.gse_batch_get_config(self._h, c))
+        return dict(bin_edge=c[0], spread_block=c[1], gather_block=c[2])
+
+    def autotune(self, xyzq, reps=3):
+        """Times each candidate launch shape on xyzq and keeps the fastest;
+        returns (config, ms per step)."""
+        _check_xyzq(xyzq, self.ct)
+        ms = ctypes.c_double()
+        self._s()
+        check(self._L.gse_batch_autotune(self._h, _ptr(xyzq), int(reps), ctypes.byref(ms)))
+        return self.get_config(), ms.value

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <rocfft/rocfft.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -256,7 +257,9 @@ struct FFT3 {
   DevBuf work;
   size_t work_f = 0, work_b = 0;
 
-  gse_status_t init(bool d, int x, int y, int z) {
+  // batch > 1: that many transforms of packed, back-to-back grids (distance
+  // X*Y*Z reals) and half spectra (distance (X/2+1)*Y*Z complex)
+  gse_status_t init(bool d, int x, int y, int z, int batch = 1) {
     dbl = d;
     X = x;
     Y = y;
@@ -264,12 +267,39 @@ struct FFT3 {
     std::call_once(g_rocfft_once, [] { rocfft_setup(); });
     const size_t lengths[3] = {(size_t)X, (size_t)Y, (size_t)Z};
     const rocfft_precision prec = dbl ? rocfft_precision_double : rocfft_precision_single;
-    if (rocfft_plan_create(&fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
-                           prec, 3, lengths, 1, nullptr) != rocfft_status_success)
-      return fail(GSE_ERROR_FFT, "rocfft_plan_create (r2c %dx%dx%d) failed", X, Y, Z);
-    if (rocfft_plan_create(&bwd, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
-                           prec, 3, lengths, 1, nullptr) != rocfft_status_success)
-      return fail(GSE_ERROR_FFT, "rocfft_plan_create (c2r %dx%dx%d) failed", X, Y, Z);
+    rocfft_plan_description df = nullptr, db = nullptr;
+    if (batch > 1) {
+      const size_t Xh = (size_t)X / 2 + 1;
+      const size_t rs[3] = {1, (size_t)X, (size_t)X * Y};
+      const size_t cs[3] = {1, Xh, Xh * Y};
+      const size_t rd = (size_t)X * Y * Z, cd = Xh * Y * Z;
+      if (rocfft_plan_description_create(&df) != rocfft_status_success ||
+          rocfft_plan_description_create(&db) != rocfft_status_success ||
+          rocfft_plan_description_set_data_layout(df, rocfft_array_type_real,
+                                                  rocfft_array_type_hermitian_interleaved,
+                                                  nullptr, nullptr, 3, rs, rd, 3, cs, cd) !=
+              rocfft_status_success ||
+          rocfft_plan_description_set_data_layout(db, rocfft_array_type_hermitian_interleaved,
+                                                  rocfft_array_type_real, nullptr, nullptr, 3,
+                                                  cs, cd, 3, rs, rd) != rocfft_status_success) {
+        if (df) rocfft_plan_description_destroy(df);
+        if (db) rocfft_plan_description_destroy(db);
+        return fail(GSE_ERROR_FFT, "rocfft plan description (%dx%dx%d x %d) failed", X, Y, Z,
+                    batch);
+      }
+    }
+    const rocfft_status sf =
+        rocfft_plan_create(&fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                           prec, 3, lengths, (size_t)batch, df);
+    const rocfft_status sb =
+        rocfft_plan_create(&bwd, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                           prec, 3, lengths, (size_t)batch, db);
+    if (df) rocfft_plan_description_destroy(df);
+    if (db) rocfft_plan_description_destroy(db);
+    if (sf != rocfft_status_success)
+      return fail(GSE_ERROR_FFT, "rocfft_plan_create (r2c %dx%dx%d x %d) failed", X, Y, Z, batch);
+    if (sb != rocfft_status_success)
+      return fail(GSE_ERROR_FFT, "rocfft_plan_create (c2r %dx%dx%d x %d) failed", X, Y, Z, batch);
     rocfft_plan_get_work_buffer_size(fwd, &work_f);
     rocfft_plan_get_work_buffer_size(bwd, &work_b);
     const size_t w = work_f > work_b ? work_f : work_b;
@@ -421,6 +451,72 @@ struct gse_gsek_s {
   size_t ev_used = 0;
   double rcut1() const { return ls1 * std::sqrt(2.0) * sigma1; }
   double rcut2() const { return ls1 * std::sqrt(2.0) * sigma2; }
+};
+
+// Batched independent systems: one layout of all systems' bins, grids and
+// spectra, one launch per stage (per bin-shape group for spread and gather,
+// per grid size for the FFTs), optionally replayed from a HIP graph.
+struct BatchGroup {  // systems sharing 2nz+1: one spread and one gather launch
+  int nz = 0;
+  int item0 = 0, nitems = 0;
+  size_t lds_spread = 0, lds_gather = 0;
+  int grid = 0;
+};
+
+struct BatchFFT {  // systems with equal grid dimensions, adjacent in the layout
+  FFT3 fft;
+  long long gofs = 0, kofs = 0;
+};
+
+struct gse_batch_s {
+  bool at_dbl = false, ct_dbl = false;
+  int nsys = 0;
+  double sigma = 0, kappa = 0, ls = 0, ls1 = 0, ccelec = 332.0716;
+  double sigma1 = 0, sigma2 = 0, rc1 = 0;
+  std::vector<int> dims, counts;
+  std::vector<double> boxes;
+  std::vector<int> order;  // internal order of systems (equal grids adjacent)
+  int ntot = 0;
+  // launch shape (set by gse_batch_set_config / the autotuner)
+  int bin_edge = 8, spread_block = 0, gather_block = 0;
+  // layout
+  std::vector<BatchSys> sys;
+  std::vector<BatchGroup> groups;
+  std::vector<BatchFFT*> ffts;
+  std::vector<int2> items;
+  std::vector<BatchChunk> cchunks, kchunks;
+  long long total_points = 0, total_k = 0, total_slots = 0;
+  int total_bins = 0;
+  DevBuf d_sys, d_pofs, d_items, d_cchunks, d_kchunks, counts_b, stats_b, ovf_b, novf_b, perm_b,
+      sorted_b, grid64_b, rho_b, phi_b, spec_b, table_b, part_b;
+  BatchDev dev;
+  hipStream_t stream = nullptr;
+  // graph of one step
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  const void* g_ptrs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+
+  void release_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+  }
+  void release_ffts() {
+    for (BatchFFT* f : ffts) {
+      f->fft.release();
+      delete f;
+    }
+    ffts.clear();
+  }
+  void release() {
+    release_graph();
+    release_ffts();
+    for (DevBuf* b : {&d_sys, &d_pofs, &d_items, &d_cchunks, &d_kchunks, &counts_b, &stats_b,
+                      &ovf_b, &novf_b, &perm_b, &sorted_b, &grid64_b, &rho_b, &phi_b, &spec_b,
+                      &table_b, &part_b})
+      b->release();
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    cap_stream = nullptr;
+  }
 };
 
 namespace {
@@ -1128,20 +1224,409 @@ gse_status_t gse_gsek_corrections(gse_gsek_t h, int n, const void* xyzq, double*
 }
 
 // ------------------------------- batched -----------------------------------
-gse_status_t gse_batch_create(gse_batch_t* out, gse_dtype_t, gse_dtype_t, int, const int*,
-                              const double*, const int*, double, double, double, double, double) {
-  if (out) *out = nullptr;
-  return fail(GSE_ERROR_NOT_SUPPORTED, "batched systems are not available yet");
+}  // extern "C"
+
+namespace {
+
+// Lays out bins, slots, grids and spectra of all systems for the current
+// launch shape and uploads the descriptors; builds the Green's tables and the
+// batched FFT plans on the first call.
+gse_status_t batch_layout(gse_batch_s* h, hipStream_t st) {
+  HIPCHK(hipStreamSynchronize(st));  // buffers may be reallocated below
+  h->release_graph();
+  const int ns = h->nsys;
+  const bool dbl = h->ct_dbl;
+  const size_t ct = tsize(dbl);
+  const bool acc32 = !dbl && g_tuning.spread_lds32;
+  h->sys.assign(ns, BatchSys{});
+  h->items.clear();
+  h->cchunks.clear();
+  h->kchunks.clear();
+  h->groups.clear();
+  long long bofs = 0, slot0 = 0, gofs = 0, kofs = 0;
+  std::vector<int> nzs;
+  for (int k = 0; k < ns; k++) {
+    const int si = h->order[k];
+    const int X = h->dims[3 * si], Y = h->dims[3 * si + 1], Z = h->dims[3 * si + 2];
+    const double bx = h->boxes[3 * si], by = h->boxes[3 * si + 1], bz = h->boxes[3 * si + 2];
+    BatchSys& S = h->sys[si];
+    S.g = make_geom(X, Y, Z, bx, by, bz, h->rc1);
+    S.b = choose_bins(S.g, acc32 ? 4 : 8, (size_t)g_tuning.brick_budget, h->bin_edge);
+    if (S.b.direct)
+      return fail(GSE_ERROR_NOT_SUPPORTED,
+                  "system %d: the support (n = %d) does not fit the batched LDS path", si,
+                  S.g.nx);
+    S.n = h->counts[si];
+    const long long c = 2LL * S.n / S.b.nbins + 32;
+    S.cap_bin = (int)((c + 15) / 16 * 16);
+    S.bofs = (int)bofs;
+    S.slot0 = slot0;
+    S.gofs = gofs;
+    S.kofs = kofs;
+    bofs += S.b.nbins;
+    slot0 += (long long)S.b.nbins * S.cap_bin;
+    gofs += (long long)X * Y * Z;
+    const long long K = (long long)(X / 2 + 1) * Y * Z;
+    kofs += K;
+    S.images = ((2 * S.g.nx + 1 + X - 1) / X) * ((2 * S.g.ny + 1 + Y - 1) / Y) *
+               ((2 * S.g.nz + 1 + Z - 1) / Z);
+    const double s1 = h->sigma1;
+    S.inv2s2 = 1.0 / (2.0 * s1 * s1);
+    S.pref = std::pow(2.0 * M_PI * s1 * s1, -1.5);
+    S.pref_bound = S.pref * spread_weight_bound(S.g, s1, false);
+    S.rc2 = h->rc1 * h->rc1;
+    S.vcell = S.g.hx * S.g.hy * S.g.hz;
+    S.prefF = std::pow(2.0 * M_PI, -1.5) * S.vcell * std::pow(s1, -5.0);
+    S.prefP = S.pref * S.vcell;
+    spread_lds_bytes(dbl, S.g, S.b, &S.spread_cap);
+    gather_lds_bytes(dbl, S.g, S.b, &S.gather_cap);
+    // k-pass chunks of 4096 elements, grid conversion chunks of 16384 points
+    S.kblk0 = (int)h->kchunks.size();
+    for (long long a = 0; a < K; a += 4096)
+      h->kchunks.push_back(BatchChunk{si, (int)a, (int)std::min<long long>(4096, K - a), 0});
+    S.nkblk = (int)h->kchunks.size() - S.kblk0;
+    const long long P = (long long)X * Y * Z;
+    for (long long a = 0; a < P; a += 16384)
+      h->cchunks.push_back(BatchChunk{si, (int)a, (int)std::min<long long>(16384, P - a), 0});
+    const int nz = 2 * S.g.nz + 1;
+    if (std::find(nzs.begin(), nzs.end(), nz) == nzs.end()) nzs.push_back(nz);
+  }
+  h->total_bins = (int)bofs;
+  h->total_slots = slot0;
+  h->total_points = gofs;
+  h->total_k = kofs;
+  // spread / gather items, grouped by 2nz+1 (compile-time z extent)
+  for (int nz : nzs) {
+    BatchGroup gr;
+    gr.nz = nz;
+    gr.item0 = (int)h->items.size();
+    for (int k = 0; k < ns; k++) {
+      const int si = h->order[k];
+      const BatchSys& S = h->sys[si];
+      if (2 * S.g.nz + 1 != nz) continue;
+      for (int b = 0; b < S.b.nbins; b++) h->items.push_back(make_int2(si, b));
+      gr.lds_spread = std::max(gr.lds_spread, spread_lds_bytes(dbl, S.g, S.b, nullptr));
+      gr.lds_gather = std::max(gr.lds_gather, gather_lds_bytes(dbl, S.g, S.b, nullptr));
+    }
+    gr.nitems = (int)h->items.size() - gr.item0;
+    const int want = std::min(gr.nitems, 2048);
+    gr.grid = std::max(8, (want + 7) / 8 * 8);
+    h->groups.push_back(gr);
+  }
+  // device buffers
+  const int n1 = h->ntot > 0 ? h->ntot : 1;
+  GSECHK(h->d_sys.ensure(sizeof(BatchSys) * ns));
+  GSECHK(h->d_items.ensure(sizeof(int2) * std::max<size_t>(1, h->items.size())));
+  GSECHK(h->d_cchunks.ensure(sizeof(BatchChunk) * std::max<size_t>(1, h->cchunks.size())));
+  GSECHK(h->d_kchunks.ensure(sizeof(BatchChunk) * std::max<size_t>(1, h->kchunks.size())));
+  GSECHK(h->counts_b.ensure(sizeof(int) * std::max(1, h->total_bins)));
+  GSECHK(h->stats_b.ensure(sizeof(BinStats) * ns));
+  GSECHK(h->ovf_b.ensure(sizeof(int) * n1));
+  GSECHK(h->novf_b.ensure(sizeof(int)));
+  GSECHK(h->perm_b.ensure(sizeof(int) * std::max<long long>(1, h->total_slots)));
+  GSECHK(h->sorted_b.ensure(4 * ct * std::max<long long>(1, h->total_slots)));
+  GSECHK(h->grid64_b.ensure(8 * (size_t)h->total_points));
+  GSECHK(h->rho_b.ensure(ct * (size_t)h->total_points));
+  GSECHK(h->phi_b.ensure(ct * (size_t)h->total_points));
+  GSECHK(h->spec_b.ensure(2 * ct * (size_t)h->total_k));
+  GSECHK(h->table_b.ensure(ct * (size_t)h->total_k));
+  GSECHK(h->part_b.ensure(sizeof(double) * std::max<size_t>(1, h->kchunks.size())));
+  HIPCHK(hipMemcpy(h->d_sys.p, h->sys.data(), sizeof(BatchSys) * ns, hipMemcpyHostToDevice));
+  if (!h->items.empty())
+    HIPCHK(hipMemcpy(h->d_items.p, h->items.data(), sizeof(int2) * h->items.size(),
+                     hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_cchunks.p, h->cchunks.data(), sizeof(BatchChunk) * h->cchunks.size(),
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_kchunks.p, h->kchunks.data(), sizeof(BatchChunk) * h->kchunks.size(),
+                   hipMemcpyHostToDevice));
+  std::vector<int> pofs(ns + 1, 0);
+  for (int si = 0; si < ns; si++) pofs[si + 1] = pofs[si] + h->counts[si];
+  GSECHK(h->d_pofs.ensure(sizeof(int) * (ns + 1)));
+  HIPCHK(hipMemcpy(h->d_pofs.p, pofs.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice));
+  BatchDev& d = h->dev;
+  d.sys = h->d_sys.as<BatchSys>();
+  d.nsys = ns;
+  d.pofs = h->d_pofs.as<int>();
+  d.ntot = h->ntot;
+  d.counts = h->counts_b.as<int>();
+  d.total_bins = h->total_bins;
+  d.stats = h->stats_b.as<BinStats>();
+  d.ovf = h->ovf_b.as<int>();
+  d.novf = h->novf_b.as<int>();
+  d.perm = h->perm_b.as<int>();
+  d.sorted = h->sorted_b.p;
+  d.grid64 = h->grid64_b.as<unsigned long long>();
+  HIPCHK(launch_fill_u64(d.grid64, (size_t)h->total_points, st));
+  // Green's tables (G_GSE with sigma2) per system, batched FFT plans per grid size
+  for (int k = 0; k < ns; k++) {
+    const int si = h->order[k];
+    const BatchSys& S = h->sys[si];
+    HIPCHK(launch_greens_table(dbl, GSE_GREENS_GSE, S.g.X, S.g.Y, S.g.Z, h->boxes[3 * si],
+                               h->boxes[3 * si + 1], h->boxes[3 * si + 2], h->sigma2,
+                               (char*)h->table_b.p + ct * S.kofs, st));
+  }
+  if (h->ffts.empty()) {
+    int k = 0;
+    while (k < ns) {
+      const int si = h->order[k];
+      int e = k + 1;
+      while (e < ns && h->dims[3 * h->order[e]] == h->dims[3 * si] &&
+             h->dims[3 * h->order[e] + 1] == h->dims[3 * si + 1] &&
+             h->dims[3 * h->order[e] + 2] == h->dims[3 * si + 2])
+        e++;
+      BatchFFT* f = new BatchFFT();
+      h->ffts.push_back(f);
+      GSECHK(f->fft.init(dbl, h->dims[3 * si], h->dims[3 * si + 1], h->dims[3 * si + 2], e - k));
+      f->gofs = h->sys[si].gofs;
+      f->kofs = h->sys[si].kofs;
+      k = e;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  return GSE_SUCCESS;
 }
-gse_status_t gse_batch_destroy(gse_batch_t) { return GSE_SUCCESS; }
-gse_status_t gse_batch_set_stream(gse_batch_t, void*) {
-  return fail(GSE_ERROR_NOT_SUPPORTED, "batched systems are not available yet");
+
+int batch_block(int want, bool dbl) { return want ? want : (dbl ? 256 : 512); }
+
+// One batched step on stream st: every stage for all systems.
+gse_status_t batch_run(gse_batch_s* h, hipStream_t st, const void* xyzq, void* fx, void* fy,
+                       void* fz, void* pot, double* energies) {
+  const bool dbl = h->ct_dbl;
+  const size_t ct = tsize(dbl);
+  const BatchDev& d = h->dev;
+  HIPCHK(launch_batch_clear(d, st));
+  HIPCHK(launch_batch_bin(dbl, d, xyzq, st));
+  const int2* items = h->d_items.as<int2>();
+  for (const BatchGroup& g : h->groups)
+    HIPCHK(launch_batch_spread(dbl, g.nz, batch_block(h->spread_block, dbl), g.lds_spread, g.grid,
+                               d, items + g.item0, g.nitems, st));
+  HIPCHK(launch_batch_spread_overflow(dbl, d, xyzq, st));
+  HIPCHK(launch_batch_convert(dbl, d, h->d_cchunks.as<BatchChunk>(), (int)h->cchunks.size(),
+                              h->rho_b.p, st));
+  for (BatchFFT* f : h->ffts)
+    GSECHK(f->fft.exec(true, (char*)h->rho_b.p + ct * f->gofs,
+                       (char*)h->spec_b.p + 2 * ct * f->kofs, st));
+  HIPCHK(launch_batch_kpass(dbl, d, h->d_kchunks.as<BatchChunk>(), (int)h->kchunks.size(),
+                            h->spec_b.p, h->table_b.p, h->part_b.as<double>(), st));
+  for (BatchFFT* f : h->ffts)
+    GSECHK(f->fft.exec(false, (char*)h->spec_b.p + 2 * ct * f->kofs,
+                       (char*)h->phi_b.p + ct * f->gofs, st));
+  GatherOut o;
+  o.fx = fx;
+  o.fy = fy;
+  o.fz = fz;
+  o.pot = pot;
+  o.fscale = o.pscale = o.vscale = h->ccelec;
+  if (fx) {
+    for (const BatchGroup& g : h->groups)
+      HIPCHK(launch_batch_gather(dbl, h->at_dbl, g.nz, batch_block(h->gather_block, dbl),
+                                 g.lds_gather, g.grid, d, items + g.item0, g.nitems, h->phi_b.p,
+                                 o, st));
+    HIPCHK(launch_batch_gather_overflow(dbl, h->at_dbl, d, xyzq, h->phi_b.p, o, st));
+  }
+  if (energies)
+    HIPCHK(launch_batch_energy(d, h->part_b.as<double>(), h->ccelec, energies, st));
+  return GSE_SUCCESS;
 }
-gse_status_t gse_batch_step(gse_batch_t, const void*, void*, void*, void*, void*, double*) {
-  return fail(GSE_ERROR_NOT_SUPPORTED, "batched systems are not available yet");
+
+gse_status_t batch_check_outputs(const gse_batch_s* h, const void* xyzq, void* fx, void* fy,
+                                 void* fz) {
+  if (h->ntot > 0 && !xyzq) return fail(GSE_ERROR_INVALID_VALUE, "xyzq is NULL");
+  if ((fx || fy || fz) && !(fx && fy && fz))
+    return fail(GSE_ERROR_INVALID_VALUE, "forceX/Y/Z must be all set or all NULL");
+  return GSE_SUCCESS;
 }
-gse_status_t gse_batch_step_graph(gse_batch_t, const void*, void*, void*, void*, void*, double*) {
-  return fail(GSE_ERROR_NOT_SUPPORTED, "batched systems are not available yet");
+
+}  // namespace
+
+extern "C" {
+
+gse_status_t gse_batch_create(gse_batch_t* out, gse_dtype_t at, gse_dtype_t ct, int nsys,
+                              const int* sizes, const double* boxes, const int* counts,
+                              double sigma, double kappa, double ls, double ls1, double ccelec) {
+  if (!out) return fail(GSE_ERROR_INVALID_VALUE, "out is NULL");
+  *out = nullptr;
+  if (!valid_pair(at, ct)) return fail(GSE_ERROR_INVALID_VALUE, "unsupported (AT, CT) pair");
+  if (nsys <= 0 || !sizes || !boxes || !counts)
+    return fail(GSE_ERROR_INVALID_VALUE, "need nsys > 0 and the sizes, boxes and counts arrays");
+  if (!(kappa >= 0.0 && kappa <= 0.5)) return fail(GSE_ERROR_INVALID_VALUE, "kappa must be in [0, 0.5]");
+  if (!(sigma > 0) || !(ls1 > 0)) return fail(GSE_ERROR_INVALID_VALUE, "sigma and lambdaSigma1 must be positive");
+  long long ntot = 0;
+  for (int k = 0; k < nsys; k++) {
+    GSECHK(check_dims(sizes[3 * k], sizes[3 * k + 1], sizes[3 * k + 2]));
+    GSECHK(check_box(boxes[3 * k], boxes[3 * k + 1], boxes[3 * k + 2]));
+    if (counts[k] < 0) return fail(GSE_ERROR_INVALID_VALUE, "negative particle count");
+    ntot += counts[k];
+  }
+  if (ntot > (1LL << 30)) return fail(GSE_ERROR_INVALID_VALUE, "too many particles");
+  auto* h = new gse_batch_s();
+  h->at_dbl = at == GSE_FLOAT64;
+  h->ct_dbl = ct == GSE_FLOAT64;
+  h->nsys = nsys;
+  h->sigma = sigma;
+  h->kappa = kappa;
+  h->ls = ls;
+  h->ls1 = ls1;
+  h->ccelec = ccelec;
+  h->sigma1 = sigma * std::sqrt(kappa);  // CpuGSE.cpp:42-43
+  h->sigma2 = sigma * std::sqrt(1.0 - 2.0 * kappa);
+  h->rc1 = ls1 * std::sqrt(2.0) * h->sigma1;
+  h->dims.assign(sizes, sizes + 3 * nsys);
+  h->boxes.assign(boxes, boxes + 3 * nsys);
+  h->counts.assign(counts, counts + nsys);
+  h->ntot = (int)ntot;
+  h->order.resize(nsys);
+  for (int k = 0; k < nsys; k++) h->order[k] = k;
+  std::stable_sort(h->order.begin(), h->order.end(), [&](int a, int b) {
+    for (int c = 0; c < 3; c++)
+      if (h->dims[3 * a + c] != h->dims[3 * b + c]) return h->dims[3 * a + c] < h->dims[3 * b + c];
+    return false;
+  });
+  gse_status_t st = batch_layout(h, nullptr);
+  if (st != GSE_SUCCESS) {
+    h->release();
+    delete h;
+    return st;
+  }
+  *out = h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_batch_destroy(gse_batch_t h) {
+  if (!h) return GSE_SUCCESS;
+  h->release();
+  delete h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_batch_set_stream(gse_batch_t h, void* s) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  h->stream = (hipStream_t)s;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_batch_step(gse_batch_t h, const void* xyzq, void* fx, void* fy, void* fz,
+                            void* pot, double* energies) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  GSECHK(batch_check_outputs(h, xyzq, fx, fy, fz));
+  return batch_run(h, h->stream, xyzq, fx, fy, fz, pot, energies);
+}
+
+gse_status_t gse_batch_step_graph(gse_batch_t h, const void* xyzq, void* fx, void* fy, void* fz,
+                                  void* pot, double* energies) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  GSECHK(batch_check_outputs(h, xyzq, fx, fy, fz));
+  const void* ptrs[6] = {xyzq, fx, fy, fz, pot, energies};
+  if (h->gexec && std::memcmp(ptrs, h->g_ptrs, sizeof(ptrs)) != 0) h->release_graph();
+  if (!h->gexec) {
+    if (!h->cap_stream) HIPCHK(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
+    // the capture stream starts after the caller's pending work
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
+    gse_status_t st = batch_run(h, h->cap_stream, xyzq, fx, fy, fz, pot, energies);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(h->cap_stream, &graph);
+    if (st != GSE_SUCCESS) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return st;
+    }
+    if (e != hipSuccess) return fail(GSE_ERROR_HIP, "stream capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      h->gexec = nullptr;
+      return fail(GSE_ERROR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+    std::memcpy(h->g_ptrs, ptrs, sizeof(ptrs));
+  }
+  HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_batch_set_config(gse_batch_t h, int bin_edge, int spread_block,
+                                  int gather_block) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  if (bin_edge < 1 || bin_edge > 64 || (spread_block != 0 && spread_block != 256 &&
+                                        spread_block != 512) ||
+      (gather_block != 0 && gather_block != 256 && gather_block != 512))
+    return fail(GSE_ERROR_INVALID_VALUE, "bad launch shape");
+  const int old[3] = {h->bin_edge, h->spread_block, h->gather_block};
+  h->bin_edge = bin_edge;
+  h->spread_block = spread_block;
+  h->gather_block = gather_block;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  gse_status_t st = batch_layout(h, h->stream);
+  if (st != GSE_SUCCESS) {  // keep the previous, working shape
+    h->bin_edge = old[0];
+    h->spread_block = old[1];
+    h->gather_block = old[2];
+    (void)batch_layout(h, h->stream);
+  }
+  return st;
+}
+
+gse_status_t gse_batch_get_config(gse_batch_t h, int* config) {
+  if (!h || !config) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  config[0] = h->bin_edge;
+  config[1] = batch_block(h->spread_block, h->ct_dbl);
+  config[2] = batch_block(h->gather_block, h->ct_dbl);
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_batch_autotune(gse_batch_t h, const void* xyzq, int reps, double* best_ms) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  if (h->ntot > 0 && !xyzq) return fail(GSE_ERROR_INVALID_VALUE, "xyzq is NULL");
+  if (reps < 1) reps = 1;
+  const size_t at = tsize(h->at_dbl);
+  DevBuf f, e;
+  GSECHK(f.ensure(3 * at * (size_t)(h->ntot > 0 ? h->ntot : 1)));
+  GSECHK(e.ensure(sizeof(double) * h->nsys));
+  char* fp = (char*)f.p;
+  const size_t fb = at * (size_t)(h->ntot > 0 ? h->ntot : 1);
+  hipEvent_t t0, t1;
+  HIPCHK(hipEventCreate(&t0));
+  HIPCHK(hipEventCreate(&t1));
+  double best = 1e300;
+  int bc[3] = {h->bin_edge, h->spread_block, h->gather_block};
+  const int edges[3] = {6, 8, 10};
+  const int blocks[2] = {256, 512};
+  gse_status_t result = GSE_SUCCESS;
+  for (int be : edges)
+    for (int sb : blocks)
+      for (int gb : blocks) {
+        h->bin_edge = be;
+        h->spread_block = sb;
+        h->gather_block = gb;
+        if (batch_layout(h, h->stream) != GSE_SUCCESS) continue;
+        // one untimed step, then reps timed steps
+        gse_status_t st = batch_run(h, h->stream, xyzq, fp, fp + fb, fp + 2 * fb, nullptr, e.as<double>());
+        if (st != GSE_SUCCESS) continue;
+        (void)hipEventRecord(t0, h->stream);
+        for (int r = 0; r < reps && st == GSE_SUCCESS; r++)
+          st = batch_run(h, h->stream, xyzq, fp, fp + fb, fp + 2 * fb, nullptr, e.as<double>());
+        (void)hipEventRecord(t1, h->stream);
+        if (hipEventSynchronize(t1) != hipSuccess || st != GSE_SUCCESS) continue;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        if (ms / reps < best) {
+          best = ms / reps;
+          bc[0] = be;
+          bc[1] = sb;
+          bc[2] = gb;
+        }
+      }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  h->bin_edge = bc[0];
+  h->spread_block = bc[1];
+  h->gather_block = bc[2];
+  result = batch_layout(h, h->stream);
+  f.release();
+  e.release();
+  if (best_ms) *best_ms = best < 1e300 ? best : -1.0;
+  if (result == GSE_SUCCESS && !(best < 1e300))
+    return fail(GSE_ERROR_HIP, "no launch shape ran");
+  return result;
 }
 
 }  // extern "C"

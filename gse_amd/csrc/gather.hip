@@ -143,42 +143,20 @@ __host__ __device__ inline GatherLds gather_lds(int vol, int cap, int nw, int ct
 }
 
 // FL: GatherFlags bit mask; NZC: 2nz+1 known at compile time (0 = runtime)
+//
+// One bin of one system: its cnt particles start at `sorted`/`perm` (slot 0
+// of the bin), the system's potential grid at `phi`; perm holds the output
+// index of each particle.  Ends with a barrier, so a workgroup may call it
+// for several bins in a row.
 template <typename CT, typename AT, int NZC, int FL, int BS>
-__global__ void __launch_bounds__(BS)
-    k_gather(GatherArgs a, const V4<CT>* __restrict__ xyzq, const V4<CT>* __restrict__ sorted,
-             const int* __restrict__ perm, const int* __restrict__ counts,
-             int* __restrict__ counts_other, const BinStats* __restrict__ stats,
-             BinStats* __restrict__ stats_other, const int* __restrict__ ovf,
-             const CT* __restrict__ phi, GatherOut o) {
-  extern __shared__ __align__(32) double smem_raw[];
+__device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt,
+                                           const V4<CT>* __restrict__ sorted,
+                                           const int* __restrict__ perm,
+                                           const CT* __restrict__ phi, const GatherOut& o,
+                                           double ds, double* smem_raw) {
   const Geom& g = a.g;
   const BinShape& b = a.b;
-  const double ds = o.dscale ? *o.dscale : 1.0;
-  if ((int)blockIdx.x == a.nwg_bins + kOverflowWG) {  // spare workgroup: k-pass finalize
-    kpass_finalize_block(o.fin_partials, kKpassBlocks, o.fin_ncomp, o.fin_pref, o.fin_energy,
-                         o.fin_virial);
-    return;
-  }
-  if ((int)blockIdx.x >= a.nwg_bins) {  // overflow particles from global memory
-    if (blockIdx.x == (unsigned)a.nwg_bins && threadIdx.x == 0) {
-      stats_other->qmax_bits = 0ull;
-      stats_other->invalid = 0;
-      stats_other->overflow = 0;
-    }
-    const int novf = stats->overflow;
-    const int gw = (blockIdx.x - a.nwg_bins) * (BS / 64) + (threadIdx.x >> 6);
-    for (int k = __builtin_amdgcn_readfirstlane(gw); k < novf; k += kOverflowWG * (BS / 64)) {
-      const int oi = ovf[k];
-      gather_one_global<CT, AT, FL>(a, o, xyzq[oi], oi, ds, phi);
-    }
-    return;
-  }
-  const int bin = xcd_remap(blockIdx.x, a.nwg_bins);
-  if (bin >= b.nbins) return;
-  const int cnt = min(counts[bin], a.cap_bin);
-  if (threadIdx.x == 0) counts_other[bin] = 0;
-  if (cnt == 0) return;
-  const int s0 = bin * a.cap_bin, s1 = s0 + cnt;
+  const int s0 = 0, s1 = cnt;
   const int bxi = bin % b.nbx, byi = (bin / b.nbx) % b.nby, bzi = bin / (b.nbx * b.nby);
   const int Lx = b.Lx, Ly = b.Ly, Lz = b.Lz, LxLy = b.Lx * b.Ly;
   const int NX = 2 * g.nx + 1, NY = 2 * g.ny + 1;
@@ -383,6 +361,44 @@ __global__ void __launch_bounds__(BS)
   }
 }
 
+
+template <typename CT, typename AT, int NZC, int FL, int BS>
+__global__ void __launch_bounds__(BS)
+    k_gather(GatherArgs a, const V4<CT>* __restrict__ xyzq, const V4<CT>* __restrict__ sorted,
+             const int* __restrict__ perm, const int* __restrict__ counts,
+             int* __restrict__ counts_other, const BinStats* __restrict__ stats,
+             BinStats* __restrict__ stats_other, const int* __restrict__ ovf,
+             const CT* __restrict__ phi, GatherOut o) {
+  extern __shared__ __align__(32) double smem_raw[];
+  const double ds = o.dscale ? *o.dscale : 1.0;
+  if ((int)blockIdx.x == a.nwg_bins + kOverflowWG) {  // spare workgroup: k-pass finalize
+    kpass_finalize_block(o.fin_partials, kKpassBlocks, o.fin_ncomp, o.fin_pref, o.fin_energy,
+                         o.fin_virial);
+    return;
+  }
+  if ((int)blockIdx.x >= a.nwg_bins) {  // overflow particles from global memory
+    if (blockIdx.x == (unsigned)a.nwg_bins && threadIdx.x == 0) {
+      stats_other->qmax_bits = 0ull;
+      stats_other->invalid = 0;
+      stats_other->overflow = 0;
+    }
+    const int novf = stats->overflow;
+    const int gw = (blockIdx.x - a.nwg_bins) * (BS / 64) + (threadIdx.x >> 6);
+    for (int k = __builtin_amdgcn_readfirstlane(gw); k < novf; k += kOverflowWG * (BS / 64)) {
+      const int oi = ovf[k];
+      gather_one_global<CT, AT, FL>(a, o, xyzq[oi], oi, ds, phi);
+    }
+    return;
+  }
+  const int bin = xcd_remap(blockIdx.x, a.nwg_bins);
+  if (bin >= a.b.nbins) return;
+  const int cnt = min(counts[bin], a.cap_bin);
+  if (threadIdx.x == 0) counts_other[bin] = 0;
+  if (cnt == 0) return;
+  const size_t s0 = (size_t)bin * a.cap_bin;
+  gather_bin<CT, AT, NZC, FL, BS>(a, bin, cnt, sorted + s0, perm + s0, phi, o, ds, smem_raw);
+}
+
 // Supports too wide for the LDS path: every binned particle (one bin of
 // capacity n) from global memory, one wave per particle.
 template <typename CT, typename AT, int FL>
@@ -408,6 +424,148 @@ __global__ void __launch_bounds__(256)
   const int gw = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int nwave = ((gridDim.x - 1) * blockDim.x) >> 6;
   for (int i = gw; i < n; i += nwave) gather_one_global<CT, AT, FL>(a, o, sorted[i], perm[i], ds, phi);
+}
+
+// ---------------------------------------------------------------------------
+// Batched independent systems (see the batched spread): one launch gathers
+// every system; workgroups walk the (system, bin) item table.
+__device__ __forceinline__ int batch_system_g(const int* __restrict__ pofs, int nsys, int i) {
+  int lo = 0, hi = nsys;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pofs[mid] <= i)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ GatherArgs batch_gather_args(const BatchSys& S) {
+  GatherArgs a;
+  a.g = S.g;
+  a.b = S.b;
+  a.inv2s2 = S.inv2s2;
+  a.rc2 = S.rc2;
+  a.prefF = S.prefF;
+  a.prefP = S.prefP;
+  a.cap = S.gather_cap;
+  a.cap_bin = S.cap_bin;
+  a.nwg_bins = 0;
+  return a;
+}
+
+template <typename CT, typename AT, int NZC, int FL, int BS>
+__global__ void __launch_bounds__(BS)
+    k_batch_gather(BatchDev d, const int2* __restrict__ items, int nitems,
+                   const CT* __restrict__ phi, GatherOut o) {
+  extern __shared__ __align__(32) double smem_raw[];
+  const int nx = (int)(gridDim.x >> 3);
+  const int x = (int)(blockIdx.x & 7), j = (int)(blockIdx.x >> 3);
+  const int chunk = (nitems + 7) >> 3;
+  const int end = min(nitems, x * chunk + chunk);
+  for (int it = x * chunk + j; it < end; it += nx) {
+    const int2 item = items[it];
+    const BatchSys& S = d.sys[item.x];
+    const int bin = item.y;
+    const int cnt = min(d.counts[S.bofs + bin], S.cap_bin);
+    if (cnt == 0) continue;
+    const long long s0 = S.slot0 + (long long)bin * S.cap_bin;
+    gather_bin<CT, AT, NZC, FL, BS>(batch_gather_args(S), bin, cnt,
+                                    (const V4<CT>*)d.sorted + s0, d.perm + s0, phi + S.gofs, o,
+                                    1.0, smem_raw);
+  }
+}
+
+template <typename CT, typename AT, int FL>
+__global__ void __launch_bounds__(256)
+    k_batch_gather_ovf(BatchDev d, const V4<CT>* __restrict__ xyzq, const CT* __restrict__ phi,
+                       GatherOut o) {
+  const int novf = *d.novf;
+  const int gw = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int nw = (gridDim.x * blockDim.x) >> 6;
+  for (int k = gw; k < novf; k += nw) {
+    const int i = d.ovf[k];
+    const BatchSys& S = d.sys[batch_system_g(d.pofs, d.nsys, i)];
+    gather_one_global<CT, AT, FL>(batch_gather_args(S), o, xyzq[i], i, 1.0, phi + S.gofs);
+  }
+}
+
+template <typename CT, typename AT, int FL>
+static hipError_t batch_gather_t(int nz, int bs, size_t lds, int grid, const BatchDev& d,
+                                 const int2* items, int nitems, const void* phi,
+                                 const GatherOut& o, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  const unsigned g = (unsigned)grid;
+  const CT* ph = (const CT*)phi;
+#define GSE_BG_LAUNCH(NZV, BSV) \
+  k_batch_gather<CT, AT, NZV, FL, BSV><<<g, BSV, lds, s>>>(d, items, nitems, ph, o)
+#define GSE_BG_BS(NZV)       \
+  if (bs == 512)             \
+    GSE_BG_LAUNCH(NZV, 512); \
+  else                       \
+    GSE_BG_LAUNCH(NZV, 256);
+  switch (nz) {
+    case 13: GSE_BG_BS(13) break;
+    case 15: GSE_BG_BS(15) break;
+    case 17: GSE_BG_BS(17) break;
+    default: GSE_BG_BS(0) break;
+  }
+#undef GSE_BG_BS
+#undef GSE_BG_LAUNCH
+  return hipGetLastError();
+}
+
+template <typename CT, typename AT>
+static hipError_t batch_gather_at(int nz, int bs, size_t lds, int grid, const BatchDev& d,
+                                  const int2* items, int nitems, const void* phi,
+                                  const GatherOut& o, hipStream_t s) {
+  if (o.pot)
+    return batch_gather_t<CT, AT, GF_SPHERE | GF_POT>(nz, bs, lds, grid, d, items, nitems, phi,
+                                                      o, s);
+  return batch_gather_t<CT, AT, GF_SPHERE>(nz, bs, lds, grid, d, items, nitems, phi, o, s);
+}
+
+hipError_t launch_batch_gather(bool ct_dbl, bool at_dbl, int nz, int block, size_t lds, int grid,
+                               const BatchDev& d, const int2* items, int nitems, const void* phi,
+                               const GatherOut& o, hipStream_t s) {
+  if (lds > 150 * 1024) return hipErrorInvalidConfiguration;
+  if (ct_dbl)
+    return batch_gather_at<double, double>(nz, block, lds, grid, d, items, nitems, phi, o, s);
+  if (at_dbl)
+    return batch_gather_at<float, double>(nz, block, lds, grid, d, items, nitems, phi, o, s);
+  return batch_gather_at<float, float>(nz, block, lds, grid, d, items, nitems, phi, o, s);
+}
+
+template <typename CT, typename AT>
+static hipError_t batch_gather_ovf_t(const BatchDev& d, const void* xyzq, const void* phi,
+                                     const GatherOut& o, hipStream_t s) {
+  if (o.pot)
+    k_batch_gather_ovf<CT, AT, GF_SPHERE | GF_POT><<<64, 256, 0, s>>>(
+        d, (const V4<CT>*)xyzq, (const CT*)phi, o);
+  else
+    k_batch_gather_ovf<CT, AT, GF_SPHERE><<<64, 256, 0, s>>>(d, (const V4<CT>*)xyzq,
+                                                            (const CT*)phi, o);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_gather_overflow(bool ct_dbl, bool at_dbl, const BatchDev& d,
+                                        const void* xyzq, const void* phi, const GatherOut& o,
+                                        hipStream_t s) {
+  if (ct_dbl) return batch_gather_ovf_t<double, double>(d, xyzq, phi, o, s);
+  if (at_dbl) return batch_gather_ovf_t<float, double>(d, xyzq, phi, o, s);
+  return batch_gather_ovf_t<float, float>(d, xyzq, phi, o, s);
+}
+
+size_t gather_lds_bytes(bool dbl, const Geom& g, const BinShape& b, int* cap_out) {
+  const int NW = 2 * (g.nx + g.ny + g.nz) + 3;
+  const int vol = b.Lx * b.Ly * b.Lz;
+  const int ct = dbl ? 8 : 4;
+  const size_t lds_total = (size_t)vol * ct + (size_t)tuning().gather_stage;
+  int cap = 256;
+  while (cap > 1 && gather_lds(vol, cap, NW, ct).total > lds_total) cap--;
+  if (cap_out) *cap_out = cap;
+  return gather_lds(vol, cap, NW, ct).total;
 }
 
 // ---------------------------------------------------------------------------

@@ -428,4 +428,63 @@ hipError_t launch_conv_pass(bool dbl, int pass, int X, int Y, int Z, int R, cons
   return hipGetLastError();
 }
 
+
+// ---------------------------------------------------------------------------
+// Batched independent systems: the k-pass over every system's half spectrum
+// (one chunk of one system per workgroup, FP64 partial per chunk) and the
+// per-system energy E_s = 1/2 ccelec h^3 sum_k w G |rho_k|^2.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_batch_kpass(BatchDev d, const BatchChunk* __restrict__ chunks, C2<T>* __restrict__ spec,
+                  const T* __restrict__ table, double* __restrict__ partials) {
+  const BatchChunk c = chunks[blockIdx.x];
+  const BatchSys& S = d.sys[c.sys];
+  const int X = S.g.X, Xh = X / 2 + 1;
+  C2<T>* sp = spec + S.kofs;
+  const T* tb = table + S.kofs;
+  double e = 0.0;
+  for (int i = c.start + threadIdx.x; i < c.start + c.count; i += blockDim.x) {
+    C2<T> v = sp[i];
+    const T g = tb[i];
+    const double re = (double)v.x, im = (double)v.y;
+    v.x = v.x * g;
+    v.y = v.y * g;
+    sp[i] = v;
+    const int ix = i % Xh;
+    const double mult = (ix == 0 || 2 * ix == X) ? 1.0 : 2.0;
+    e += mult * (double)g * (re * re + im * im);
+  }
+  double acc[1] = {e};
+  block_sum_store<1>(acc, partials);
+}
+
+__global__ void __launch_bounds__(64)
+    k_batch_energy(BatchDev d, const double* __restrict__ partials, double ccelec,
+                   double* __restrict__ energies) {
+  const BatchSys& S = d.sys[blockIdx.x];
+  double v = 0.0;
+  for (int b = threadIdx.x; b < S.nkblk; b += 64) v += partials[S.kblk0 + b];
+  v = wave_sum(v);
+  if (threadIdx.x == 0) energies[blockIdx.x] = 0.5 * ccelec * S.vcell * v;
+}
+
+hipError_t launch_batch_kpass(bool dbl, const BatchDev& d, const BatchChunk* chunks, int nchunks,
+                              void* spec, const void* table, double* partials, hipStream_t s) {
+  if (nchunks <= 0) return hipSuccess;
+  if (dbl)
+    k_batch_kpass<double><<<nchunks, 256, 0, s>>>(d, chunks, (C2<double>*)spec,
+                                                  (const double*)table, partials);
+  else
+    k_batch_kpass<float><<<nchunks, 256, 0, s>>>(d, chunks, (C2<float>*)spec, (const float*)table,
+                                                 partials);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_energy(const BatchDev& d, const double* partials, double ccelec,
+                               double* energies, hipStream_t s) {
+  if (d.nsys <= 0) return hipSuccess;
+  k_batch_energy<<<d.nsys, 64, 0, s>>>(d, partials, ccelec, energies);
+  return hipGetLastError();
+}
+
 }  // namespace gse

@@ -172,6 +172,78 @@ hipError_t launch_gather(bool ct_double, bool at_double, const Geom& g,
 // particles (processed without LDS staging).
 constexpr int kOverflowWG = 8;
 
+// ---- batched independent systems -------------------------------------
+// One descriptor per system, in device memory.  Particles of all systems are
+// concatenated in the caller's system order (system s owns [pofs, pofs+n));
+// grids, spectra and Green's tables are concatenated in the batch's internal
+// order (systems with equal grid dimensions adjacent, so rocFFT runs them as
+// one batched transform); bins and slots are numbered per system from bofs
+// and slot0.
+struct BatchSys {
+  Geom g;
+  BinShape b;
+  int n;            // particles of the system
+  int pofs;         // first particle in the concatenated arrays
+  int bofs;         // first bin in the concatenated counts
+  int cap_bin;      // slots per bin
+  long long slot0;  // first slot in sorted / perm
+  long long gofs;   // first grid point in grid64 / rho / phi
+  long long kofs;   // first half-spectrum element in spectrum / table
+  int images;
+  int spread_cap;   // particles staged per chunk in the spread
+  int gather_cap;   // particles staged per chunk in the gather
+  int kblk0, nkblk; // k-pass chunks of this system
+  double inv2s2, pref, pref_bound, rc2, prefF, prefP, vcell;
+};
+
+struct BatchChunk {  // a contiguous range of one system's grid or spectrum
+  int sys;
+  int start;
+  int count;
+  int pad;
+};
+
+struct BatchDev {
+  const BatchSys* sys = nullptr;
+  int nsys = 0;
+  const int* pofs = nullptr;  // [nsys+1] particle offsets (caller order)
+  int ntot = 0;               // particles of all systems
+  int* counts = nullptr;      // [total bins]
+  int total_bins = 0;
+  BinStats* stats = nullptr;  // [nsys]
+  int* ovf = nullptr;         // [ntot] global particle indices
+  int* novf = nullptr;        // [1]
+  int* perm = nullptr;        // [slots] global particle index
+  void* sorted = nullptr;     // [slots] 4-vectors
+  unsigned long long* grid64 = nullptr;
+};
+
+// clears counts, statistics and the overflow counter
+hipError_t launch_batch_clear(const BatchDev& d, hipStream_t s);
+hipError_t launch_batch_bin(bool dbl, const BatchDev& d, const void* xyzq, hipStream_t s);
+// items: (system, bin) pairs of systems sharing 2n_z+1 = nz; lds: bytes
+hipError_t launch_batch_spread(bool dbl, int nz, int block, size_t lds, int grid,
+                               const BatchDev& d, const int2* items, int nitems, hipStream_t s);
+// the overflow list of all systems (global atomics)
+hipError_t launch_batch_spread_overflow(bool dbl, const BatchDev& d, const void* xyzq,
+                                        hipStream_t s);
+hipError_t launch_batch_convert(bool dbl, const BatchDev& d, const BatchChunk* chunks,
+                                int nchunks, void* rho, hipStream_t s);
+hipError_t launch_batch_kpass(bool dbl, const BatchDev& d, const BatchChunk* chunks,
+                              int nchunks, void* spec, const void* table, double* partials,
+                              hipStream_t s);
+hipError_t launch_batch_energy(const BatchDev& d, const double* partials, double ccelec,
+                               double* energies, hipStream_t s);
+hipError_t launch_batch_gather(bool ct_dbl, bool at_dbl, int nz, int block, size_t lds,
+                               int grid, const BatchDev& d, const int2* items, int nitems,
+                               const void* phi, const GatherOut& o, hipStream_t s);
+hipError_t launch_batch_gather_overflow(bool ct_dbl, bool at_dbl, const BatchDev& d,
+                                        const void* xyzq, const void* phi, const GatherOut& o,
+                                        hipStream_t s);
+// LDS bytes and particles staged per chunk of one system's spread / gather
+size_t spread_lds_bytes(bool dbl, const Geom& g, const BinShape& b, int* cap);
+size_t gather_lds_bytes(bool dbl, const Geom& g, const BinShape& b, int* cap);
+
 // k-space
 hipError_t launch_greens_table(bool dbl, int kind, int X, int Y, int Z,
                                double bx, double by, double bz, double sigma,

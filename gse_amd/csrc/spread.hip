@@ -124,10 +124,14 @@ constexpr int kMaxK = 5;  // ceil((2n+1)^2/64) for n <= 8
 
 // Adds one particle's cube straight into the global int64 grid (overflow
 // particles and supports too wide for the LDS path); one wave per particle.
+// S is the particle's scale; each rounded contribution is shifted left by
+// `shift` (the int32 path rounds at its home bin's scale 2^eb and moves to
+// the global 2^e exactly, so a particle adds the same integers whether it
+// went through its bin's brick or the overflow list).
 template <typename CT>
 __device__ void spread_one_global(const Geom& g, double inv2s2, double pref, double S,
                                   int normalize, double pref1, const V4<CT>& p,
-                                  unsigned long long* __restrict__ grid64) {
+                                  unsigned long long* __restrict__ grid64, int shift = 0) {
   const int lane = threadIdx.x & 63;
   const int NX = 2 * g.nx + 1, NY = 2 * g.ny + 1, NZ = 2 * g.nz + 1;
   const double x = (double)p.x, y = (double)p.y, z = (double)p.z;
@@ -164,7 +168,7 @@ __device__ void spread_one_global(const Geom& g, double inv2s2, double pref, dou
     const double v = ((qps * (double)wx) * (double)wy) * (double)wzv;
     const int gx = (wx0 + tx) % g.X, gy = (wy0 + ty) % g.Y, gz = (wz0 + tz) % g.Z;
     atomicAdd(&grid64[gx + (size_t)g.X * (gy + (size_t)g.Y * gz)],
-              (unsigned long long)fixed_round(v));
+              (unsigned long long)(fixed_round(v) << shift));
   }
 }
 
@@ -202,48 +206,58 @@ __host__ __device__ inline SpreadLds spread_lds(int vol, int cap, int nw, int ct
   return l;
 }
 
+// Bin of a particle: the bin of its wrapped cube origin (as k_bin files it).
+template <typename CT>
+__device__ __forceinline__ int home_bin(const Geom& g, const BinShape& b, const V4<CT>& p) {
+  const int ox = wrapi((int)round((double)p.x * g.ihx) - g.nx, g.X);
+  const int oy = wrapi((int)round((double)p.y * g.ihy) - g.ny, g.Y);
+  const int oz = wrapi((int)round((double)p.z * g.ihz) - g.nz, g.Z);
+  return ox / b.Bx + b.nbx * (oy / b.By + b.nby * (oz / b.Bz));
+}
+
+// One overflow particle (one wave) straight into the int64 grid, at the scale
+// its bin's brick would have used.
+template <typename CT, typename ACC>
+__device__ __forceinline__ void spread_overflow(const SpreadArgs& a, int eg,
+                                                unsigned long long qbits,
+                                                const int* __restrict__ counts, const V4<CT>& p,
+                                                unsigned long long* __restrict__ grid64) {
+  int shift = 0;
+  if (sizeof(ACC) == 4) {
+    const int eb = fixed_exponent_bin(qbits, counts[home_bin<CT>(a.g, a.b, p)], a.pref_bound);
+    shift = eg > eb ? eg - eb : 0;
+  }
+  spread_one_global<CT>(a.g, a.inv2s2, a.pref, ldexp(1.0, eg - shift), a.normalize, a.pref1, p,
+                        grid64, shift);
+}
+
 // ACC: LDS accumulator, unsigned long long (int64 at the global scale 2^e) or
 // unsigned (int32 at a per-bin scale 2^eb, eb <= e, from the bin's particle
 // count, shifted exactly to 2^e when the brick is flushed).
+//
+// One bin of one system: the bin's cnt staged particles start at `sorted`
+// (slot 0 of the bin), the system's int64 grid at `grid64`; eg is the
+// system's exponent, qbits its max |q| and cnt_all the bin's particle count
+// including overflow (it sets the int32 scale).  Ends with a barrier, so a
+// workgroup may call it for several bins in a row.
 template <typename CT, int NZC, int BS, typename ACC>
-__global__ void __launch_bounds__(BS)
-    k_spread(SpreadArgs a, const V4<CT>* __restrict__ xyzq, const V4<CT>* __restrict__ sorted,
-             const int* __restrict__ counts, int* __restrict__ counts_other,
-             const BinStats* __restrict__ stats, BinStats* __restrict__ stats_other,
-             const int* __restrict__ ovf, unsigned long long* __restrict__ grid64) {
-  extern __shared__ __align__(32) unsigned char spread_smem[];
-  ACC* const brick = (ACC*)spread_smem;
+__device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt, int cnt_all,
+                                           int eg, unsigned long long qbits,
+                                           const V4<CT>* __restrict__ sorted,
+                                           unsigned long long* __restrict__ grid64,
+                                           unsigned char* smem) {
+  ACC* const brick = (ACC*)smem;
   const Geom& g = a.g;
   const BinShape& b = a.b;
-  const BinStats st = *stats;
-  const int eg = fixed_exponent(st.qmax_bits, a.n - st.invalid, a.images, a.pref_bound);
   const double S = ldexp(1.0, eg);
-  if ((int)blockIdx.x >= a.nwg_bins) {  // overflow particles, global atomics
-    if (blockIdx.x == (unsigned)a.nwg_bins && threadIdx.x == 0) {
-      stats_other->qmax_bits = 0ull;
-      stats_other->invalid = 0;
-      stats_other->overflow = 0;
-    }
-    const int wg = blockIdx.x - a.nwg_bins;
-    const int gw = wg * (BS / 64) + (threadIdx.x >> 6);
-    for (int k = __builtin_amdgcn_readfirstlane(gw); k < st.overflow; k += kOverflowWG * (BS / 64))
-      spread_one_global<CT>(g, a.inv2s2, a.pref, S, a.normalize, a.pref1, xyzq[ovf[k]], grid64);
-    return;
-  }
-  const int bin = xcd_remap(blockIdx.x, a.nwg_bins);
-  if (bin >= b.nbins) return;
-  const int cnt = min(counts[bin], a.cap_bin);
-  if (threadIdx.x == 0) counts_other[bin] = 0;
-  if (cnt == 0) return;
   // int32 bricks: per-bin exponent so that the bin's brick sums stay below 2^30
   int shift = 0;
   double Sb = S;
   if (sizeof(ACC) == 4) {
-    const int eb = fixed_exponent_bin(st.qmax_bits, cnt, a.pref_bound);
+    const int eb = fixed_exponent_bin(qbits, cnt_all, a.pref_bound);
     shift = eg > eb ? eg - eb : 0;
     Sb = ldexp(S, -shift);
   }
-  const int s0 = bin * a.cap_bin, s1 = s0 + cnt;
   const int bxi = bin % b.nbx, byi = (bin / b.nbx) % b.nby, bzi = bin / (b.nbx * b.nby);
   const int Lx = b.Lx, Ly = b.Ly, Lz = b.Lz, LxLy = b.Lx * b.Ly;
   const int vol = LxLy * Lz;
@@ -258,7 +272,6 @@ __global__ void __launch_bounds__(BS)
   double* sQ = (double*)(lds + lay.oQ);  // [cap]
   int* sB = (int*)(lds + lay.oB);        // [cap]
   for (int j = threadIdx.x; j < vol; j += BS) brick[j] = (ACC)0;
-
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   constexpr int nwv = BS / 64;
@@ -285,8 +298,8 @@ __global__ void __launch_bounds__(BS)
     k_off[k] = k_tx[k] + Lx * k_ty[k];
   }
 
-  for (int c0 = s0; c0 < s1; c0 += cap) {
-    const int na = min(cap, s1 - c0);
+  for (int c0 = 0; c0 < cnt; c0 += cap) {
+    const int na = min(cap, cnt - c0);
     // phase 0: the chunk's particles into LDS in one coalesced pass
     for (int t = threadIdx.x; t < na; t += BS) sX[t] = sorted[c0 + t];
     __syncthreads();
@@ -384,6 +397,38 @@ __global__ void __launch_bounds__(BS)
       lz++;
     }
   }
+  __syncthreads();
+}
+
+template <typename CT, int NZC, int BS, typename ACC>
+__global__ void __launch_bounds__(BS)
+    k_spread(SpreadArgs a, const V4<CT>* __restrict__ xyzq, const V4<CT>* __restrict__ sorted,
+             const int* __restrict__ counts, int* __restrict__ counts_other,
+             const BinStats* __restrict__ stats, BinStats* __restrict__ stats_other,
+             const int* __restrict__ ovf, unsigned long long* __restrict__ grid64) {
+  extern __shared__ __align__(32) unsigned char spread_smem[];
+  const BinStats st = *stats;
+  const int eg = fixed_exponent(st.qmax_bits, a.n - st.invalid, a.images, a.pref_bound);
+  if ((int)blockIdx.x >= a.nwg_bins) {  // overflow particles, global atomics
+    if (blockIdx.x == (unsigned)a.nwg_bins && threadIdx.x == 0) {
+      stats_other->qmax_bits = 0ull;
+      stats_other->invalid = 0;
+      stats_other->overflow = 0;
+    }
+    const int wg = blockIdx.x - a.nwg_bins;
+    const int gw = wg * (BS / 64) + (threadIdx.x >> 6);
+    for (int k = __builtin_amdgcn_readfirstlane(gw); k < st.overflow; k += kOverflowWG * (BS / 64))
+      spread_overflow<CT, ACC>(a, eg, st.qmax_bits, counts, xyzq[ovf[k]], grid64);
+    return;
+  }
+  const int bin = xcd_remap(blockIdx.x, a.nwg_bins);
+  if (bin >= a.b.nbins) return;
+  const int call = counts[bin];
+  const int cnt = min(call, a.cap_bin);
+  if (threadIdx.x == 0) counts_other[bin] = 0;
+  if (cnt == 0) return;
+  spread_bin<CT, NZC, BS, ACC>(a, bin, cnt, call, eg, st.qmax_bits,
+                               sorted + (size_t)bin * a.cap_bin, grid64, spread_smem);
 }
 
 // Column-owner spread: thread j owns brick column (x, y) = (j % Lx, j / Lx)
@@ -437,18 +482,19 @@ __global__ void __launch_bounds__(BS)
     const int wg = blockIdx.x - a.nwg_bins;
     const int gw = wg * (BS / 64) + (threadIdx.x >> 6);
     for (int k = __builtin_amdgcn_readfirstlane(gw); k < st.overflow; k += kOverflowWG * (BS / 64))
-      spread_one_global<CT>(g, a.inv2s2, a.pref, S, a.normalize, a.pref1, xyzq[ovf[k]], grid64);
+      spread_overflow<CT, ACC>(a, eg, st.qmax_bits, counts, xyzq[ovf[k]], grid64);
     return;
   }
   const int bin = xcd_remap(blockIdx.x, a.nwg_bins);
   if (bin >= b.nbins) return;
-  const int cnt = min(counts[bin], a.cap_bin);
+  const int call = counts[bin];
+  const int cnt = min(call, a.cap_bin);
   if (threadIdx.x == 0) counts_other[bin] = 0;
   if (cnt == 0) return;
   int shift = 0;
   double Sb = S;
   if (sizeof(ACC) == 4) {  // int32 columns: per-bin exponent keeps each sum below 2^30
-    const int eb = fixed_exponent_bin(st.qmax_bits, cnt, a.pref_bound);
+    const int eb = fixed_exponent_bin(st.qmax_bits, call, a.pref_bound);
     shift = eg > eb ? eg - eb : 0;
     Sb = ldexp(S, -shift);
   }
@@ -632,6 +678,173 @@ static unsigned grid_for(size_t count, unsigned threads, unsigned cap = 4096) {
   if (g < 1) g = 1;
   if (g > cap) g = cap;
   return (unsigned)g;
+}
+
+// ---------------------------------------------------------------------------
+// Batched independent systems: the same per-bin bodies, one launch for all
+// systems.  Spread and gather walk a table of (system, bin) items with a
+// fixed grid of workgroups (each XCD takes one contiguous range of items, so
+// the bins of one system meet in one L2).
+// ---------------------------------------------------------------------------
+// System owning concatenated particle i: largest s with pofs[s] <= i.
+__device__ __forceinline__ int batch_system(const int* __restrict__ pofs, int nsys, int i) {
+  int lo = 0, hi = nsys;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pofs[mid] <= i)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Item it of the XCD-aware persistent walk: workgroup w runs on XCD w % 8
+// and takes the items of that XCD's contiguous range in steps of the number
+// of workgroups per XCD.
+struct ItemWalk {
+  int it, end, step;
+};
+__device__ __forceinline__ ItemWalk item_walk(int nitems) {
+  const int nx = (int)(gridDim.x >> 3);  // workgroups per XCD (grid is a multiple of 8)
+  const int x = (int)(blockIdx.x & 7), j = (int)(blockIdx.x >> 3);
+  const int chunk = (nitems + 7) >> 3;
+  const int b = x * chunk;
+  const int e = min(nitems, b + chunk);
+  return ItemWalk{b + j, e, nx};
+}
+
+__device__ __forceinline__ SpreadArgs batch_spread_args(const BatchSys& S) {
+  SpreadArgs a;
+  a.g = S.g;
+  a.b = S.b;
+  a.inv2s2 = S.inv2s2;
+  a.pref = S.pref;
+  a.pref1 = 0.0;
+  a.normalize = 0;
+  a.cap = S.spread_cap;
+  a.cap_bin = S.cap_bin;
+  a.n = S.n;
+  a.images = S.images;
+  a.pref_bound = S.pref_bound;
+  a.nwg_bins = 0;
+  return a;
+}
+
+__global__ void k_batch_clear(int* counts, int nbins, BinStats* stats, int nsys, int* novf) {
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x, st = gridDim.x * blockDim.x;
+  for (int i = i0; i < nbins; i += st) counts[i] = 0;
+  for (int i = i0; i < nsys; i += st) {
+    stats[i].qmax_bits = 0ull;
+    stats[i].invalid = 0;
+    stats[i].overflow = 0;
+  }
+  if (i0 == 0) *novf = 0;
+}
+
+template <typename CT>
+__global__ void __launch_bounds__(256)
+    k_batch_bin(BatchDev d, const V4<CT>* __restrict__ xyzq) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int s = -1;
+  double aq = 0.0;
+  if (i < d.ntot) {
+    s = batch_system(d.pofs, d.nsys, i);
+    const BatchSys& S = d.sys[s];
+    const Geom& g = S.g;
+    const V4<CT> p = xyzq[i];
+    const double fx = (double)p.x * g.ihx;
+    const double fy = (double)p.y * g.ihy;
+    const double fz = (double)p.z * g.ihz;
+    const double lim = 1.0e9;
+    const bool ok = fabs(fx) < lim && fabs(fy) < lim && fabs(fz) < lim && isfinite((double)p.w);
+    if (ok) {
+      const int ox = wrapi((int)round(fx) - g.nx, g.X);
+      const int oy = wrapi((int)round(fy) - g.ny, g.Y);
+      const int oz = wrapi((int)round(fz) - g.nz, g.Z);
+      const BinShape& b = S.b;
+      const int bin = ox / b.Bx + b.nbx * (oy / b.By + b.nby * (oz / b.Bz));
+      const int slot = atomicAdd(&d.counts[S.bofs + bin], 1);
+      if (slot < S.cap_bin) {
+        const long long pos = S.slot0 + (long long)bin * S.cap_bin + slot;
+        ((V4<CT>*)d.sorted)[pos] = p;
+        d.perm[pos] = i;
+      } else {
+        d.ovf[atomicAdd(d.novf, 1)] = i;
+      }
+      aq = fabs((double)p.w);
+    } else {
+      atomicAdd(&d.stats[s].invalid, 1);
+    }
+  }
+  // max |q| per system: one atomic per wave when the wave holds one system
+  const int s0 = __builtin_amdgcn_readfirstlane(s);
+  const bool uniform = __all(s == s0);
+  if (uniform) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) aq = fmax(aq, __shfl_xor(aq, m, kWave));
+    if (lane == 0 && s0 >= 0 && aq > 0.0)
+      atomicMax(&d.stats[s0].qmax_bits, (unsigned long long)__double_as_longlong(aq));
+  } else if (s >= 0 && aq > 0.0) {
+    atomicMax(&d.stats[s].qmax_bits, (unsigned long long)__double_as_longlong(aq));
+  }
+}
+
+template <typename CT, int NZC, int BS, typename ACC>
+__global__ void __launch_bounds__(BS)
+    k_batch_spread(BatchDev d, const int2* __restrict__ items, int nitems) {
+  extern __shared__ __align__(32) unsigned char spread_smem[];
+  const ItemWalk w = item_walk(nitems);
+  for (int it = w.it; it < w.end; it += w.step) {
+    const int2 item = items[it];
+    const BatchSys& S = d.sys[item.x];
+    const int bin = item.y;
+    const int call = d.counts[S.bofs + bin];
+    const int cnt = min(call, S.cap_bin);
+    if (cnt == 0) continue;
+    const BinStats st = d.stats[item.x];
+    const int eg = fixed_exponent(st.qmax_bits, S.n - st.invalid, S.images, S.pref_bound);
+    const SpreadArgs a = batch_spread_args(S);
+    spread_bin<CT, NZC, BS, ACC>(a, bin, cnt, call, eg, st.qmax_bits,
+                                 (const V4<CT>*)d.sorted + S.slot0 + (long long)bin * S.cap_bin,
+                                 d.grid64 + S.gofs, spread_smem);
+  }
+}
+
+// overflow particles of all systems, one wave each
+template <typename CT, typename ACC>
+__global__ void __launch_bounds__(256)
+    k_batch_spread_ovf(BatchDev d, const V4<CT>* __restrict__ xyzq) {
+  const int novf = *d.novf;
+  const int gw = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int nw = (gridDim.x * blockDim.x) >> 6;
+  for (int k = gw; k < novf; k += nw) {
+    const int i = d.ovf[k];
+    const int s = batch_system(d.pofs, d.nsys, i);
+    const BatchSys& S = d.sys[s];
+    const BinStats st = d.stats[s];
+    const int eg = fixed_exponent(st.qmax_bits, S.n - st.invalid, S.images, S.pref_bound);
+    spread_overflow<CT, ACC>(batch_spread_args(S), eg, st.qmax_bits, d.counts + S.bofs, xyzq[i],
+                             d.grid64 + S.gofs);
+  }
+}
+
+// int64 -> T for every system's grid (one chunk per workgroup), int64 cleared
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_batch_convert(BatchDev d, const BatchChunk* __restrict__ chunks, T* __restrict__ rho) {
+  const BatchChunk c = chunks[blockIdx.x];
+  const BatchSys& S = d.sys[c.sys];
+  const BinStats st = d.stats[c.sys];
+  const double inv =
+      ldexp(1.0, -fixed_exponent(st.qmax_bits, S.n - st.invalid, S.images, S.pref_bound));
+  unsigned long long* g = d.grid64 + S.gofs + c.start;
+  T* out = rho + S.gofs + c.start;
+  for (int i = threadIdx.x; i < c.count; i += blockDim.x) {
+    out[i] = (T)((double)(long long)g[i] * inv);
+    g[i] = 0ull;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -876,6 +1089,92 @@ hipError_t launch_fill(bool dbl, void* p, size_t count, hipStream_t s) {
     k_fill<double><<<grid_for(count, 256, 8192), 256, 0, s>>>((double*)p, count);
   else
     k_fill<float><<<grid_for(count, 256, 8192), 256, 0, s>>>((float*)p, count);
+  return hipGetLastError();
+}
+
+
+size_t spread_lds_bytes(bool dbl, const Geom& g, const BinShape& b, int* cap_out) {
+  const Tuning& tu = tuning();
+  const bool acc32 = !dbl && tu.spread_lds32;
+  const int NW = 2 * (g.nx + g.ny + g.nz) + 3;
+  const int vol = b.Lx * b.Ly * b.Lz;
+  const int vol_eff = vol * (acc32 ? 4 : 8) / 8 + 1;
+  const size_t lds_total = (size_t)vol_eff * 8 + (size_t)tu.spread_stage;
+  const int ct = dbl ? 8 : 4;
+  int cap = 256;
+  while (cap > 1 && spread_lds(vol_eff, cap, NW, ct).total > lds_total) cap--;
+  if (cap_out) *cap_out = cap;
+  return spread_lds(vol_eff, cap, NW, ct).total;
+}
+
+hipError_t launch_batch_clear(const BatchDev& d, hipStream_t s) {
+  const int n = d.total_bins > d.nsys ? d.total_bins : d.nsys;
+  k_batch_clear<<<grid_for(n, 256), 256, 0, s>>>(d.counts, d.total_bins, d.stats, d.nsys, d.novf);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_bin(bool dbl, const BatchDev& d, const void* xyzq, hipStream_t s) {
+  if (d.ntot <= 0) return hipSuccess;
+  const unsigned grid = (unsigned)((d.ntot + 255) / 256);
+  if (dbl)
+    k_batch_bin<double><<<grid, 256, 0, s>>>(d, (const V4<double>*)xyzq);
+  else
+    k_batch_bin<float><<<grid, 256, 0, s>>>(d, (const V4<float>*)xyzq);
+  return hipGetLastError();
+}
+
+template <typename CT, typename ACC>
+static hipError_t batch_spread_t(int nz, int bs, size_t lds, int grid, const BatchDev& d,
+                                 const int2* items, int nitems, hipStream_t s) {
+  if (nitems > 0) {
+    const unsigned g = (unsigned)grid;
+#define GSE_BS_LAUNCH(NZV, BSV) \
+  k_batch_spread<CT, NZV, BSV, ACC><<<g, BSV, lds, s>>>(d, items, nitems)
+#define GSE_BS_BS(NZV)          \
+  if (bs == 512)                \
+    GSE_BS_LAUNCH(NZV, 512);    \
+  else                          \
+    GSE_BS_LAUNCH(NZV, 256);
+    switch (nz) {
+      case 13: GSE_BS_BS(13) break;
+      case 15: GSE_BS_BS(15) break;
+      case 17: GSE_BS_BS(17) break;
+      default: GSE_BS_BS(0) break;
+    }
+#undef GSE_BS_BS
+#undef GSE_BS_LAUNCH
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_spread(bool dbl, int nz, int block, size_t lds, int grid,
+                               const BatchDev& d, const int2* items, int nitems, hipStream_t s) {
+  if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
+  if (dbl) return batch_spread_t<double, unsigned long long>(nz, block, lds, grid, d, items,
+                                                             nitems, s);
+  if (tuning().spread_lds32)
+    return batch_spread_t<float, unsigned>(nz, block, lds, grid, d, items, nitems, s);
+  return batch_spread_t<float, unsigned long long>(nz, block, lds, grid, d, items, nitems, s);
+}
+
+hipError_t launch_batch_spread_overflow(bool dbl, const BatchDev& d, const void* xyzq,
+                                        hipStream_t s) {
+  if (dbl)
+    k_batch_spread_ovf<double, unsigned long long><<<64, 256, 0, s>>>(d, (const V4<double>*)xyzq);
+  else if (tuning().spread_lds32)
+    k_batch_spread_ovf<float, unsigned><<<64, 256, 0, s>>>(d, (const V4<float>*)xyzq);
+  else
+    k_batch_spread_ovf<float, unsigned long long><<<64, 256, 0, s>>>(d, (const V4<float>*)xyzq);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_convert(bool dbl, const BatchDev& d, const BatchChunk* chunks,
+                                int nchunks, void* rho, hipStream_t s) {
+  if (nchunks <= 0) return hipSuccess;
+  if (dbl)
+    k_batch_convert<double><<<nchunks, 256, 0, s>>>(d, chunks, (double*)rho);
+  else
+    k_batch_convert<float><<<nchunks, 256, 0, s>>>(d, chunks, (float*)rho);
   return hipGetLastError();
 }
 

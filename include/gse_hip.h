@@ -304,13 +304,21 @@ GSE_API gse_status_t gse_gsek_corrections(gse_gsek_t h, int numCoord, const void
                                           double* out);
 
 /* ------------------------------------------------------------------------
- * Batched independent systems (build-defined)
+ * Batched independent systems (build-defined; no reference counterpart:
+ * the reference runs one system per CpuGSEk object)
+ *
+ * One launch per stage covers all systems: spread and gather walk a table of
+ * (system, bin) items with a fixed grid of workgroups, the FFTs run as one
+ * batched rocFFT transform per distinct grid size, the k-pass and the
+ * per-system energies are single launches.  gse_batch_step_graph captures
+ * that sequence once in a HIP graph and replays it.
  * ---------------------------------------------------------------------- */
 typedef struct gse_batch_s* gse_batch_t;
 
 /* n systems; for system s: sizes[3s..3s+2] grid, boxes[3s..3s+2] box,
  * counts[s] particles.  All systems share sigma, kappa, lambdas, ccelec and
- * the (AT,CT) pair.  Host arrays. */
+ * the (AT,CT) pair.  Host arrays.  Systems whose support does not fit the
+ * LDS path give GSE_ERROR_NOT_SUPPORTED. */
 GSE_API gse_status_t gse_batch_create(gse_batch_t* out, gse_dtype_t at, gse_dtype_t ct,
                                       int nsys, const int* sizes, const double* boxes,
                                       const int* counts, double sigma, double kappa,
@@ -318,15 +326,30 @@ GSE_API gse_status_t gse_batch_create(gse_batch_t* out, gse_dtype_t at, gse_dtyp
 GSE_API gse_status_t gse_batch_destroy(gse_batch_t h);
 GSE_API gse_status_t gse_batch_set_stream(gse_batch_t h, void* hip_stream);
 /* xyzq: all systems' particles concatenated in system order (device).
- * forces: AT arrays of the same concatenated length (sphere force x ccelec);
- * potential may be NULL; energies: device, nsys doubles. */
+ * forceX/Y/Z: AT arrays of the same concatenated length, the reference
+ * sphere-support force times ccelec (all NULL: no gather); potential
+ * (phi_i, AT) may be NULL; energies: device, nsys doubles (may be NULL),
+ * E_s = 1/2 ccelec h^3 sum rho phi of system s, as CpuGSEk computes it. */
 GSE_API gse_status_t gse_batch_step(gse_batch_t h, const void* xyzq, void* forceX, void* forceY,
                                     void* forceZ, void* potential, double* energies);
-/* Capture one batched step in a HIP graph and replay it on later calls
- * (the pointers given at capture time are reused). */
+/* As gse_batch_step, replayed from a HIP graph captured at the first call
+ * (and again whenever a pointer differs from the captured ones).  The
+ * capture waits for the handle's stream once; replays are ordered on it. */
 GSE_API gse_status_t gse_batch_step_graph(gse_batch_t h, const void* xyzq, void* forceX,
                                           void* forceY, void* forceZ, void* potential,
                                           double* energies);
+/* Launch shape of the batch: bin edge in cells (1..64), threads per spread
+ * and gather workgroup (256, 512; 0 = default).  Re-lays out the batch. */
+GSE_API gse_status_t gse_batch_set_config(gse_batch_t h, int bin_edge, int spread_block,
+                                          int gather_block);
+/* config[0..2] = bin edge, spread block, gather block in use (host). */
+GSE_API gse_status_t gse_batch_get_config(gse_batch_t h, int* config);
+/* Autotuner: times `reps` batched steps on xyzq for each candidate shape
+ * (bin edge 6, 8, 10 x spread block 256, 512 x gather block 256, 512),
+ * keeps the fastest and writes its mean ms per step to *best_ms (host, may
+ * be NULL).  Synchronises. */
+GSE_API gse_status_t gse_batch_autotune(gse_batch_t h, const void* xyzq, int reps,
+                                        double* best_ms);
 
 #ifdef __cplusplus
 }
