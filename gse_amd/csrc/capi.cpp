@@ -118,7 +118,8 @@ gse_status_t check_sigma_rcut(double sigma, double rcut) {
 struct Engine {
   bool ct_dbl = true, at_dbl = true;
   int X = 0, Y = 0, Z = 0;
-  DevBuf counts[2], stats[2], ovf, perm, sorted, grid64, vir6, part;
+  DevBuf counts[2], stats[2], ovf, perm, sorted, grid64, vir6, part, bricks, bshift;
+  bool brick_flush = false;     // the last spread stored bricks (k_brick_reduce converts)
   int cur = 1;                  // buffer of the latest binning
   long long clean[2] = {0, 0};  // leading counts (and the stats) known zero
   bool grid64_clean = false;
@@ -128,7 +129,7 @@ struct Engine {
 
   void release() {
     for (DevBuf* b : {&counts[0], &counts[1], &stats[0], &stats[1], &ovf, &perm, &sorted,
-                      &grid64, &vir6, &part})
+                      &grid64, &vir6, &part, &bricks, &bshift})
       b->release();
   }
 
@@ -199,7 +200,19 @@ struct Engine {
   // the consumer kernels clear the other buffer
   void consumed() { clean[cur ^ 1] = shape.nbins; }
 
+  bool acc32() const { return !ct_dbl && g_tuning.spread_lds32; }
+
   gse_status_t spread_binned(double sigma, bool normalize, hipStream_t s) {
+    brick_flush = g_tuning.spread_flush && !g_tuning.spread_cols && brick_reduce_ok(geom, shape);
+    bin.bricks = nullptr;
+    bin.bshift = nullptr;
+    if (brick_flush) {
+      const size_t vol = (size_t)shape.Lx * shape.Ly * shape.Lz;
+      GSECHK(bricks.ensure((acc32() ? 4 : 8) * vol * (size_t)shape.nbins));
+      GSECHK(bshift.ensure(sizeof(int) * (size_t)shape.nbins));
+      bin.bricks = bricks.p;
+      bin.bshift = bshift.as<int>();
+    }
     HIPCHK(launch_spread(ct_dbl, geom, shape, sigma, normalize, bin,
                          grid64.as<unsigned long long>(), s));
     consumed();
@@ -207,8 +220,12 @@ struct Engine {
   }
 
   gse_status_t convert(void* rho, hipStream_t s) {
-    HIPCHK(launch_fixed_to_grid(ct_dbl, grid64.as<unsigned long long>(), rho, (size_t)X * Y * Z,
-                                bin, true, s));
+    if (brick_flush)
+      HIPCHK(launch_brick_reduce(ct_dbl, acc32(), geom, shape, bin,
+                                 grid64.as<unsigned long long>(), rho, s));
+    else
+      HIPCHK(launch_fixed_to_grid(ct_dbl, grid64.as<unsigned long long>(), rho,
+                                  (size_t)X * Y * Z, bin, true, s));
     return GSE_SUCCESS;
   }
 
@@ -599,6 +616,8 @@ gse_status_t gse_set_tuning_param(const char* name, int value) {
   else if (n == "gather_stage" && value >= 1024 && value <= 96 * 1024) t.gather_stage = value;
   else if (n == "spread_lds32" && (value == 0 || value == 1)) t.spread_lds32 = value;
   else if (n == "spread_cols" && (value == 0 || value == 1)) t.spread_cols = value;
+  else if (n == "spread_flush" && (value == 0 || value == 1)) t.spread_flush = value;
+  else if (n == "reduce_tile" && (value == 0 || value == 1)) t.reduce_tile = value;
   else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter or value out of range: %s=%d",
                    name, value);
   return GSE_SUCCESS;
@@ -616,6 +635,8 @@ gse_status_t gse_get_tuning_param(const char* name, int* value) {
   else if (n == "gather_stage") *value = t.gather_stage;
   else if (n == "spread_lds32") *value = t.spread_lds32;
   else if (n == "spread_cols") *value = t.spread_cols;
+  else if (n == "spread_flush") *value = t.spread_flush;
+  else if (n == "reduce_tile") *value = t.reduce_tile;
   else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter: %s", name);
   return GSE_SUCCESS;
 }

@@ -142,6 +142,17 @@ __host__ __device__ inline GatherLds gather_lds(int vol, int cap, int nw, int ct
   return l;
 }
 
+// Row pitch of the staged potential brick: FP32 bricks on grids whose rows
+// split into aligned 16-byte pieces (X and the bin edge multiples of 4) are
+// staged with 16-byte loads and stores, so their rows are padded to a
+// multiple of 4 values.
+__host__ __device__ inline bool brick_v4(int ct_bytes, int X, int Bx) {
+  return ct_bytes == 4 && X % 4 == 0 && Bx % 4 == 0;
+}
+__host__ __device__ inline int brick_pitch(int ct_bytes, int X, int Bx, int Lx) {
+  return brick_v4(ct_bytes, X, Bx) ? (Lx + 3) & ~3 : Lx;
+}
+
 // FL: GatherFlags bit mask; NZC: 2nz+1 known at compile time (0 = runtime)
 //
 // One bin of one system: its cnt particles start at `sorted`/`perm` (slot 0
@@ -158,12 +169,13 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
   const BinShape& b = a.b;
   const int s0 = 0, s1 = cnt;
   const int bxi = bin % b.nbx, byi = (bin / b.nbx) % b.nby, bzi = bin / (b.nbx * b.nby);
-  const int Lx = b.Lx, Ly = b.Ly, Lz = b.Lz, LxLy = b.Lx * b.Ly;
+  const int Lx = b.Lx, Ly = b.Ly, Lz = b.Lz;
+  const int Px = brick_pitch(sizeof(CT), g.X, b.Bx, Lx), PxLy = Px * Ly;
   const int NX = 2 * g.nx + 1, NY = 2 * g.ny + 1;
   const int NZ = NZC > 0 ? NZC : 2 * g.nz + 1;
   const int NW = NX + NY + NZ;
   const int cap = a.cap;
-  const GatherLds lay = gather_lds(LxLy * Lz, cap, NW, sizeof(CT));
+  const GatherLds lay = gather_lds(PxLy * Lz, cap, NW, sizeof(CT));
   char* lds = (char*)smem_raw;
   V4<CT>* sX = (V4<CT>*)(lds + lay.oX);
   CT* sW = (CT*)(lds + lay.oW);  // [cap][NW][2] (w, d)
@@ -175,28 +187,89 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   constexpr int nwv = BS / 64;
-  {  // stage the potential brick: rows of Lx lanes, no per-element division
+  // the first chunk's particles (cap <= 256 <= BS: at most one per thread)
+  // are loaded first, so their latency overlaps the brick staging
+  V4<CT> pre{};
+  int pre_p = 0;
+  if ((int)threadIdx.x < min(a.cap, cnt)) {
+    pre = sorted[threadIdx.x];
+    pre_p = perm[threadIdx.x];
+  }
+  {  // stage the potential brick: rows of Lx lanes, no per-element division.
+     // Loads go out in batches of kStageBatch before their LDS stores, so a
+     // thread has that many in flight instead of one.  Wrapped grid rows come
+     // from two small tables (Ly, Lz <= 64: the brick may be wider than the
+     // grid, so one conditional wrap would not do).
+    __shared__ int s_gy[64], s_gz[64];
+    if (threadIdx.x < (unsigned)Ly) {
+      int v = byi * b.By + (int)threadIdx.x;
+      while (v >= g.Y) v -= g.Y;
+      s_gy[threadIdx.x] = v;
+    } else if (threadIdx.x >= 64 && threadIdx.x < 64u + (unsigned)Lz) {
+      int v = bzi * b.Bz + (int)threadIdx.x - 64;
+      while (v >= g.Z) v -= g.Z;
+      s_gz[threadIdx.x - 64] = v;
+    }
+    __syncthreads();
+    constexpr int kStageBatch = 8;
+    if (sizeof(CT) == 4 && brick_v4(4, g.X, b.Bx)) {
+      // 16-byte pieces: row r = ly + Ly*lz, piece j covers columns 4j..4j+3
+      const int nq = Px >> 2, xq = g.X >> 2, gq0 = (bxi * b.Bx) >> 2;
+      const int total = Ly * Lz * nq;
+      const float inv_nq = 1.0f / (float)nq, inv_ly = 1.0f / (float)Ly;
+      for (int e0 = threadIdx.x; e0 < total; e0 += kStageBatch * BS) {
+        float4 v[kStageBatch];
+        int dst[kStageBatch];
+#pragma unroll
+        for (int u = 0; u < kStageBatch; u++) {
+          const int e = e0 + u * BS;
+          const bool ok = e < total;
+          const int row = (int)(((float)e + 0.5f) * inv_nq);
+          const int j = e - row * nq;
+          int lz = (int)(((float)row + 0.5f) * inv_ly);
+          const int ly = row - lz * Ly;
+          lz = lz < Lz ? lz : Lz - 1;
+          const int qx = (gq0 + j) % xq;
+          dst[u] = ok ? Px * row + 4 * j : -1;
+          v[u] = ok ? ((const float4*)phi)[qx + (size_t)xq * (s_gy[ly] + (size_t)g.Y * s_gz[lz])]
+                    : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < kStageBatch; u++)
+          if (dst[u] >= 0) *(float4*)&brick[dst[u]] = v[u];
+      }
+    } else {
     const int rows_per = 64 / Lx;
     const int rl = lane / Lx;
     const int lx = lane - rl * Lx;
+    const bool row_lane = rl < rows_per;
     int gx = bxi * b.Bx + lx;
     while (gx >= g.X) gx -= g.X;
     const int rows = Ly * Lz;
     const int rstep = nwv * rows_per;
+    const int rstep_z = rstep / Ly, rstep_y = rstep - rstep_z * Ly;
     int r = wid * rows_per + rl;
     int ly = r % Ly, lz = r / Ly;
-    for (; r < rows; r += rstep) {
-      if (rl < rows_per) {
-        int gy = byi * b.By + ly, gz = bzi * b.Bz + lz;
-        while (gy >= g.Y) gy -= g.Y;
-        while (gz >= g.Z) gz -= g.Z;
-        brick[lx + Lx * (ly + Ly * lz)] = phi[gx + (size_t)g.X * (gy + (size_t)g.Y * gz)];
+    for (; r < rows; r += kStageBatch * rstep) {
+      CT v[kStageBatch];
+      int dst[kStageBatch];
+#pragma unroll
+      for (int u = 0; u < kStageBatch; u++) {
+        const bool ok = row_lane && r + u * rstep < rows;
+        const int tz = lz < Lz ? lz : Lz - 1;  // rows past the end: any valid entry
+        dst[u] = ok ? lx + Px * (ly + Ly * lz) : -1;
+        v[u] = ok ? phi[gx + (size_t)g.X * (s_gy[ly] + (size_t)g.Y * s_gz[tz])] : (CT)0;
+        ly += rstep_y;
+        lz += rstep_z;
+        if (ly >= Ly) {
+          ly -= Ly;
+          lz++;
+        }
       }
-      ly += rstep;
-      while (ly >= Ly) {
-        ly -= Ly;
-        lz++;
-      }
+#pragma unroll
+      for (int u = 0; u < kStageBatch; u++)
+        if (dst[u] >= 0) brick[dst[u]] = v[u];
+    }
     }
   }
 
@@ -221,7 +294,7 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
     k_ok[k] = (k < KI) && idx < NXY;
     k_ty[k] = k_ok[k] ? idx / NX : 0;
     k_tx[k] = k_ok[k] ? idx - k_ty[k] * NX : 0;
-    k_off[k] = k_tx[k] + Lx * k_ty[k];
+    k_off[k] = k_tx[k] + Px * k_ty[k];
   }
 
   constexpr bool SPH = (FL & GF_SPHERE) != 0;
@@ -235,9 +308,9 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
   for (int c0 = s0; c0 < s1; c0 += cap) {
     const int na = min(cap, s1 - c0);
     // phase 0: particles and original indices into LDS, one coalesced pass
-    for (int t = threadIdx.x; t < na; t += BS) {
-      sX[t] = sorted[c0 + t];
-      sP[t] = perm[c0 + t];
+    if ((int)threadIdx.x < na) {
+      sX[threadIdx.x] = c0 == 0 ? pre : sorted[c0 + threadIdx.x];
+      sP[threadIdx.x] = c0 == 0 ? pre_p : perm[c0 + threadIdx.x];
     }
     __syncthreads();
     // phase 1: one wave per particle; lane j holds the reference gather's
@@ -257,7 +330,7 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
         const int ox = wrapi((int)round(x * g.ihx) - g.nx, g.X) - bxi * b.Bx;
         const int oy = wrapi((int)round(y * g.ihy) - g.ny, g.Y) - byi * b.By;
         const int oz = wrapi((int)round(z * g.ihz) - g.nz, g.Z) - bzi * b.Bz;
-        sB[ai] = ox + Lx * (oy + Ly * oz);
+        sB[ai] = ox + Px * (oy + Ly * oz);
         sQ[ai] = (double)p.w;
       }
     }
@@ -293,7 +366,7 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
         if (NZC > 0) {
 #pragma unroll
           for (int t = 0; t < NZA; t++) {
-            const CT ph = brick[a0 + t * LxLy];
+            const CT ph = brick[a0 + t * PxLy];
             const AT wph = (AT)(wzr[t] * ph);
             const AT dzt = (AT)dzr[t];
             if (NEED_B0) B0 += wph;
@@ -310,7 +383,7 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
             const CT wzt = w[2 * (NX + NY + t)];
             const CT dzc = w[2 * (NX + NY + t) + 1];
             const CT dzc2 = mul_rn(dzc, dzc);
-            const CT ph = brick[a0 + t * LxLy];
+            const CT ph = brick[a0 + t * PxLy];
             const AT wph = (AT)(wzt * ph);
             const AT dzt = (AT)dzc;
             if (NEED_B0) B0 += wph;
@@ -353,7 +426,7 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
       for (int c = 0; c < 13; c++) {
         const bool used = (c < 3 && SPH) || (c >= 3 && c < 6 && CUB) || (c == 6 && POT) ||
                           (c >= 7 && VIR);
-        if (used) f[c] = wave_sum(f[c]);
+        if (used) f[c] = wave_total(f[c]);
       }
       if (lane == 0) gather_store<AT, FL>(a, o, sP[ai], sQ[ai], ds, f);
     }
@@ -559,8 +632,8 @@ hipError_t launch_batch_gather_overflow(bool ct_dbl, bool at_dbl, const BatchDev
 
 size_t gather_lds_bytes(bool dbl, const Geom& g, const BinShape& b, int* cap_out) {
   const int NW = 2 * (g.nx + g.ny + g.nz) + 3;
-  const int vol = b.Lx * b.Ly * b.Lz;
   const int ct = dbl ? 8 : 4;
+  const int vol = brick_pitch(ct, g.X, b.Bx, b.Lx) * b.Ly * b.Lz;
   const size_t lds_total = (size_t)vol * ct + (size_t)tuning().gather_stage;
   int cap = 256;
   while (cap > 1 && gather_lds(vol, cap, NW, ct).total > lds_total) cap--;
@@ -582,7 +655,7 @@ static hipError_t gather_nz(GatherArgs a, const Binning& w, const void* phi, con
     return hipGetLastError();
   }
   const int NW = 2 * (a.g.nx + a.g.ny + a.g.nz) + 3;
-  const int vol = a.b.Lx * a.b.Ly * a.b.Lz;
+  const int vol = brick_pitch(sizeof(CT), a.g.X, a.b.Bx, a.b.Lx) * a.b.Ly * a.b.Lz;
   const Tuning& tu = tuning();
   const size_t lds_total = (size_t)vol * sizeof(CT) + (size_t)tu.gather_stage;
   int cap = 256;

@@ -48,6 +48,30 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// Wave total through DPP: xor-1 and xor-2 quad permutes, half-row and row
+// mirrors (16-lane sums in every lane), then the four row sums read out and
+// added in a fixed order.  The result is uniform and deterministic; no LDS
+// traffic, unlike the ds_bpermute butterfly of wave_sum.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <typename T>
+__device__ __forceinline__ T wave_total(T v) {
+  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
+  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
+  v += dpp_move<0x141>(v);  // row_half_mirror: 8-lane sums
+  v += dpp_move<0x140>(v);  // row_mirror: 16-lane sums
+  const T r0 = readlane(v, 0), r1 = readlane(v, 16), r2 = readlane(v, 32), r3 = readlane(v, 48);
+  return (r0 + r1) + (r2 + r3);
+}
+
 // Non-negative modulo
 __device__ __forceinline__ int wrapi(int i, int m) {
   int r = i % m;

@@ -44,6 +44,8 @@ struct Tuning {
   int gather_stage = 16 * 1024;
   int spread_lds32 = 1;        // FP32 grids: 32-bit accumulators with a per-bin scale
   int spread_cols = 0;         // column-owner spread in registers (else LDS-atomic brick)
+  int spread_flush = 1;        // 1: bricks stored and summed by k_brick_reduce; 0: atomics
+  int reduce_tile = 1;         // tiled brick reduce where the grid allows it
 };
 Tuning& tuning();
 
@@ -72,6 +74,8 @@ struct Binning {
   void* sorted = nullptr;      // [nbins*cap_bin] 4-vectors of the compute type
   int images = 1;              // cube images per grid point (2n+1 > M wraps)
   double pref_bound = 0.0;     // bound of |q|-free contribution: pref * max weight
+  void* bricks = nullptr;      // [nbins][Lz][Ly][Lx] LDS bricks (NULL: atomic flush)
+  int* bshift = nullptr;       // [nbins] brick shift to 2^e, -1 for an empty bin
 };
 
 // Fixed-point exponent e (scale 2^e) of one spread: every contribution
@@ -129,6 +133,13 @@ double spread_weight_bound(const Geom& g, double sigma, bool normalize);
 hipError_t launch_fixed_to_grid(bool ct_double, const unsigned long long* grid64,
                                 void* out, size_t count, const Binning& w,
                                 bool clear, hipStream_t s);
+
+// Brick-store flush: usable for this shape (LDS path, bounded brick overlap)?
+bool brick_reduce_ok(const Geom& g, const BinShape& b);
+// rho = 2^-e (sum of the bricks covering each point + the overflow int64 grid)
+hipError_t launch_brick_reduce(bool ct_double, bool acc32, const Geom& g, const BinShape& b,
+                               const Binning& w, unsigned long long* grid64, void* out,
+                               hipStream_t s);
 
 hipError_t launch_fill_u64(unsigned long long* p, size_t count, hipStream_t s);
 hipError_t launch_fill(bool dbl, void* p, size_t count, hipStream_t s);

@@ -20,6 +20,8 @@
 // max|q|, the particle count and the support (fixed_exponent).
 #include <math.h>
 
+#include <algorithm>
+
 #include "gse_device.h"
 #include "gse_internal.h"
 
@@ -238,14 +240,18 @@ __device__ __forceinline__ void spread_overflow(const SpreadArgs& a, int eg,
 // One bin of one system: the bin's cnt staged particles start at `sorted`
 // (slot 0 of the bin), the system's int64 grid at `grid64`; eg is the
 // system's exponent, qbits its max |q| and cnt_all the bin's particle count
-// including overflow (it sets the int32 scale).  Ends with a barrier, so a
-// workgroup may call it for several bins in a row.
+// including overflow (it sets the int32 scale).  The finished brick is either
+// added into grid64 with 64-bit atomics (brick_out NULL) or stored as it is
+// into brick_out, with its shift to the global scale in shift_out[bin], for
+// k_brick_reduce.  Ends with a barrier, so a workgroup may call it for
+// several bins in a row.
 template <typename CT, int NZC, int BS, typename ACC>
 __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt, int cnt_all,
                                            int eg, unsigned long long qbits,
                                            const V4<CT>* __restrict__ sorted,
                                            unsigned long long* __restrict__ grid64,
-                                           unsigned char* smem) {
+                                           unsigned char* smem, ACC* __restrict__ brick_out,
+                                           int* __restrict__ shift_out) {
   ACC* const brick = (ACC*)smem;
   const Geom& g = a.g;
   const BinShape& b = a.b;
@@ -271,6 +277,10 @@ __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt
   double* sW = (double*)(lds + lay.oW);  // [cap][NW]
   double* sQ = (double*)(lds + lay.oQ);  // [cap]
   int* sB = (int*)(lds + lay.oB);        // [cap]
+  // the first chunk's particle (cap <= 256 <= BS: at most one per thread) is
+  // loaded before the brick is cleared, so its latency overlaps the clear
+  V4<CT> pre{};
+  if ((int)threadIdx.x < min(cap, cnt)) pre = sorted[threadIdx.x];
   for (int j = threadIdx.x; j < vol; j += BS) brick[j] = (ACC)0;
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -301,7 +311,7 @@ __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt
   for (int c0 = 0; c0 < cnt; c0 += cap) {
     const int na = min(cap, cnt - c0);
     // phase 0: the chunk's particles into LDS in one coalesced pass
-    for (int t = threadIdx.x; t < na; t += BS) sX[t] = sorted[c0 + t];
+    if ((int)threadIdx.x < na) sX[threadIdx.x] = c0 == 0 ? pre : sorted[c0 + threadIdx.x];
     __syncthreads();
     // phase 1: one wave per particle, lane j computes factor j
     for (int ai = wid; ai < na; ai += nwv) {
@@ -365,6 +375,17 @@ __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt
     __syncthreads();
   }
 
+  if (brick_out) {  // plain coalesced stores; k_brick_reduce sums the bricks
+    if (sizeof(ACC) == 4 && (vol & 3) == 0) {  // 16-byte pieces (the brick is 32-byte aligned)
+      const int vq = vol >> 2;
+      for (int j = threadIdx.x; j < vq; j += BS) ((uint4*)brick_out)[j] = ((const uint4*)brick)[j];
+    } else {
+      for (int j = threadIdx.x; j < vol; j += BS) brick_out[j] = brick[j];
+    }
+    if (threadIdx.x == 0) shift_out[bin] = shift;
+    __syncthreads();
+    return;
+  }
   // flush: rows of Lx lanes (rows_per = 64/Lx per wave), no per-element
   // division; (ly, lz) advance by the constant row stride
   const int X = g.X, Y = g.Y, Z = g.Z;
@@ -405,7 +426,8 @@ __global__ void __launch_bounds__(BS)
     k_spread(SpreadArgs a, const V4<CT>* __restrict__ xyzq, const V4<CT>* __restrict__ sorted,
              const int* __restrict__ counts, int* __restrict__ counts_other,
              const BinStats* __restrict__ stats, BinStats* __restrict__ stats_other,
-             const int* __restrict__ ovf, unsigned long long* __restrict__ grid64) {
+             const int* __restrict__ ovf, unsigned long long* __restrict__ grid64,
+             ACC* __restrict__ bricks, int* __restrict__ bshift) {
   extern __shared__ __align__(32) unsigned char spread_smem[];
   const BinStats st = *stats;
   const int eg = fixed_exponent(st.qmax_bits, a.n - st.invalid, a.images, a.pref_bound);
@@ -425,10 +447,79 @@ __global__ void __launch_bounds__(BS)
   if (bin >= a.b.nbins) return;
   const int call = counts[bin];
   const int cnt = min(call, a.cap_bin);
-  if (threadIdx.x == 0) counts_other[bin] = 0;
+  if (threadIdx.x == 0) {
+    counts_other[bin] = 0;
+    if (bricks && cnt == 0) bshift[bin] = -1;  // no brick this step
+  }
   if (cnt == 0) return;
+  const size_t vol = (size_t)a.b.Lx * a.b.Ly * a.b.Lz;
   spread_bin<CT, NZC, BS, ACC>(a, bin, cnt, call, eg, st.qmax_bits,
-                               sorted + (size_t)bin * a.cap_bin, grid64, spread_smem);
+                               sorted + (size_t)bin * a.cap_bin, grid64, spread_smem,
+                               bricks ? bricks + (size_t)bin * vol : nullptr, bshift);
+}
+
+// Bricks covering grid coordinate p on one axis: every (bin b, brick cell l)
+// with (b*B + l) == p mod M, l < L.  Returns the count (at most KR).
+constexpr int KR = 8;
+__device__ __forceinline__ int brick_axis(int p, int M, int B, int nb, int L, int (&bl)[KR],
+                                          int (&ll)[KR]) {
+  int c = 0;
+  for (int q = p; q - (nb - 1) * B < L; q += M) {
+    int blo = q - L + 1;
+    blo = blo <= 0 ? 0 : (blo + B - 1) / B;
+    const int bhi = min(q / B, nb - 1);
+    for (int b = blo; b <= bhi && c < KR; b++) {
+      bl[c] = b;
+      ll[c] = q - b * B;
+      c++;
+    }
+  }
+  return c;
+}
+
+// Grid = sum of the bricks covering each point (int64, exact, fixed order)
+// plus the overflow particles' int64 grid, times 2^-e; the int64 grid is
+// cleared again where it was used.
+template <typename T, typename ACC>
+__global__ void __launch_bounds__(256)
+    k_brick_reduce(Geom g, BinShape b, const ACC* __restrict__ bricks,
+                   const int* __restrict__ bshift, const BinStats* __restrict__ stats, int n,
+                   int images, double pref_bound, unsigned long long* __restrict__ grid64,
+                   T* __restrict__ out) {
+  const BinStats st = *stats;
+  const double inv =
+      ldexp(1.0, -fixed_exponent(st.qmax_bits, n - st.invalid, images, pref_bound));
+  const bool ovf = st.overflow > 0;
+  const size_t vol = (size_t)b.Lx * b.Ly * b.Lz;
+  const size_t np = (size_t)g.X * g.Y * g.Z;
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < np;
+       p += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(p % g.X);
+    const size_t t = p / g.X;
+    const int y = (int)(t % g.Y), z = (int)(t / g.Y);
+    int bx[KR], lx[KR], by[KR], ly[KR], bz[KR], lz[KR];
+    const int nx = brick_axis(x, g.X, b.Bx, b.nbx, b.Lx, bx, lx);
+    const int ny = brick_axis(y, g.Y, b.By, b.nby, b.Ly, by, ly);
+    const int nz = brick_axis(z, g.Z, b.Bz, b.nbz, b.Lz, bz, lz);
+    long long sum = 0;
+    for (int iz = 0; iz < nz; iz++)
+      for (int iy = 0; iy < ny; iy++) {
+        const int brow = b.nbx * (by[iy] + b.nby * bz[iz]);
+        const int crow = b.Lx * (ly[iy] + b.Ly * lz[iz]);
+        for (int ix = 0; ix < nx; ix++) {
+          const int bin = bx[ix] + brow;
+          const int sh = bshift[bin];
+          if (sh < 0) continue;
+          const ACC raw = bricks[(size_t)bin * vol + lx[ix] + crow];
+          sum += sizeof(ACC) == 4 ? ((long long)(int)raw << sh) : (long long)raw;
+        }
+      }
+    if (ovf) {
+      sum += (long long)grid64[p];
+      grid64[p] = 0ull;
+    }
+    out[p] = (T)((double)sum * inv);
+  }
 }
 
 // Column-owner spread: thread j owns brick column (x, y) = (j % Lx, j / Lx)
@@ -680,6 +771,132 @@ static unsigned grid_for(size_t count, unsigned threads, unsigned cap = 4096) {
   return (unsigned)g;
 }
 
+// Tiled form of k_brick_reduce for bins of 8^3 cells on grids that are whole
+// numbers of bins (>= 3 per axis) with bricks at most 3 bins wide: one
+// 512-thread workgroup per bin interior, each thread one grid point; the
+// bricks of the bin and of its neighbours at -1, -2 bins that cover it are
+// walked as a uniform loop, so brick shifts are scalar loads and each brick
+// row is read as 8 consecutive values.
+template <typename T, typename ACC>
+__global__ void __launch_bounds__(512)
+    k_brick_reduce_tile(Geom g, BinShape b, int nwg, const ACC* __restrict__ bricks,
+                        const int* __restrict__ bshift, const BinStats* __restrict__ stats, int n,
+                        int images, double pref_bound, unsigned long long* __restrict__ grid64,
+                        T* __restrict__ out) {
+  const int bin = xcd_remap(blockIdx.x, nwg);
+  if (bin >= b.nbins) return;
+  const BinStats st = *stats;
+  const double inv =
+      ldexp(1.0, -fixed_exponent(st.qmax_bits, n - st.invalid, images, pref_bound));
+  const int cx = bin % b.nbx, cy = (bin / b.nbx) % b.nby, cz = bin / (b.nbx * b.nby);
+  const int tx = threadIdx.x & 7, ty = (threadIdx.x >> 3) & 7, tz = threadIdx.x >> 6;
+  const int Lx = b.Lx, Ly = b.Ly, Lz = b.Lz;
+  const size_t vol = (size_t)Lx * Ly * Lz;
+  // the 27 candidate bricks (the bin and its neighbours at -1, -2 bins on
+  // each axis): every shift and every value is loaded before any is summed,
+  // so the loads overlap instead of each waiting for the previous one
+  int sh[27];
+  ACC raw[27];
+#pragma unroll
+  for (int k = 0; k < 27; k++) {
+    const int dx = k % 3, dy = (k / 3) % 3, dz = k / 9;
+    int bx = cx - dx, by = cy - dy, bz = cz - dz;
+    bx += bx < 0 ? b.nbx : 0;
+    by += by < 0 ? b.nby : 0;
+    bz += bz < 0 ? b.nbz : 0;
+    sh[k] = bshift[bx + b.nbx * (by + b.nby * bz)];
+  }
+#pragma unroll
+  for (int k = 0; k < 27; k++) {
+    const int dx = k % 3, dy = (k / 3) % 3, dz = k / 9;
+    int bx = cx - dx, by = cy - dy, bz = cz - dz;
+    bx += bx < 0 ? b.nbx : 0;
+    by += by < 0 ? b.nby : 0;
+    bz += bz < 0 ? b.nbz : 0;
+    const int lx = tx + 8 * dx, ly = ty + 8 * dy, lz = tz + 8 * dz;
+    const bool ok = sh[k] >= 0 && lx < Lx && ly < Ly && lz < Lz;
+    raw[k] = ok ? bricks[(size_t)(bx + b.nbx * (by + b.nby * bz)) * vol + lx + Lx * (ly + Ly * lz)]
+                : (ACC)0;
+  }
+  long long sum = 0;
+#pragma unroll
+  for (int k = 0; k < 27; k++)
+    if (sh[k] >= 0)
+      sum += sizeof(ACC) == 4 ? ((long long)(int)raw[k] << sh[k]) : (long long)raw[k];
+  const size_t p = (size_t)(8 * cx + tx) + (size_t)g.X * ((8 * cy + ty) + (size_t)g.Y * (8 * cz + tz));
+  if (st.overflow > 0) {
+    sum += (long long)grid64[p];
+    grid64[p] = 0ull;
+  }
+  out[p] = (T)((double)sum * inv);
+}
+
+// k_brick_reduce_tile with 32-bit bricks whose rows are whole 16-byte
+// pieces (Lx a multiple of 4): 128 threads per bin interior, each thread four
+// consecutive x points read and written as 16-byte pieces, the 27 candidate
+// bricks taken one z layer (9 bricks) at a time.
+template <typename T>
+__global__ void __launch_bounds__(128)
+    k_brick_reduce_tile4(Geom g, BinShape b, int nwg, const unsigned* __restrict__ bricks,
+                         const int* __restrict__ bshift, const BinStats* __restrict__ stats,
+                         int n, int images, double pref_bound,
+                         unsigned long long* __restrict__ grid64, T* __restrict__ out) {
+  const int bin = xcd_remap(blockIdx.x, nwg);
+  if (bin >= b.nbins) return;
+  const BinStats st = *stats;
+  const double inv =
+      ldexp(1.0, -fixed_exponent(st.qmax_bits, n - st.invalid, images, pref_bound));
+  const int cx = bin % b.nbx, cy = (bin / b.nbx) % b.nby, cz = bin / (b.nbx * b.nby);
+  const int tq = threadIdx.x & 1, ty = (threadIdx.x >> 1) & 7, tz = threadIdx.x >> 4;
+  const int Lx = b.Lx, Ly = b.Ly, Lz = b.Lz;
+  const size_t vol = (size_t)Lx * Ly * Lz;
+  long long sum[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int dz = 0; dz < 3; dz++) {
+    int sh[9];
+    uint4 raw[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const int dx = k % 3, dy = k / 3;
+      int bx = cx - dx, by = cy - dy, bz = cz - dz;
+      bx += bx < 0 ? b.nbx : 0;
+      by += by < 0 ? b.nby : 0;
+      bz += bz < 0 ? b.nbz : 0;
+      const int nbin = bx + b.nbx * (by + b.nby * bz);
+      sh[k] = bshift[nbin];
+      const int lx = 4 * tq + 8 * dx, ly = ty + 8 * dy, lz = tz + 8 * dz;
+      const bool ok = sh[k] >= 0 && lx < Lx && ly < Ly && lz < Lz;
+      raw[k] = ok ? *(const uint4*)&bricks[(size_t)nbin * vol + lx + Lx * (ly + Ly * lz)]
+                  : make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      if (sh[k] < 0) continue;
+      sum[0] += (long long)(int)raw[k].x << sh[k];
+      sum[1] += (long long)(int)raw[k].y << sh[k];
+      sum[2] += (long long)(int)raw[k].z << sh[k];
+      sum[3] += (long long)(int)raw[k].w << sh[k];
+    }
+  }
+  const size_t p = (size_t)(8 * cx + 4 * tq) + (size_t)g.X * ((8 * cy + ty) + (size_t)g.Y * (8 * cz + tz));
+  if (st.overflow > 0) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      sum[c] += (long long)grid64[p + c];
+      grid64[p + c] = 0ull;
+    }
+  }
+  T o4[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) o4[c] = (T)((double)sum[c] * inv);
+  if (sizeof(T) == 4) {
+    *(float4*)&out[p] = make_float4((float)o4[0], (float)o4[1], (float)o4[2], (float)o4[3]);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; c++) out[p + c] = o4[c];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Batched independent systems: the same per-bin bodies, one launch for all
 // systems.  Spread and gather walk a table of (system, bin) items with a
@@ -808,7 +1025,7 @@ __global__ void __launch_bounds__(BS)
     const SpreadArgs a = batch_spread_args(S);
     spread_bin<CT, NZC, BS, ACC>(a, bin, cnt, call, eg, st.qmax_bits,
                                  (const V4<CT>*)d.sorted + S.slot0 + (long long)bin * S.cap_bin,
-                                 d.grid64 + S.gofs, spread_smem);
+                                 d.grid64 + S.gofs, spread_smem, (ACC*)nullptr, nullptr);
   }
 }
 
@@ -1020,7 +1237,8 @@ static hipError_t spread_t(const Geom& g, const BinShape& b, double sigma, bool 
   const int bs = tu.spread_block ? tu.spread_block : (sizeof(CT) == 4 ? 512 : 256);
 #define GSE_SPREAD_LAUNCH(NZV, BSV, ACCT)                                                    \
   k_spread<CT, NZV, BSV, ACCT><<<nwg, BSV, lds, s>>>(a, xq, srt, w.counts, w.counts_other,  \
-                                                     w.stats, w.stats_other, w.ovf, grid64)
+                                                     w.stats, w.stats_other, w.ovf, grid64, \
+                                                     (ACCT*)w.bricks, w.bshift)
 #define GSE_SPREAD_BS(NZV, ACCT)                      \
   if (bs == 512)                                      \
     GSE_SPREAD_LAUNCH(NZV, 512, ACCT);                \
@@ -1076,6 +1294,75 @@ hipError_t launch_fixed_to_grid(bool dbl, const unsigned long long* grid64, void
     k_fixed_to_grid<float><<<grid, 256, 0, s>>>((unsigned long long*)grid64, (float*)out, count,
                                                 w.stats, w.n, w.images, w.pref_bound,
                                                 clear ? 1 : 0);
+  return hipGetLastError();
+}
+
+static int brick_axis_max(int M, int B, int nb, int L) {
+  int worst = 0;
+  for (int p = 0; p < M; p++) {
+    int c = 0;
+    for (int q = p; q - (nb - 1) * B < L; q += M) {
+      int blo = q - L + 1;
+      blo = blo <= 0 ? 0 : (blo + B - 1) / B;
+      const int bhi = std::min(q / B, nb - 1);
+      if (bhi >= blo) c += bhi - blo + 1;
+    }
+    worst = std::max(worst, c);
+  }
+  return worst;
+}
+
+bool brick_reduce_ok(const Geom& g, const BinShape& b) {
+  if (b.direct) return false;
+  return brick_axis_max(g.X, b.Bx, b.nbx, b.Lx) <= KR &&
+         brick_axis_max(g.Y, b.By, b.nby, b.Ly) <= KR &&
+         brick_axis_max(g.Z, b.Bz, b.nbz, b.Lz) <= KR;
+}
+
+static bool brick_tile_ok(const Geom& g, const BinShape& b) {
+  return b.Bx == 8 && b.By == 8 && b.Bz == 8 && g.X % 8 == 0 && g.Y % 8 == 0 && g.Z % 8 == 0 &&
+         b.nbx >= 3 && b.nby >= 3 && b.nbz >= 3 && b.Lx <= 24 && b.Ly <= 24 && b.Lz <= 24;
+}
+
+hipError_t launch_brick_reduce(bool dbl, bool acc32, const Geom& g, const BinShape& b,
+                               const Binning& w, unsigned long long* grid64, void* out,
+                               hipStream_t s) {
+  if (brick_tile_ok(g, b) && tuning().reduce_tile) {
+    const int nwg = (b.nbins + 7) / 8 * 8;
+    if (!dbl && acc32 && b.Lx % 4 == 0) {
+      k_brick_reduce_tile4<float><<<nwg, 128, 0, s>>>(g, b, nwg, (const unsigned*)w.bricks,
+                                                      w.bshift, w.stats, w.n, w.images,
+                                                      w.pref_bound, grid64, (float*)out);
+      return hipGetLastError();
+    }
+    if (dbl)
+      k_brick_reduce_tile<double, unsigned long long><<<nwg, 512, 0, s>>>(
+          g, b, nwg, (const unsigned long long*)w.bricks, w.bshift, w.stats, w.n, w.images,
+          w.pref_bound, grid64, (double*)out);
+    else if (acc32)
+      k_brick_reduce_tile<float, unsigned><<<nwg, 512, 0, s>>>(
+          g, b, nwg, (const unsigned*)w.bricks, w.bshift, w.stats, w.n, w.images, w.pref_bound,
+          grid64, (float*)out);
+    else
+      k_brick_reduce_tile<float, unsigned long long><<<nwg, 512, 0, s>>>(
+          g, b, nwg, (const unsigned long long*)w.bricks, w.bshift, w.stats, w.n, w.images,
+          w.pref_bound, grid64, (float*)out);
+    return hipGetLastError();
+  }
+  const size_t np = (size_t)g.X * g.Y * g.Z;
+  const unsigned grid = grid_for(np, 256, 8192);
+  if (dbl)
+    k_brick_reduce<double, unsigned long long><<<grid, 256, 0, s>>>(
+        g, b, (const unsigned long long*)w.bricks, w.bshift, w.stats, w.n, w.images,
+        w.pref_bound, grid64, (double*)out);
+  else if (acc32)
+    k_brick_reduce<float, unsigned><<<grid, 256, 0, s>>>(g, b, (const unsigned*)w.bricks,
+                                                          w.bshift, w.stats, w.n, w.images,
+                                                          w.pref_bound, grid64, (float*)out);
+  else
+    k_brick_reduce<float, unsigned long long><<<grid, 256, 0, s>>>(
+        g, b, (const unsigned long long*)w.bricks, w.bshift, w.stats, w.n, w.images,
+        w.pref_bound, grid64, (float*)out);
   return hipGetLastError();
 }
 

@@ -65,7 +65,10 @@ GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
  *   spread_stage  LDS bytes for staged particles in the spread (16 KiB)
  *   gather_stage  LDS bytes for staged particles in the gather (16 KiB)
  *   spread_lds32  FP32 grids: 32-bit accumulators, per-bin scale (0/1, 1)
- *   spread_cols   column-owner register spread (0/1, 0); 0 = LDS-atomic brick */
+ *   spread_cols   column-owner register spread (0/1, 0); 0 = LDS-atomic brick
+ *   spread_flush  1: each bin stores its brick and one pass sums the bricks
+ *                 covering each grid point; 0: bricks added with atomics (0/1, 1)
+ *   reduce_tile   that sum by one workgroup per 8^3 bin where the grid allows (0/1, 1) */
 GSE_API gse_status_t gse_set_tuning_param(const char* name, int value);
 GSE_API gse_status_t gse_get_tuning_param(const char* name, int* value);
 
