@@ -118,7 +118,7 @@ gse_status_t check_sigma_rcut(double sigma, double rcut) {
 struct Engine {
   bool ct_dbl = true, at_dbl = true;
   int X = 0, Y = 0, Z = 0;
-  DevBuf counts[2], stats[2], ovf, perm, sorted, grid64, vir6, part, bricks, bshift;
+  DevBuf counts[2], stats[2], ovf, perm, loc, sorted, grid64, vir6, part, bricks, bshift;
   bool brick_flush = false;     // the last spread stored bricks (k_brick_reduce converts)
   int cur = 1;                  // buffer of the latest binning
   long long clean[2] = {0, 0};  // leading counts (and the stats) known zero
@@ -128,7 +128,7 @@ struct Engine {
   BinShape shape{};
 
   void release() {
-    for (DevBuf* b : {&counts[0], &counts[1], &stats[0], &stats[1], &ovf, &perm, &sorted,
+    for (DevBuf* b : {&counts[0], &counts[1], &stats[0], &stats[1], &ovf, &perm, &loc, &sorted,
                       &grid64, &vir6, &part, &bricks, &bshift})
       b->release();
   }
@@ -154,6 +154,7 @@ struct Engine {
     const size_t slots = (size_t)nb * cap_bin;
     GSECHK(sorted.ensure(4 * tsize(ct_dbl) * slots));
     GSECHK(perm.ensure(sizeof(int) * slots));
+    GSECHK(loc.ensure(sizeof(int) * slots));
     GSECHK(ovf.ensure(sizeof(int) * (size_t)(n > 0 ? n : 1)));
     GSECHK(part.ensure(sizeof(double) * kRedBlocks * 8));
     for (int k = 0; k < 2; k++) {
@@ -188,6 +189,7 @@ struct Engine {
     bin.stats_other = stats[c ^ 1].as<BinStats>();
     bin.ovf = ovf.as<int>();
     bin.perm = perm.as<int>();
+    bin.loc = loc.as<int>();
     bin.sorted = sorted.p;
     bin.images = ((2 * g.nx + 1 + X - 1) / X) * ((2 * g.ny + 1 + Y - 1) / Y) *
                  ((2 * g.nz + 1 + Z - 1) / Z);
@@ -505,7 +507,7 @@ struct gse_batch_s {
   long long total_points = 0, total_k = 0, total_slots = 0;
   int total_bins = 0;
   DevBuf d_sys, d_pofs, d_items, d_cchunks, d_kchunks, counts_b, stats_b, ovf_b, novf_b, perm_b,
-      sorted_b, grid64_b, rho_b, phi_b, spec_b, table_b, part_b;
+      loc_b, sorted_b, grid64_b, rho_b, phi_b, spec_b, table_b, part_b;
   BatchDev dev;
   hipStream_t stream = nullptr;
   // graph of one step
@@ -528,8 +530,8 @@ struct gse_batch_s {
     release_graph();
     release_ffts();
     for (DevBuf* b : {&d_sys, &d_pofs, &d_items, &d_cchunks, &d_kchunks, &counts_b, &stats_b,
-                      &ovf_b, &novf_b, &perm_b, &sorted_b, &grid64_b, &rho_b, &phi_b, &spec_b,
-                      &table_b, &part_b})
+                      &ovf_b, &novf_b, &perm_b, &loc_b, &sorted_b, &grid64_b, &rho_b, &phi_b,
+                      &spec_b, &table_b, &part_b})
       b->release();
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
     cap_stream = nullptr;
@@ -1345,6 +1347,7 @@ gse_status_t batch_layout(gse_batch_s* h, hipStream_t st) {
   GSECHK(h->ovf_b.ensure(sizeof(int) * n1));
   GSECHK(h->novf_b.ensure(sizeof(int)));
   GSECHK(h->perm_b.ensure(sizeof(int) * std::max<long long>(1, h->total_slots)));
+  GSECHK(h->loc_b.ensure(sizeof(int) * std::max<long long>(1, h->total_slots)));
   GSECHK(h->sorted_b.ensure(4 * ct * std::max<long long>(1, h->total_slots)));
   GSECHK(h->grid64_b.ensure(8 * (size_t)h->total_points));
   GSECHK(h->rho_b.ensure(ct * (size_t)h->total_points));
@@ -1375,6 +1378,7 @@ gse_status_t batch_layout(gse_batch_s* h, hipStream_t st) {
   d.ovf = h->ovf_b.as<int>();
   d.novf = h->novf_b.as<int>();
   d.perm = h->perm_b.as<int>();
+  d.loc = h->loc_b.as<int>();
   d.sorted = h->sorted_b.p;
   d.grid64 = h->grid64_b.as<unsigned long long>();
   HIPCHK(launch_fill_u64(d.grid64, (size_t)h->total_points, st));

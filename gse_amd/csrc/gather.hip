@@ -163,6 +163,7 @@ template <typename CT, typename AT, int NZC, int FL, int BS>
 __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt,
                                            const V4<CT>* __restrict__ sorted,
                                            const int* __restrict__ perm,
+                                           const int* __restrict__ loc,
                                            const CT* __restrict__ phi, const GatherOut& o,
                                            double ds, double* smem_raw) {
   const Geom& g = a.g;
@@ -190,10 +191,11 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
   // the first chunk's particles (cap <= 256 <= BS: at most one per thread)
   // are loaded first, so their latency overlaps the brick staging
   V4<CT> pre{};
-  int pre_p = 0;
+  int pre_p = 0, pre_l = 0;
   if ((int)threadIdx.x < min(a.cap, cnt)) {
     pre = sorted[threadIdx.x];
     pre_p = perm[threadIdx.x];
+    pre_l = loc[threadIdx.x];
   }
   {  // stage the potential brick: rows of Lx lanes, no per-element division.
      // Loads go out in batches of kStageBatch before their LDS stores, so a
@@ -304,13 +306,22 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
   constexpr bool NEED_B0 = CUB || POT || VIR;
   constexpr bool NEED_B1 = CUB || VIR;
   constexpr int NZA = NZC > 0 ? NZC : 1;
+  // FP32 without the virial: the sphere test dx^2+dy^2 < rc^2-dz^2 with the
+  // right side kept per z (one compare per point).  FP64 keeps the
+  // reference's rounding, (dx^2+dy^2)+dz^2 < rc^2 (CpuGaussCharge.cpp:157).
+  constexpr bool FAST_SPH = SPH && !VIR && sizeof(CT) == 4;
 
   for (int c0 = s0; c0 < s1; c0 += cap) {
     const int na = min(cap, s1 - c0);
     // phase 0: particles and original indices into LDS, one coalesced pass
     if ((int)threadIdx.x < na) {
-      sX[threadIdx.x] = c0 == 0 ? pre : sorted[c0 + threadIdx.x];
+      const V4<CT> p = c0 == 0 ? pre : sorted[c0 + threadIdx.x];
+      const int l = c0 == 0 ? pre_l : loc[c0 + threadIdx.x];
+      sX[threadIdx.x] = p;
       sP[threadIdx.x] = c0 == 0 ? pre_p : perm[c0 + threadIdx.x];
+      // brick offset of the cube origin (stored by k_bin) and the charge
+      sB[threadIdx.x] = (l & 255) + Px * (((l >> 8) & 255) + Ly * (l >> 16));
+      sQ[threadIdx.x] = (double)p.w;
     }
     __syncthreads();
     // phase 1: one wave per particle; lane j holds the reference gather's
@@ -325,14 +336,6 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
         sW[2 * ((size_t)ai * NW + lane)] = gexp(mul_rn(mul_rn(-d, d), inv2s2));
         sW[2 * ((size_t)ai * NW + lane) + 1] = d;
       }
-      if (lane == 0) {
-        const double x = (double)p.x, y = (double)p.y, z = (double)p.z;
-        const int ox = wrapi((int)round(x * g.ihx) - g.nx, g.X) - bxi * b.Bx;
-        const int oy = wrapi((int)round(y * g.ihy) - g.ny, g.Y) - byi * b.By;
-        const int oz = wrapi((int)round(z * g.ihz) - g.nz, g.Z) - bzi * b.Bz;
-        sB[ai] = ox + Px * (oy + Ly * oz);
-        sQ[ai] = (double)p.w;
-      }
     }
     __syncthreads();
     // phase 2: one wave per particle
@@ -345,7 +348,7 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
         for (int t = 0; t < NZA; t++) {
           wzr[t] = w[2 * (NX + NY + t)];
           dzr[t] = w[2 * (NX + NY + t) + 1];
-          dz2[t] = mul_rn(dzr[t], dzr[t]);
+          dz2[t] = FAST_SPH ? rc2 - mul_rn(dzr[t], dzr[t]) : mul_rn(dzr[t], dzr[t]);
         }
       }
       AT f[13];
@@ -373,7 +376,8 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
             if (NEED_B1) B1 = fmac(wph, dzt, B1);
             if (VIR) B2 = fmac(wph, (AT)dz2[t], B2);
             if (SPH) {
-              const AT ws = add_rn(dxy2, dz2[t]) < rc2 ? wph : (AT)0;
+              const bool in = FAST_SPH ? dxy2 < dz2[t] : add_rn(dxy2, dz2[t]) < rc2;
+              const AT ws = in ? wph : (AT)0;
               A0 += ws;
               A1 = fmac(ws, dzt, A1);
             }
@@ -441,7 +445,7 @@ __global__ void __launch_bounds__(BS)
              const int* __restrict__ perm, const int* __restrict__ counts,
              int* __restrict__ counts_other, const BinStats* __restrict__ stats,
              BinStats* __restrict__ stats_other, const int* __restrict__ ovf,
-             const CT* __restrict__ phi, GatherOut o) {
+             const CT* __restrict__ phi, GatherOut o, const int* __restrict__ loc) {
   extern __shared__ __align__(32) double smem_raw[];
   const double ds = o.dscale ? *o.dscale : 1.0;
   if ((int)blockIdx.x == a.nwg_bins + kOverflowWG) {  // spare workgroup: k-pass finalize
@@ -469,7 +473,8 @@ __global__ void __launch_bounds__(BS)
   if (threadIdx.x == 0) counts_other[bin] = 0;
   if (cnt == 0) return;
   const size_t s0 = (size_t)bin * a.cap_bin;
-  gather_bin<CT, AT, NZC, FL, BS>(a, bin, cnt, sorted + s0, perm + s0, phi, o, ds, smem_raw);
+  gather_bin<CT, AT, NZC, FL, BS>(a, bin, cnt, sorted + s0, perm + s0, loc + s0, phi, o, ds,
+                                  smem_raw);
 }
 
 // Supports too wide for the LDS path: every binned particle (one bin of
@@ -545,8 +550,8 @@ __global__ void __launch_bounds__(BS)
     if (cnt == 0) continue;
     const long long s0 = S.slot0 + (long long)bin * S.cap_bin;
     gather_bin<CT, AT, NZC, FL, BS>(batch_gather_args(S), bin, cnt,
-                                    (const V4<CT>*)d.sorted + s0, d.perm + s0, phi + S.gofs, o,
-                                    1.0, smem_raw);
+                                    (const V4<CT>*)d.sorted + s0, d.perm + s0, d.loc + s0,
+                                    phi + S.gofs, o, 1.0, smem_raw);
   }
 }
 
@@ -666,7 +671,8 @@ static hipError_t gather_nz(GatherArgs a, const Binning& w, const void* phi, con
   a.nwg_bins = ((a.b.nbins + 7) / 8) * 8;
   const unsigned nwg = (unsigned)(a.nwg_bins + kOverflowWG + (o.fin_partials ? 1 : 0));
   const int bs = tu.gather_block ? tu.gather_block : (sizeof(CT) == 4 ? 512 : 256);
-#define GSE_G_ARGS a, xq, srt, w.perm, w.counts, w.counts_other, w.stats, w.stats_other, w.ovf, ph, o
+#define GSE_G_ARGS a, xq, srt, w.perm, w.counts, w.counts_other, w.stats, w.stats_other, w.ovf, ph, o, \
+                   w.loc
 #define GSE_G_BS(NZV)                                                      \
   if (bs == 512)                                                           \
     k_gather<CT, AT, NZV, FL, 512><<<nwg, 512, lds, s>>>(GSE_G_ARGS);     \

@@ -71,6 +71,7 @@ struct Binning {
   BinStats* stats_other = nullptr;
   int* ovf = nullptr;          // [n]
   int* perm = nullptr;         // [nbins*cap_bin]
+  int* loc = nullptr;          // [nbins*cap_bin] cube origin in the bin: ox | oy << 8 | oz << 16
   void* sorted = nullptr;      // [nbins*cap_bin] 4-vectors of the compute type
   int images = 1;              // cube images per grid point (2n+1 > M wraps)
   double pref_bound = 0.0;     // bound of |q|-free contribution: pref * max weight
@@ -225,6 +226,7 @@ struct BatchDev {
   int* ovf = nullptr;         // [ntot] global particle indices
   int* novf = nullptr;        // [1]
   int* perm = nullptr;        // [slots] global particle index
+  int* loc = nullptr;         // [slots] cube origin in the bin (ox | oy << 8 | oz << 16)
   void* sorted = nullptr;     // [slots] 4-vectors
   unsigned long long* grid64 = nullptr;
 };

@@ -54,7 +54,7 @@ template <typename CT>
 __global__ void __launch_bounds__(256)
     k_bin(Geom g, BinShape b, int n, int cap_bin, const V4<CT>* __restrict__ xyzq,
           int* __restrict__ counts, BinStats* __restrict__ stats, int* __restrict__ ovf,
-          int* __restrict__ perm, V4<CT>* __restrict__ sorted) {
+          int* __restrict__ perm, int* __restrict__ loc, V4<CT>* __restrict__ sorted) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double aq = 0.0;
   if (i < n) {
@@ -71,12 +71,14 @@ __global__ void __launch_bounds__(256)
       const int ox = wrapi((int)round(fx) - g.nx, g.X);
       const int oy = wrapi((int)round(fy) - g.ny, g.Y);
       const int oz = wrapi((int)round(fz) - g.nz, g.Z);
-      const int bin = ox / b.Bx + b.nbx * (oy / b.By + b.nby * (oz / b.Bz));
+      const int qx = ox / b.Bx, qy = oy / b.By, qz = oz / b.Bz;
+      const int bin = qx + b.nbx * (qy + b.nby * qz);
       const int slot = atomicAdd(&counts[bin], 1);
       if (slot < cap_bin) {
         const size_t pos = (size_t)bin * cap_bin + slot;
         sorted[pos] = p;
         perm[pos] = i;
+        loc[pos] = (ox - qx * b.Bx) | ((oy - qy * b.By) << 8) | ((oz - qz * b.Bz) << 16);
       } else {
         ovf[atomicAdd(&stats->overflow, 1)] = i;
       }
@@ -249,6 +251,7 @@ template <typename CT, int NZC, int BS, typename ACC>
 __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt, int cnt_all,
                                            int eg, unsigned long long qbits,
                                            const V4<CT>* __restrict__ sorted,
+                                           const int* __restrict__ loc,
                                            unsigned long long* __restrict__ grid64,
                                            unsigned char* smem, ACC* __restrict__ brick_out,
                                            int* __restrict__ shift_out) {
@@ -280,7 +283,11 @@ __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt
   // the first chunk's particle (cap <= 256 <= BS: at most one per thread) is
   // loaded before the brick is cleared, so its latency overlaps the clear
   V4<CT> pre{};
-  if ((int)threadIdx.x < min(cap, cnt)) pre = sorted[threadIdx.x];
+  int pre_l = 0;
+  if ((int)threadIdx.x < min(cap, cnt)) {
+    pre = sorted[threadIdx.x];
+    pre_l = loc[threadIdx.x];
+  }
   for (int j = threadIdx.x; j < vol; j += BS) brick[j] = (ACC)0;
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -310,24 +317,26 @@ __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt
 
   for (int c0 = 0; c0 < cnt; c0 += cap) {
     const int na = min(cap, cnt - c0);
-    // phase 0: the chunk's particles into LDS in one coalesced pass
-    if ((int)threadIdx.x < na) sX[threadIdx.x] = c0 == 0 ? pre : sorted[c0 + threadIdx.x];
+    // phase 0: the chunk's particles into LDS in one coalesced pass, with
+    // their brick offsets (from the cube origin k_bin stored) and scaled charges
+    if ((int)threadIdx.x < na) {
+      const V4<CT> p = c0 == 0 ? pre : sorted[c0 + threadIdx.x];
+      const int l = c0 == 0 ? pre_l : loc[c0 + threadIdx.x];
+      sX[threadIdx.x] = p;
+      sB[threadIdx.x] = (l & 255) + Lx * (((l >> 8) & 255) + Ly * (l >> 16));
+      sQ[threadIdx.x] = ((double)p.w * a.pref) * Sb;
+    }
     __syncthreads();
-    // phase 1: one wave per particle, lane j computes factor j
+    // phase 1: one wave per particle, lane j computes factor j of its axis
     for (int ai = wid; ai < na; ai += nwv) {
-      const V4<CT> p = sX[ai];
-      const double x = (double)p.x, y = (double)p.y, z = (double)p.z;
-      const int cx = (int)round(x * g.ihx) - g.nx;
-      const int cy = (int)round(y * g.ihy) - g.ny;
-      const int cz = (int)round(z * g.ihz) - g.nz;
-      if (role.axis >= 0)
-        sW[(size_t)ai * NW + lane] = (double)lane_weight<CT>(role, cx, cy, cz, x, y, z, g, a.inv2s2);
-      if (lane == 0) {
-        const int ox = wrapi(cx, g.X) - bxi * b.Bx;  // in [0, Bx)
-        const int oy = wrapi(cy, g.Y) - byi * b.By;
-        const int oz = wrapi(cz, g.Z) - bzi * b.Bz;
-        sB[ai] = ox + Lx * (oy + Ly * oz);
-        sQ[ai] = ((double)p.w * a.pref) * Sb;
+      if (role.axis >= 0) {
+        const V4<CT> p = sX[ai];
+        const double xv = role.axis == 0 ? (double)p.x : role.axis == 1 ? (double)p.y : (double)p.z;
+        const double ih = role.axis == 0 ? g.ihx : role.axis == 1 ? g.ihy : g.ihz;
+        const int n = role.axis == 0 ? g.nx : role.axis == 1 ? g.ny : g.nz;
+        const int c = (int)round(xv * ih) - n;
+        sW[(size_t)ai * NW + lane] =
+            (double)lane_weight<CT>(role, c, c, c, xv, xv, xv, g, a.inv2s2);
       }
     }
     __syncthreads();
@@ -427,7 +436,7 @@ __global__ void __launch_bounds__(BS)
              const int* __restrict__ counts, int* __restrict__ counts_other,
              const BinStats* __restrict__ stats, BinStats* __restrict__ stats_other,
              const int* __restrict__ ovf, unsigned long long* __restrict__ grid64,
-             ACC* __restrict__ bricks, int* __restrict__ bshift) {
+             ACC* __restrict__ bricks, int* __restrict__ bshift, const int* __restrict__ loc) {
   extern __shared__ __align__(32) unsigned char spread_smem[];
   const BinStats st = *stats;
   const int eg = fixed_exponent(st.qmax_bits, a.n - st.invalid, a.images, a.pref_bound);
@@ -454,7 +463,8 @@ __global__ void __launch_bounds__(BS)
   if (cnt == 0) return;
   const size_t vol = (size_t)a.b.Lx * a.b.Ly * a.b.Lz;
   spread_bin<CT, NZC, BS, ACC>(a, bin, cnt, call, eg, st.qmax_bits,
-                               sorted + (size_t)bin * a.cap_bin, grid64, spread_smem,
+                               sorted + (size_t)bin * a.cap_bin, loc + (size_t)bin * a.cap_bin,
+                               grid64, spread_smem,
                                bricks ? bricks + (size_t)bin * vol : nullptr, bshift);
 }
 
@@ -981,12 +991,14 @@ __global__ void __launch_bounds__(256)
       const int oy = wrapi((int)round(fy) - g.ny, g.Y);
       const int oz = wrapi((int)round(fz) - g.nz, g.Z);
       const BinShape& b = S.b;
-      const int bin = ox / b.Bx + b.nbx * (oy / b.By + b.nby * (oz / b.Bz));
+      const int qx = ox / b.Bx, qy = oy / b.By, qz = oz / b.Bz;
+      const int bin = qx + b.nbx * (qy + b.nby * qz);
       const int slot = atomicAdd(&d.counts[S.bofs + bin], 1);
       if (slot < S.cap_bin) {
         const long long pos = S.slot0 + (long long)bin * S.cap_bin + slot;
         ((V4<CT>*)d.sorted)[pos] = p;
         d.perm[pos] = i;
+        d.loc[pos] = (ox - qx * b.Bx) | ((oy - qy * b.By) << 8) | ((oz - qz * b.Bz) << 16);
       } else {
         d.ovf[atomicAdd(d.novf, 1)] = i;
       }
@@ -1023,9 +1035,10 @@ __global__ void __launch_bounds__(BS)
     const BinStats st = d.stats[item.x];
     const int eg = fixed_exponent(st.qmax_bits, S.n - st.invalid, S.images, S.pref_bound);
     const SpreadArgs a = batch_spread_args(S);
-    spread_bin<CT, NZC, BS, ACC>(a, bin, cnt, call, eg, st.qmax_bits,
-                                 (const V4<CT>*)d.sorted + S.slot0 + (long long)bin * S.cap_bin,
-                                 d.grid64 + S.gofs, spread_smem, (ACC*)nullptr, nullptr);
+    const long long s0 = S.slot0 + (long long)bin * S.cap_bin;
+    spread_bin<CT, NZC, BS, ACC>(a, bin, cnt, call, eg, st.qmax_bits, (const V4<CT>*)d.sorted + s0,
+                                 d.loc + s0, d.grid64 + S.gofs, spread_smem, (ACC*)nullptr,
+                                 nullptr);
   }
 }
 
@@ -1146,11 +1159,11 @@ hipError_t launch_bin(bool dbl, const Geom& g, const BinShape& b, Binning& w, hi
     const unsigned grid = (unsigned)((w.n + 255) / 256);
     if (dbl)
       k_bin<double><<<grid, 256, 0, s>>>(g, b, w.n, w.cap_bin, (const V4<double>*)w.xyzq,
-                                         w.counts, w.stats, w.ovf, w.perm,
+                                         w.counts, w.stats, w.ovf, w.perm, w.loc,
                                          (V4<double>*)w.sorted);
     else
       k_bin<float><<<grid, 256, 0, s>>>(g, b, w.n, w.cap_bin, (const V4<float>*)w.xyzq, w.counts,
-                                        w.stats, w.ovf, w.perm, (V4<float>*)w.sorted);
+                                        w.stats, w.ovf, w.perm, w.loc, (V4<float>*)w.sorted);
   }
   return hipGetLastError();
 }
@@ -1238,7 +1251,7 @@ static hipError_t spread_t(const Geom& g, const BinShape& b, double sigma, bool 
 #define GSE_SPREAD_LAUNCH(NZV, BSV, ACCT)                                                    \
   k_spread<CT, NZV, BSV, ACCT><<<nwg, BSV, lds, s>>>(a, xq, srt, w.counts, w.counts_other,  \
                                                      w.stats, w.stats_other, w.ovf, grid64, \
-                                                     (ACCT*)w.bricks, w.bshift)
+                                                     (ACCT*)w.bricks, w.bshift, w.loc)
 #define GSE_SPREAD_BS(NZV, ACCT)                      \
   if (bs == 512)                                      \
     GSE_SPREAD_LAUNCH(NZV, 512, ACCT);                \
