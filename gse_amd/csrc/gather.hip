@@ -190,13 +190,11 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
   constexpr int nwv = BS / 64;
   // the first chunk's particles (cap <= 256 <= BS: at most one per thread)
   // are loaded first, so their latency overlaps the brick staging
-  V4<CT> pre{};
-  int pre_p = 0, pre_l = 0;
-  if ((int)threadIdx.x < min(a.cap, cnt)) {
-    pre = sorted[threadIdx.x];
-    pre_p = perm[threadIdx.x];
-    pre_l = loc[threadIdx.x];
-  }
+  // (the index is clamped into the bin's slots, so the loads need no branch
+  // and the values stay in registers; slots past cnt are never used)
+  const int pslot = min((int)threadIdx.x, a.cap_bin - 1);
+  const V4<CT> pre = sorted[pslot];
+  const int pre_p = perm[pslot], pre_l = loc[pslot];
   {  // stage the potential brick: rows of Lx lanes, no per-element division.
      // Loads go out in batches of kStageBatch before their LDS stores, so a
      // thread has that many in flight instead of one.  Wrapped grid rows come
@@ -315,10 +313,17 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
     const int na = min(cap, s1 - c0);
     // phase 0: particles and original indices into LDS, one coalesced pass
     if ((int)threadIdx.x < na) {
-      const V4<CT> p = c0 == 0 ? pre : sorted[c0 + threadIdx.x];
-      const int l = c0 == 0 ? pre_l : loc[c0 + threadIdx.x];
+      CT px = pre.x, py = pre.y, pz = pre.z, pw = pre.w;
+      int l = pre_l, pp = pre_p;
+      if (c0 != 0) {
+        const V4<CT> q = sorted[c0 + threadIdx.x];
+        px = q.x, py = q.y, pz = q.z, pw = q.w;
+        l = loc[c0 + threadIdx.x];
+        pp = perm[c0 + threadIdx.x];
+      }
+      const V4<CT> p{px, py, pz, pw};
       sX[threadIdx.x] = p;
-      sP[threadIdx.x] = c0 == 0 ? pre_p : perm[c0 + threadIdx.x];
+      sP[threadIdx.x] = pp;
       // brick offset of the cube origin (stored by k_bin) and the charge
       sB[threadIdx.x] = (l & 255) + Px * (((l >> 8) & 255) + Ly * (l >> 16));
       sQ[threadIdx.x] = (double)p.w;

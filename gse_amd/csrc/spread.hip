@@ -282,12 +282,11 @@ __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt
   int* sB = (int*)(lds + lay.oB);        // [cap]
   // the first chunk's particle (cap <= 256 <= BS: at most one per thread) is
   // loaded before the brick is cleared, so its latency overlaps the clear
-  V4<CT> pre{};
-  int pre_l = 0;
-  if ((int)threadIdx.x < min(cap, cnt)) {
-    pre = sorted[threadIdx.x];
-    pre_l = loc[threadIdx.x];
-  }
+  // (the index is clamped into the bin's slots, so the load needs no branch
+  // and the values stay in registers; slots past cnt are never used)
+  const int pslot = min((int)threadIdx.x, a.cap_bin - 1);
+  const V4<CT> pre = sorted[pslot];
+  const int pre_l = loc[pslot];
   for (int j = threadIdx.x; j < vol; j += BS) brick[j] = (ACC)0;
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -304,15 +303,17 @@ __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt
   } else {
     role.axis = -1; role.j = 0;
   }
+  // column k of this lane: factor indices k_tx, NX + k_ty and brick offset
+  // k_off (-1: no column)
   int k_tx[kMaxK], k_ty[kMaxK], k_off[kMaxK];
-  bool k_ok[kMaxK];
 #pragma unroll
   for (int k = 0; k < kMaxK; k++) {
     const int idx = lane + 64 * k;
-    k_ok[k] = (k < KI) && idx < NXY;
-    k_ty[k] = k_ok[k] ? idx / NX : 0;
-    k_tx[k] = k_ok[k] ? idx - k_ty[k] * NX : 0;
-    k_off[k] = k_tx[k] + Lx * k_ty[k];
+    const bool ok = (k < KI) && idx < NXY;
+    const int ty = ok ? idx / NX : 0;
+    k_tx[k] = ok ? idx - ty * NX : 0;
+    k_ty[k] = NX + ty;
+    k_off[k] = ok ? k_tx[k] + Lx * ty : -1;
   }
 
   for (int c0 = 0; c0 < cnt; c0 += cap) {
@@ -320,8 +321,14 @@ __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt
     // phase 0: the chunk's particles into LDS in one coalesced pass, with
     // their brick offsets (from the cube origin k_bin stored) and scaled charges
     if ((int)threadIdx.x < na) {
-      const V4<CT> p = c0 == 0 ? pre : sorted[c0 + threadIdx.x];
-      const int l = c0 == 0 ? pre_l : loc[c0 + threadIdx.x];
+      CT px = pre.x, py = pre.y, pz = pre.z, pw = pre.w;
+      int l = pre_l;
+      if (c0 != 0) {
+        const V4<CT> q = sorted[c0 + threadIdx.x];
+        px = q.x, py = q.y, pz = q.z, pw = q.w;
+        l = loc[c0 + threadIdx.x];
+      }
+      const V4<CT> p{px, py, pz, pw};
       sX[threadIdx.x] = p;
       sB[threadIdx.x] = (l & 255) + Lx * (((l >> 8) & 255) + Ly * (l >> 16));
       sQ[threadIdx.x] = ((double)p.w * a.pref) * Sb;
@@ -366,9 +373,9 @@ __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt
 #pragma unroll
       for (int k = 0; k < kMaxK; k++) {
         if (k >= KI) break;
-        if (k_ok[k]) {
+        if (k_off[k] >= 0) {
           // ((q pref 2^e) wx) wy: the reference's product order, scaled exactly
-          const double qxy = (qps * w[k_tx[k]]) * w[NX + k_ty[k]];
+          const double qxy = (qps * w[k_tx[k]]) * w[k_ty[k]];
           const int a0 = base + k_off[k];
           if (NZC > 0) {
 #pragma unroll
@@ -430,8 +437,10 @@ __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt
   __syncthreads();
 }
 
+// 512-thread workgroups: three per CU (6 waves per SIMD, at most 80 VGPRs),
+// so one workgroup's staging overlaps two others' work
 template <typename CT, int NZC, int BS, typename ACC>
-__global__ void __launch_bounds__(BS)
+__global__ void __launch_bounds__(BS, BS == 512 ? 6 : 1)
     k_spread(SpreadArgs a, const V4<CT>* __restrict__ xyzq, const V4<CT>* __restrict__ sorted,
              const int* __restrict__ counts, int* __restrict__ counts_other,
              const BinStats* __restrict__ stats, BinStats* __restrict__ stats_other,
