@@ -540,8 +540,8 @@ __device__ __forceinline__ GatherArgs batch_gather_args(const BatchSys& S) {
 
 template <typename CT, typename AT, int NZC, int FL, int BS>
 __global__ void __launch_bounds__(BS)
-    k_batch_gather(BatchDev d, const int2* __restrict__ items, int nitems,
-                   const CT* __restrict__ phi, GatherOut o) {
+    k_batch_gather(BatchDev d, const BatchSys* __restrict__ sysp, const int2* __restrict__ items,
+                   int nitems, const CT* __restrict__ phi, GatherOut o) {
   extern __shared__ __align__(32) double smem_raw[];
   const int nx = (int)(gridDim.x >> 3);
   const int x = (int)(blockIdx.x & 7), j = (int)(blockIdx.x >> 3);
@@ -549,7 +549,7 @@ __global__ void __launch_bounds__(BS)
   const int end = min(nitems, x * chunk + chunk);
   for (int it = x * chunk + j; it < end; it += nx) {
     const int2 item = items[it];
-    const BatchSys& S = d.sys[item.x];
+    const BatchSys& S = sysp[item.x];
     const int bin = item.y;
     const int cnt = min(d.counts[S.bofs + bin], S.cap_bin);
     if (cnt == 0) continue;
@@ -582,7 +582,7 @@ static hipError_t batch_gather_t(int nz, int bs, size_t lds, int grid, const Bat
   const unsigned g = (unsigned)grid;
   const CT* ph = (const CT*)phi;
 #define GSE_BG_LAUNCH(NZV, BSV) \
-  k_batch_gather<CT, AT, NZV, FL, BSV><<<g, BSV, lds, s>>>(d, items, nitems, ph, o)
+  k_batch_gather<CT, AT, NZV, FL, BSV><<<g, BSV, lds, s>>>(d, d.sys, items, nitems, ph, o)
 #define GSE_BG_BS(NZV)       \
   if (bs == 512)             \
     GSE_BG_LAUNCH(NZV, 512); \

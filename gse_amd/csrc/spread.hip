@@ -303,6 +303,23 @@ __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt
   } else {
     role.axis = -1; role.j = 0;
   }
+  // this lane's factor axis: spacing, inverse spacing and support half-width
+  // (selected once, as values, so no field of `a` is indexed at run time)
+  double lh = 0.0, lih = 0.0;
+  int ln = 0;
+  {
+    const double hx = g.hx, hy = g.hy, hz = g.hz, ihx = g.ihx, ihy = g.ihy, ihz = g.ihz;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    if (role.axis == 0) {
+      lh = hx; lih = ihx; ln = nx;
+    } else if (role.axis == 1) {
+      lh = hy; lih = ihy; ln = ny;
+    } else {
+      lh = hz; lih = ihz; ln = nz;
+    }
+  }
+  const double lane_j = (double)role.j;
+  const CT inv2s2_c = (CT)a.inv2s2;
   // column k of this lane: factor indices k_tx, NX + k_ty and brick offset
   // k_off (-1: no column)
   int k_tx[kMaxK], k_ty[kMaxK], k_off[kMaxK];
@@ -339,11 +356,11 @@ __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt
       if (role.axis >= 0) {
         const V4<CT> p = sX[ai];
         const double xv = role.axis == 0 ? (double)p.x : role.axis == 1 ? (double)p.y : (double)p.z;
-        const double ih = role.axis == 0 ? g.ihx : role.axis == 1 ? g.ihy : g.ihz;
-        const int n = role.axis == 0 ? g.nx : role.axis == 1 ? g.ny : g.nz;
-        const int c = (int)round(xv * ih) - n;
-        sW[(size_t)ai * NW + lane] =
-            (double)lane_weight<CT>(role, c, c, c, xv, xv, xv, g, a.inv2s2);
+        const int c = (int)round(xv * lih) - ln;
+        // t = (c*h - x) + j*h, rounded as lane_weight does (CpuGaussCharge.cpp:314-329)
+        const double t = add_rn(sub_rn(mul_rn((double)c, lh), xv), mul_rn(lane_j, lh));
+        const CT tc = (CT)t;
+        sW[(size_t)ai * NW + lane] = (double)gexp(mul_rn(mul_rn(-tc, tc), inv2s2_c));
       }
     }
     __syncthreads();
@@ -1029,14 +1046,17 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// sysp and items are read-only for the whole launch (separate restrict
+// arguments, so their uniform loads go to scalar registers)
 template <typename CT, int NZC, int BS, typename ACC>
 __global__ void __launch_bounds__(BS)
-    k_batch_spread(BatchDev d, const int2* __restrict__ items, int nitems) {
+    k_batch_spread(BatchDev d, const BatchSys* __restrict__ sysp, const int2* __restrict__ items,
+                   int nitems) {
   extern __shared__ __align__(32) unsigned char spread_smem[];
   const ItemWalk w = item_walk(nitems);
   for (int it = w.it; it < w.end; it += w.step) {
     const int2 item = items[it];
-    const BatchSys& S = d.sys[item.x];
+    const BatchSys& S = sysp[item.x];
     const int bin = item.y;
     const int call = d.counts[S.bofs + bin];
     const int cnt = min(call, S.cap_bin);
@@ -1438,7 +1458,7 @@ static hipError_t batch_spread_t(int nz, int bs, size_t lds, int grid, const Bat
   if (nitems > 0) {
     const unsigned g = (unsigned)grid;
 #define GSE_BS_LAUNCH(NZV, BSV) \
-  k_batch_spread<CT, NZV, BSV, ACC><<<g, BSV, lds, s>>>(d, items, nitems)
+  k_batch_spread<CT, NZV, BSV, ACC><<<g, BSV, lds, s>>>(d, d.sys, items, nitems)
 #define GSE_BS_BS(NZV)          \
   if (bs == 512)                \
     GSE_BS_LAUNCH(NZV, 512);    \
