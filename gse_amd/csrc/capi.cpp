@@ -480,6 +480,7 @@ struct BatchGroup {  // systems sharing 2nz+1: one spread and one gather launch
   int item0 = 0, nitems = 0;
   size_t lds_spread = 0, lds_gather = 0;
   int grid = 0;
+  bool tile4 = true;  // every brick row is whole 16-byte pieces (32-bit bricks)
 };
 
 struct BatchFFT {  // systems with equal grid dimensions, adjacent in the layout
@@ -507,7 +508,8 @@ struct gse_batch_s {
   long long total_points = 0, total_k = 0, total_slots = 0;
   int total_bins = 0;
   DevBuf d_sys, d_pofs, d_items, d_cchunks, d_kchunks, counts_b, stats_b, ovf_b, novf_b, perm_b,
-      loc_b, sorted_b, grid64_b, rho_b, phi_b, spec_b, table_b, part_b;
+      loc_b, sorted_b, grid64_b, rho_b, phi_b, spec_b, table_b, part_b, bricks_b, bshift_b;
+  bool brick_flush = false;  // bricks stored and summed by the tiled reduce
   BatchDev dev;
   hipStream_t stream = nullptr;
   // graph of one step
@@ -531,7 +533,7 @@ struct gse_batch_s {
     release_ffts();
     for (DevBuf* b : {&d_sys, &d_pofs, &d_items, &d_cchunks, &d_kchunks, &counts_b, &stats_b,
                       &ovf_b, &novf_b, &perm_b, &loc_b, &sorted_b, &grid64_b, &rho_b, &phi_b,
-                      &spec_b, &table_b, &part_b})
+                      &spec_b, &table_b, &part_b, &bricks_b, &bshift_b})
       b->release();
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
     cap_stream = nullptr;
@@ -1266,7 +1268,8 @@ gse_status_t batch_layout(gse_batch_s* h, hipStream_t st) {
   h->cchunks.clear();
   h->kchunks.clear();
   h->groups.clear();
-  long long bofs = 0, slot0 = 0, gofs = 0, kofs = 0;
+  long long bofs = 0, slot0 = 0, gofs = 0, kofs = 0, brick0 = 0;
+  bool tiles = true;
   std::vector<int> nzs;
   for (int k = 0; k < ns; k++) {
     const int si = h->order[k];
@@ -1286,6 +1289,9 @@ gse_status_t batch_layout(gse_batch_s* h, hipStream_t st) {
     S.slot0 = slot0;
     S.gofs = gofs;
     S.kofs = kofs;
+    S.brick0 = brick0;
+    brick0 += (long long)S.b.nbins * S.b.Lx * S.b.Ly * S.b.Lz;
+    tiles = tiles && brick_tile_ok(S.g, S.b);
     bofs += S.b.nbins;
     slot0 += (long long)S.b.nbins * S.cap_bin;
     gofs += (long long)X * Y * Z;
@@ -1329,6 +1335,7 @@ gse_status_t batch_layout(gse_batch_s* h, hipStream_t st) {
       if (2 * S.g.nz + 1 != nz) continue;
       for (int b = 0; b < S.b.nbins; b++) h->items.push_back(make_int2(si, b));
       gr.lds_spread = std::max(gr.lds_spread, spread_lds_bytes(dbl, S.g, S.b, nullptr));
+      gr.tile4 = gr.tile4 && acc32 && S.b.Lx % 4 == 0;
       gr.lds_gather = std::max(gr.lds_gather, gather_lds_bytes(dbl, S.g, S.b, nullptr));
     }
     gr.nitems = (int)h->items.size() - gr.item0;
@@ -1355,6 +1362,11 @@ gse_status_t batch_layout(gse_batch_s* h, hipStream_t st) {
   GSECHK(h->spec_b.ensure(2 * ct * (size_t)h->total_k));
   GSECHK(h->table_b.ensure(ct * (size_t)h->total_k));
   GSECHK(h->part_b.ensure(sizeof(double) * std::max<size_t>(1, h->kchunks.size())));
+  h->brick_flush = g_tuning.spread_flush && tiles;
+  if (h->brick_flush) {
+    GSECHK(h->bricks_b.ensure((acc32 ? 4 : 8) * (size_t)std::max<long long>(1, brick0)));
+    GSECHK(h->bshift_b.ensure(sizeof(int) * std::max(1, h->total_bins)));
+  }
   HIPCHK(hipMemcpy(h->d_sys.p, h->sys.data(), sizeof(BatchSys) * ns, hipMemcpyHostToDevice));
   if (!h->items.empty())
     HIPCHK(hipMemcpy(h->d_items.p, h->items.data(), sizeof(int2) * h->items.size(),
@@ -1381,6 +1393,8 @@ gse_status_t batch_layout(gse_batch_s* h, hipStream_t st) {
   d.loc = h->loc_b.as<int>();
   d.sorted = h->sorted_b.p;
   d.grid64 = h->grid64_b.as<unsigned long long>();
+  d.bricks = h->brick_flush ? h->bricks_b.p : nullptr;
+  d.bshift = h->brick_flush ? h->bshift_b.as<int>() : nullptr;
   HIPCHK(launch_fill_u64(d.grid64, (size_t)h->total_points, st));
   // Green's tables (G_GSE with sigma2) per system, batched FFT plans per grid size
   for (int k = 0; k < ns; k++) {
@@ -1426,8 +1440,15 @@ gse_status_t batch_run(gse_batch_s* h, hipStream_t st, const void* xyzq, void* f
     HIPCHK(launch_batch_spread(dbl, g.nz, batch_block(h->spread_block, dbl), g.lds_spread, g.grid,
                                d, items + g.item0, g.nitems, st));
   HIPCHK(launch_batch_spread_overflow(dbl, d, xyzq, st));
-  HIPCHK(launch_batch_convert(dbl, d, h->d_cchunks.as<BatchChunk>(), (int)h->cchunks.size(),
-                              h->rho_b.p, st));
+  if (h->brick_flush) {
+    const bool acc32 = !dbl && g_tuning.spread_lds32;
+    for (const BatchGroup& g : h->groups)
+      HIPCHK(launch_batch_reduce(dbl, acc32, g.tile4, d, items + g.item0, g.nitems, h->rho_b.p,
+                                 st));
+  } else {
+    HIPCHK(launch_batch_convert(dbl, d, h->d_cchunks.as<BatchChunk>(), (int)h->cchunks.size(),
+                                h->rho_b.p, st));
+  }
   for (BatchFFT* f : h->ffts)
     GSECHK(f->fft.exec(true, (char*)h->rho_b.p + ct * f->gofs,
                        (char*)h->spec_b.p + 2 * ct * f->kofs, st));

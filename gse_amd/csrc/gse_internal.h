@@ -205,6 +205,7 @@ struct BatchSys {
   int spread_cap;   // particles staged per chunk in the spread
   int gather_cap;   // particles staged per chunk in the gather
   int kblk0, nkblk; // k-pass chunks of this system
+  long long brick0; // first brick element (brick-store flush)
   double inv2s2, pref, pref_bound, rc2, prefF, prefP, vcell;
 };
 
@@ -229,6 +230,8 @@ struct BatchDev {
   int* loc = nullptr;         // [slots] cube origin in the bin (ox | oy << 8 | oz << 16)
   void* sorted = nullptr;     // [slots] 4-vectors
   unsigned long long* grid64 = nullptr;
+  void* bricks = nullptr;     // brick-store flush: [sum of nbins*vol] (NULL: atomic flush)
+  int* bshift = nullptr;      // [total bins] brick shift, -1 for an empty bin
 };
 
 // clears counts, statistics and the overflow counter
@@ -242,6 +245,13 @@ hipError_t launch_batch_spread_overflow(bool dbl, const BatchDev& d, const void*
                                         hipStream_t s);
 hipError_t launch_batch_convert(bool dbl, const BatchDev& d, const BatchChunk* chunks,
                                 int nchunks, void* rho, hipStream_t s);
+// brick-store flush: sums the bricks of the items' systems into rho (one
+// workgroup per item; tile4: 32-bit bricks with rows of whole 16-byte pieces)
+hipError_t launch_batch_reduce(bool dbl, bool acc32, bool tile4, const BatchDev& d,
+                               const int2* items, int nitems, void* rho, hipStream_t s);
+// the tiled reduce applies: bins of 8^3 on a grid of >= 3 whole bins per axis,
+// bricks at most 3 bins wide
+bool brick_tile_ok(const Geom& g, const BinShape& b);
 hipError_t launch_batch_kpass(bool dbl, const BatchDev& d, const BatchChunk* chunks,
                               int nchunks, void* spec, const void* table, double* partials,
                               hipStream_t s);

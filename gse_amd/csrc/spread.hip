@@ -813,17 +813,14 @@ static unsigned grid_for(size_t count, unsigned threads, unsigned cap = 4096) {
 // bricks of the bin and of its neighbours at -1, -2 bins that cover it are
 // walked as a uniform loop, so brick shifts are scalar loads and each brick
 // row is read as 8 consecutive values.
+// One bin interior of one system (bricks, bshift, grid64 and out at the
+// system's base); inv = 2^-e, ovf: the overflow int64 grid holds data.
 template <typename T, typename ACC>
-__global__ void __launch_bounds__(512)
-    k_brick_reduce_tile(Geom g, BinShape b, int nwg, const ACC* __restrict__ bricks,
-                        const int* __restrict__ bshift, const BinStats* __restrict__ stats, int n,
-                        int images, double pref_bound, unsigned long long* __restrict__ grid64,
-                        T* __restrict__ out) {
-  const int bin = xcd_remap(blockIdx.x, nwg);
-  if (bin >= b.nbins) return;
-  const BinStats st = *stats;
-  const double inv =
-      ldexp(1.0, -fixed_exponent(st.qmax_bits, n - st.invalid, images, pref_bound));
+__device__ __forceinline__ void reduce_tile_bin(const Geom& g, const BinShape& b, int bin,
+                                                const ACC* __restrict__ bricks,
+                                                const int* __restrict__ bshift, double inv,
+                                                bool ovf, unsigned long long* __restrict__ grid64,
+                                                T* __restrict__ out) {
   const int cx = bin % b.nbx, cy = (bin / b.nbx) % b.nby, cz = bin / (b.nbx * b.nby);
   const int tx = threadIdx.x & 7, ty = (threadIdx.x >> 3) & 7, tz = threadIdx.x >> 6;
   const int Lx = b.Lx, Ly = b.Ly, Lz = b.Lz;
@@ -860,11 +857,25 @@ __global__ void __launch_bounds__(512)
     if (sh[k] >= 0)
       sum += sizeof(ACC) == 4 ? ((long long)(int)raw[k] << sh[k]) : (long long)raw[k];
   const size_t p = (size_t)(8 * cx + tx) + (size_t)g.X * ((8 * cy + ty) + (size_t)g.Y * (8 * cz + tz));
-  if (st.overflow > 0) {
+  if (ovf) {
     sum += (long long)grid64[p];
     grid64[p] = 0ull;
   }
   out[p] = (T)((double)sum * inv);
+}
+
+template <typename T, typename ACC>
+__global__ void __launch_bounds__(512)
+    k_brick_reduce_tile(Geom g, BinShape b, int nwg, const ACC* __restrict__ bricks,
+                        const int* __restrict__ bshift, const BinStats* __restrict__ stats, int n,
+                        int images, double pref_bound, unsigned long long* __restrict__ grid64,
+                        T* __restrict__ out) {
+  const int bin = xcd_remap(blockIdx.x, nwg);
+  if (bin >= b.nbins) return;
+  const BinStats st = *stats;
+  const double inv =
+      ldexp(1.0, -fixed_exponent(st.qmax_bits, n - st.invalid, images, pref_bound));
+  reduce_tile_bin<T, ACC>(g, b, bin, bricks, bshift, inv, st.overflow > 0, grid64, out);
 }
 
 // k_brick_reduce_tile with 32-bit bricks whose rows are whole 16-byte
@@ -872,16 +883,12 @@ __global__ void __launch_bounds__(512)
 // consecutive x points read and written as 16-byte pieces, the 27 candidate
 // bricks taken one z layer (9 bricks) at a time.
 template <typename T>
-__global__ void __launch_bounds__(128)
-    k_brick_reduce_tile4(Geom g, BinShape b, int nwg, const unsigned* __restrict__ bricks,
-                         const int* __restrict__ bshift, const BinStats* __restrict__ stats,
-                         int n, int images, double pref_bound,
-                         unsigned long long* __restrict__ grid64, T* __restrict__ out) {
-  const int bin = xcd_remap(blockIdx.x, nwg);
-  if (bin >= b.nbins) return;
-  const BinStats st = *stats;
-  const double inv =
-      ldexp(1.0, -fixed_exponent(st.qmax_bits, n - st.invalid, images, pref_bound));
+__device__ __forceinline__ void reduce_tile4_bin(const Geom& g, const BinShape& b, int bin,
+                                                 const unsigned* __restrict__ bricks,
+                                                 const int* __restrict__ bshift, double inv,
+                                                 bool ovf,
+                                                 unsigned long long* __restrict__ grid64,
+                                                 T* __restrict__ out) {
   const int cx = bin % b.nbx, cy = (bin / b.nbx) % b.nby, cz = bin / (b.nbx * b.nby);
   const int tq = threadIdx.x & 1, ty = (threadIdx.x >> 1) & 7, tz = threadIdx.x >> 4;
   const int Lx = b.Lx, Ly = b.Ly, Lz = b.Lz;
@@ -915,7 +922,7 @@ __global__ void __launch_bounds__(128)
     }
   }
   const size_t p = (size_t)(8 * cx + 4 * tq) + (size_t)g.X * ((8 * cy + ty) + (size_t)g.Y * (8 * cz + tz));
-  if (st.overflow > 0) {
+  if (ovf) {
 #pragma unroll
     for (int c = 0; c < 4; c++) {
       sum[c] += (long long)grid64[p + c];
@@ -931,6 +938,20 @@ __global__ void __launch_bounds__(128)
 #pragma unroll
     for (int c = 0; c < 4; c++) out[p + c] = o4[c];
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(128)
+    k_brick_reduce_tile4(Geom g, BinShape b, int nwg, const unsigned* __restrict__ bricks,
+                         const int* __restrict__ bshift, const BinStats* __restrict__ stats,
+                         int n, int images, double pref_bound,
+                         unsigned long long* __restrict__ grid64, T* __restrict__ out) {
+  const int bin = xcd_remap(blockIdx.x, nwg);
+  if (bin >= b.nbins) return;
+  const BinStats st = *stats;
+  const double inv =
+      ldexp(1.0, -fixed_exponent(st.qmax_bits, n - st.invalid, images, pref_bound));
+  reduce_tile4_bin<T>(g, b, bin, bricks, bshift, inv, st.overflow > 0, grid64, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -1060,14 +1081,19 @@ __global__ void __launch_bounds__(BS)
     const int bin = item.y;
     const int call = d.counts[S.bofs + bin];
     const int cnt = min(call, S.cap_bin);
-    if (cnt == 0) continue;
+    if (cnt == 0) {
+      if (d.bricks && threadIdx.x == 0) d.bshift[S.bofs + bin] = -1;  // no brick this step
+      continue;
+    }
     const BinStats st = d.stats[item.x];
     const int eg = fixed_exponent(st.qmax_bits, S.n - st.invalid, S.images, S.pref_bound);
     const SpreadArgs a = batch_spread_args(S);
     const long long s0 = S.slot0 + (long long)bin * S.cap_bin;
+    const long long vol = (long long)S.b.Lx * S.b.Ly * S.b.Lz;
     spread_bin<CT, NZC, BS, ACC>(a, bin, cnt, call, eg, st.qmax_bits, (const V4<CT>*)d.sorted + s0,
-                                 d.loc + s0, d.grid64 + S.gofs, spread_smem, (ACC*)nullptr,
-                                 nullptr);
+                                 d.loc + s0, d.grid64 + S.gofs, spread_smem,
+                                 d.bricks ? (ACC*)d.bricks + S.brick0 + bin * vol : nullptr,
+                                 d.bshift + S.bofs);
   }
 }
 
@@ -1087,6 +1113,26 @@ __global__ void __launch_bounds__(256)
     spread_overflow<CT, ACC>(batch_spread_args(S), eg, st.qmax_bits, d.counts + S.bofs, xyzq[i],
                              d.grid64 + S.gofs);
   }
+}
+
+// brick-store flush of the batch: one workgroup per (system, bin) item
+template <typename T, typename ACC, bool TILE4>
+__global__ void __launch_bounds__(TILE4 ? 128 : 512)
+    k_batch_reduce(BatchDev d, const BatchSys* __restrict__ sysp, const int2* __restrict__ items,
+                   T* __restrict__ rho) {
+  const int2 item = items[blockIdx.x];
+  const BatchSys& S = sysp[item.x];
+  const BinStats st = d.stats[item.x];
+  const double inv =
+      ldexp(1.0, -fixed_exponent(st.qmax_bits, S.n - st.invalid, S.images, S.pref_bound));
+  const bool ovf = *d.novf > 0;
+  const ACC* bricks = (const ACC*)d.bricks + S.brick0;
+  if constexpr (TILE4)
+    reduce_tile4_bin<T>(S.g, S.b, item.y, (const unsigned*)bricks, d.bshift + S.bofs, inv, ovf,
+                        d.grid64 + S.gofs, rho + S.gofs);
+  else
+    reduce_tile_bin<T, ACC>(S.g, S.b, item.y, bricks, d.bshift + S.bofs, inv, ovf,
+                            d.grid64 + S.gofs, rho + S.gofs);
 }
 
 // int64 -> T for every system's grid (one chunk per workgroup), int64 cleared
@@ -1361,7 +1407,7 @@ bool brick_reduce_ok(const Geom& g, const BinShape& b) {
          brick_axis_max(g.Z, b.Bz, b.nbz, b.Lz) <= KR;
 }
 
-static bool brick_tile_ok(const Geom& g, const BinShape& b) {
+bool brick_tile_ok(const Geom& g, const BinShape& b) {
   return b.Bx == 8 && b.By == 8 && b.Bz == 8 && g.X % 8 == 0 && g.Y % 8 == 0 && g.Z % 8 == 0 &&
          b.nbx >= 3 && b.nby >= 3 && b.nbz >= 3 && b.Lx <= 24 && b.Ly <= 24 && b.Lz <= 24;
 }
@@ -1494,6 +1540,22 @@ hipError_t launch_batch_spread_overflow(bool dbl, const BatchDev& d, const void*
     k_batch_spread_ovf<float, unsigned><<<64, 256, 0, s>>>(d, (const V4<float>*)xyzq);
   else
     k_batch_spread_ovf<float, unsigned long long><<<64, 256, 0, s>>>(d, (const V4<float>*)xyzq);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_reduce(bool dbl, bool acc32, bool tile4, const BatchDev& d,
+                               const int2* items, int nitems, void* rho, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  if (dbl)
+    k_batch_reduce<double, unsigned long long, false><<<nitems, 512, 0, s>>>(d, d.sys, items,
+                                                                             (double*)rho);
+  else if (acc32 && tile4)
+    k_batch_reduce<float, unsigned, true><<<nitems, 128, 0, s>>>(d, d.sys, items, (float*)rho);
+  else if (acc32)
+    k_batch_reduce<float, unsigned, false><<<nitems, 512, 0, s>>>(d, d.sys, items, (float*)rho);
+  else
+    k_batch_reduce<float, unsigned long long, false><<<nitems, 512, 0, s>>>(d, d.sys, items,
+                                                                            (float*)rho);
   return hipGetLastError();
 }
 

@@ -179,3 +179,23 @@ def test_batch_bad_arguments():
         b.step(torch.zeros((11, 4), device="cuda"))
     with pytest.raises(api._lib.GSEError):
         b.set_config(bin_edge=0)
+
+
+def test_batch_fp64_brick_path_with_overflow():
+    """Grids of whole 8-cell bins (>= 3 per axis) take the brick-store flush
+    and tiled reduce; one system also overfills a bin, so the reduce adds the
+    overflow particles' int64 grid.  FP64 within 1e-10 of the oracle."""
+    rng = np.random.default_rng(43)
+    cl = np.empty((900, 4))
+    cl[:, :3] = 7.0 + rng.random((900, 3)) * 0.8
+    cl[:, 3] = 0.25 + 0.5 * rng.random(900)
+    cl = np.concatenate([cl, systems.water_box(100, 24.0, seed=44)])
+    sysl = [(systems.water_box(200, 24.0, seed=45), (24, 24, 24), (24.0,) * 3),
+            (cl, (24, 24, 24), (24.0,) * 3),
+            (systems.water_box(300, 30.0, seed=46), (32, 32, 32), (30.0,) * 3),
+            (systems.water_box(250, 26.0, seed=47), (32, 32, 32), (26.0,) * 3)]
+    refs = run_oracle(sysl)
+    b = make_batch(F64, sysl)
+    o = b.step(torch.from_numpy(concat(sysl)).cuda(), force=True, potential=True)
+    torch.cuda.synchronize()
+    check_against(sysl, refs, o, 1e-10, 1e-10)
