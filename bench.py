@@ -71,6 +71,38 @@ def time_steps(fn, steps, warmup, torch):
     return (time.perf_counter() - t0) / steps
 
 
+def bench_conv(api, torch, dev):
+    """The separable three-pass sigma2 convolution of mode R (spreadChargeOnGrid,
+    11 taps per axis at h ~ 1 A) on its own, FP32 with compensated sums, at
+    M = 80 and M = 216: HBM roofline with 3 passes x (4 B read + 4 B write)
+    per grid point as the algorithmic bytes."""
+    out = {}
+    for M, L in ((80, 78.3), (216, 215.3)):
+        h = L / M
+        s2 = 2.12 * math.sqrt(0.2)
+        R = int(math.ceil(3.0 * math.sqrt(2.0) * s2 / h))
+        k = np.exp(-(h * np.arange(-R, R + 1)) ** 2 / (2 * s2 * s2))
+        conv = api.CpuConvolution(torch.float32, M, M, M)
+        conv.setupKernel(R, R, R, k, k, k)
+        a = torch.rand((M, M, M), device=dev, dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            conv.runXYZ(a, b)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 50
+        e0.record()
+        for _ in range(reps):
+            conv.runXYZ(a, b)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        nbytes = 3 * 8 * M ** 3
+        out[f"M{M}"] = {"taps": 2 * R + 1, "ms": round(ms, 5),
+                        "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                        "frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def bench_batched(api, systems, torch, dev, nsys=256):
     """256 independent water boxes (600..8000 charges, M in {24, 32, 40}),
     FP32: the batched step (autotuned, replayed from a HIP graph) against
@@ -199,6 +231,11 @@ def main():
         # conv + gather roofline as the survey defines it (mode R includes the conv)
         extras["stage_roofline_frac"] = {
             k: round(sb[k] / (ours[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 5) for k in sb}
+        extras["conv_roofline"] = bench_conv(api, torch, dev)
+        om = {}
+        sm = time_steps(lambda: gs.step(x, mode=1, force=True, potential=True, out=om),
+                        max(20, args.steps // 5), 5, torch)
+        extras["mode_R_steps_per_s"] = round(1.0 / sm, 3)
         extras["batched"] = bench_batched(api, systems, torch, dev)
         line["extras"] = extras
 

@@ -445,6 +445,7 @@ struct gse_convolution_s {
   bool dbl = true;
   int X = 0, Y = 0, Z = 0;
   int R[3] = {-1, -1, -1};
+  bool sym[3] = {false, false, false};  // taps symmetric about the centre
   DevBuf taps[3], tmpA, tmpB;
   hipStream_t stream = nullptr;
 };
@@ -464,6 +465,8 @@ struct gse_gsek_s {
   Engine eng;
   KSolver ks;
   DevBuf rhoS, phiM, rhoR, taps[3], tmp, tmp2, dout, part;
+  int tapsR[3] = {0, 0, 0};
+  double taps_key[5] = {NAN, NAN, NAN, NAN, NAN};  // sigma2, rcut2, box: taps are current
   hipStream_t stream = nullptr;
   bool profiling = false;
   std::vector<hipEvent_t> events;  // (GSE_NUM_STAGES + 1) per recorded step
@@ -567,14 +570,28 @@ gse_status_t make_ongrid_taps(bool dbl, int X, int Y, int Z, double sigma, doubl
   return GSE_SUCCESS;
 }
 
+// on-grid sigma2 taps of a GSE-k handle, rebuilt only when sigma2, rcut2 or
+// the box change (the upload is synchronous)
+gse_status_t gsek_taps(gse_gsek_s* h) {
+  const double key[5] = {h->sigma2, h->rcut2(), h->bx, h->by, h->bz};
+  if (std::memcmp(key, h->taps_key, sizeof(key)) == 0) return GSE_SUCCESS;
+  GSECHK(make_ongrid_taps(h->ct_dbl, h->X, h->Y, h->Z, h->sigma2, h->rcut2(), h->bx, h->by, h->bz,
+                          h->taps, h->tapsR));
+  std::memcpy(h->taps_key, key, sizeof(key));
+  return GSE_SUCCESS;
+}
+
+// sym[a]: the taps of axis a are symmetric (the on-grid Gaussians always are)
+const bool kSymTaps[3] = {true, true, true};
 gse_status_t conv3(bool dbl, int X, int Y, int Z, const int (&R)[3], DevBuf (&taps)[3],
-                   const void* in, void* out, DevBuf& tA, DevBuf& tB, hipStream_t s) {
+                   const void* in, void* out, DevBuf& tA, DevBuf& tB, hipStream_t s,
+                   const bool (&sym)[3] = kSymTaps) {
   const size_t bytes = tsize(dbl) * (size_t)X * Y * Z;
   GSECHK(tA.ensure(bytes));
   GSECHK(tB.ensure(bytes));
-  HIPCHK(launch_conv_pass(dbl, 0, X, Y, Z, R[0], taps[0].p, in, tA.p, s));
-  HIPCHK(launch_conv_pass(dbl, 1, X, Y, Z, R[1], taps[1].p, tA.p, tB.p, s));
-  HIPCHK(launch_conv_pass(dbl, 2, X, Y, Z, R[2], taps[2].p, tB.p, out, s));
+  HIPCHK(launch_conv_pass(dbl, 0, X, Y, Z, R[0], taps[0].p, in, tA.p, s, sym[0]));
+  HIPCHK(launch_conv_pass(dbl, 1, X, Y, Z, R[1], taps[1].p, tA.p, tB.p, s, sym[1]));
+  HIPCHK(launch_conv_pass(dbl, 2, X, Y, Z, R[2], taps[2].p, tB.p, out, s, sym[2]));
   return GSE_SUCCESS;
 }
 
@@ -868,6 +885,11 @@ gse_status_t gse_convolution_setup_kernel(gse_convolution_t h, int Rx, int Ry, i
     GSECHK(h->taps[a].ensure(bytes));
     HIPCHK(hipMemcpy(h->taps[a].p, k[a], bytes, hipMemcpyHostToDevice));
     h->R[a] = R[a];
+    bool sym = true;
+    for (int i = 1; i <= R[a]; i++)
+      sym = sym && (h->dbl ? ((const double*)k[a])[R[a] - i] == ((const double*)k[a])[R[a] + i]
+                           : ((const float*)k[a])[R[a] - i] == ((const float*)k[a])[R[a] + i]);
+    h->sym[a] = sym;
   }
   return GSE_SUCCESS;
 }
@@ -875,14 +897,16 @@ gse_status_t gse_convolution_setup_kernel(gse_convolution_t h, int Rx, int Ry, i
 gse_status_t gse_convolution_run(gse_convolution_t h, const void* in, void* out) {
   if (!h || !in || !out) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
   if (h->R[0] < 0) return fail(GSE_ERROR_INVALID_VALUE, "setupKernel has not been called");
-  HIPCHK(launch_conv_pass(h->dbl, 0, h->X, h->Y, h->Z, h->R[0], h->taps[0].p, in, out, h->stream));
+  HIPCHK(launch_conv_pass(h->dbl, 0, h->X, h->Y, h->Z, h->R[0], h->taps[0].p, in, out, h->stream,
+                          h->sym[0]));
   return GSE_SUCCESS;
 }
 
 gse_status_t gse_convolution_run_xyz(gse_convolution_t h, const void* in, void* out) {
   if (!h || !in || !out) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
   if (h->R[0] < 0) return fail(GSE_ERROR_INVALID_VALUE, "setupKernel has not been called");
-  return conv3(h->dbl, h->X, h->Y, h->Z, h->R, h->taps, in, out, h->tmpA, h->tmpB, h->stream);
+  return conv3(h->dbl, h->X, h->Y, h->Z, h->R, h->taps, in, out, h->tmpA, h->tmpB, h->stream,
+               h->sym);
 }
 
 // ------------------------------- CpuFFTSolver ------------------------------
@@ -1057,10 +1081,8 @@ gse_status_t gse_gsek_spread_charge1(gse_gsek_t h, int n, const void* xyzq) {
 
 gse_status_t gse_gsek_spread_charge2(gse_gsek_t h, void* rhoOut) {
   if (!h || !rhoOut) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
-  int R[3];
-  GSECHK(make_ongrid_taps(h->ct_dbl, h->X, h->Y, h->Z, h->sigma2, h->rcut2(), h->bx, h->by, h->bz,
-                          h->taps, R));
-  return conv3(h->ct_dbl, h->X, h->Y, h->Z, R, h->taps, h->rhoS.p, rhoOut, h->tmp, h->tmp2,
+  GSECHK(gsek_taps(h));
+  return conv3(h->ct_dbl, h->X, h->Y, h->Z, h->tapsR, h->taps, h->rhoS.p, rhoOut, h->tmp, h->tmp2,
                h->stream);
 }
 
@@ -1146,11 +1168,9 @@ static gse_status_t gsek_run(gse_gsek_t h, int n, const void* xyzq, int mode, in
     GSECHK(h->ks.solve(GSE_GREENS_GSE, h->sigma2, h->bx, h->by, h->bz, h->rhoS.p, h->phiM.p, eout,
                        virial, h->ccelec, s, h->profiling ? h : nullptr, flags ? &o : nullptr));
   } else {
-    int R[3];
-    GSECHK(make_ongrid_taps(h->ct_dbl, h->X, h->Y, h->Z, h->sigma2, h->rcut2(), h->bx, h->by,
-                            h->bz, h->taps, R));
+    GSECHK(gsek_taps(h));
     GSECHK(h->rhoR.ensure(tsize(h->ct_dbl) * (size_t)h->X * h->Y * h->Z));
-    GSECHK(conv3(h->ct_dbl, h->X, h->Y, h->Z, R, h->taps, h->rhoS.p, h->rhoR.p, h->tmp,
+    GSECHK(conv3(h->ct_dbl, h->X, h->Y, h->Z, h->tapsR, h->taps, h->rhoS.p, h->rhoR.p, h->tmp,
                  h->tmp2, s));
     GSECHK(h->ks.solve(GSE_GREENS_GSE, 0.0, h->bx, h->by, h->bz, h->rhoR.p, h->phiM.p, nullptr,
                        nullptr, h->ccelec, s, h->profiling ? h : nullptr));

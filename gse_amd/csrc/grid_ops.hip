@@ -9,12 +9,14 @@
 //                  virial moments, all in FP64, per block
 //  k_dot / k_sum_columns / k_reduce_partials   deterministic FP64 reductions
 //                  (fixed block count, fixed combination order)
-//  k_conv          one separable periodic convolution pass (gather form,
+//  k_conv_x / k_conv_yz  one separable periodic convolution pass (gather form,
 //                  CpuGaussCharge.cpp:448-514 / CpuConvolution.cpp:134-148);
 //                  FP32 sums are compensated (TwoProd/TwoSum)
 //  k_efield        CpuGaussCharge.cpp:62-99
 //  k_dipole        CpuGaussCharge.cpp:189-226
 #include <math.h>
+
+#include <algorithm>
 
 #include "gse_device.h"
 #include "gse_internal.h"
@@ -233,37 +235,152 @@ __global__ void k_efield(int X, int Y, int Z, T ihx, T ihy, T ihz, const T* __re
   }
 }
 
-// One separable pass in gather form: out[i] = sum_{ii=-R..R} k[ii+R] in[i+ii]
-// along the pass axis, periodic, summed in ii order.  Threads run x-fastest,
-// so loads are coalesced on every axis.
-template <typename T, int PASS>
+// Separable periodic convolution passes in gather form (CpuGaussCharge.cpp:
+// 448-514, CpuConvolution.cpp:134-148): out[i] = sum_{ii=-R..R} k[ii+R]
+// in[i+ii] along the pass axis, summed in ii order (FP32: compensated).
+// FP32: Kahan-compensated sum of the products (the summation error is carried
+// in c; -ffp-contract=off keeps each step rounded as written).
+template <typename T>
+struct ConvAcc {
+  T s = 0, c = 0;
+  __device__ __forceinline__ void add(T a, T b) {
+    const T y = a * b - c;
+    const T t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ __forceinline__ T value() const { return s; }
+};
+template <>
+struct ConvAcc<double> {
+  double s = 0.0;
+  __device__ __forceinline__ void add(double a, double b) { s += a * b; }
+  __device__ __forceinline__ double value() const { return s; }
+};
+
+__device__ __forceinline__ int wrap_small(int q, int M) {
+  while (q < 0) q += M;
+  while (q >= M) q -= M;
+  return q;
+}
+
+// x pass: each workgroup takes 4 rows (y, z) at a time, one 64-lane wave per
+// row; a row and its periodic halo of R values on each side are staged in LDS
+// with one coalesced read, then each lane produces 4 consecutive outputs
+// (RC > 0: from a register window) reading its inputs from LDS.
+template <typename T, int RC, bool SYM>
 __global__ void __launch_bounds__(256)
-    k_conv(int X, int Y, int Z, int R, const T* __restrict__ taps, const T* __restrict__ in,
-           T* __restrict__ out) {
-  const size_t n = (size_t)X * Y * Z;
-  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
-       p += (size_t)gridDim.x * blockDim.x) {
-    const int ix = (int)(p % X);
-    const size_t t = p / X;
-    const int iy = (int)(t % Y), iz = (int)(t / Y);
-    const int M = PASS == 0 ? X : PASS == 1 ? Y : Z;
-    const int c = PASS == 0 ? ix : PASS == 1 ? iy : iz;
-    const size_t stride = PASS == 0 ? 1 : PASS == 1 ? (size_t)X : (size_t)X * Y;
-    const size_t base = p - (size_t)c * stride;
-    if (sizeof(T) == 8) {
-      T s = 0;
-      for (int ii = -R; ii <= R; ii++) {
-        const int q = wrapi(c + ii, M);
-        s += taps[ii + R] * in[base + (size_t)q * stride];
+    k_conv_x(int X, int nrows, int R, const T* __restrict__ taps, const T* __restrict__ in,
+             T* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char conv_smem[];
+  const int nt = 2 * R + 1;
+  const int pitch = X + 2 * R;
+  T* tap = (T*)conv_smem;
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  T* row = tap + ((nt + 3) & ~3) + wv * pitch;
+  for (int i = threadIdx.x; i < nt; i += blockDim.x) tap[i] = taps[i];
+  for (int r0 = 4 * blockIdx.x; r0 < nrows; r0 += 4 * gridDim.x) {
+    const int r = r0 + wv;
+    if (r < nrows) {
+      const T* src = in + (size_t)r * X;
+      for (int i = ln; i < pitch; i += 64) row[i] = src[wrap_small(i - R, X)];
+    }
+    __syncthreads();
+    if (r < nrows) {
+      T* dst = out + (size_t)r * X;
+      if constexpr (RC > 0) {
+        constexpr int W = 4 + 2 * RC, NT = 2 * RC + 1;
+        for (int x0 = 4 * ln; x0 < X; x0 += 256) {
+          T win[W];
+#pragma unroll
+          for (int i = 0; i < W; i++) win[i] = row[min(x0 + i, pitch - 1)];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            if (x0 + u < X) {
+              ConvAcc<T> acc;
+              if constexpr (SYM) {
+                acc.add(tap[RC], win[u + RC]);
+#pragma unroll
+                for (int i = 1; i <= RC; i++)
+                  acc.add(tap[RC + i], win[u + RC - i] + win[u + RC + i]);
+              } else {
+#pragma unroll
+                for (int t = 0; t < NT; t++) acc.add(tap[t], win[u + t]);
+              }
+              dst[x0 + u] = acc.value();
+            }
+          }
+        }
+      } else {
+        for (int x = ln; x < X; x += 64) {
+          ConvAcc<T> acc;
+          for (int t = 0; t < nt; t++) acc.add(tap[t], row[x + t]);
+          dst[x] = acc.value();
+        }
       }
-      out[p] = s;
-    } else {
-      KSum<T> s;
-      for (int ii = -R; ii <= R; ii++) {
-        const int q = wrapi(c + ii, M);
-        s.add_prod(taps[ii + R], in[base + (size_t)q * stride]);
+    }
+    __syncthreads();
+  }
+}
+
+// y or z pass: an axis of length N with element stride S; x (contiguous) is
+// tiled by 64 lanes, the axis by kConvTJ outputs, `outer` is the remaining
+// axis (stride So).  The (kConvTJ + 2R) x 64 input tile is staged in LDS with
+// coalesced row reads, each thread then produces kConvTJ/4 outputs.
+constexpr int kConvTJ = 32;
+// RC > 0: R == RC known at compile time; each thread then produces 8
+// consecutive outputs from a register window of 8 + 2R inputs.
+template <typename T, int RC, bool SYM>
+__global__ void __launch_bounds__(256)
+    k_conv_yz(int X, int N, long long S, long long So, int R, const T* __restrict__ taps,
+              const T* __restrict__ in, T* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char conv_smem[];
+  const int nt = 2 * R + 1;
+  T* tap = (T*)conv_smem;
+  T* tile = tap + ((nt + 3) & ~3);
+  const int xl = threadIdx.x & 63, gq = threadIdx.x >> 6;
+  const int x = blockIdx.x * 64 + xl;
+  const int j0 = blockIdx.y * kConvTJ;
+  const size_t obase = (size_t)blockIdx.z * (size_t)So;
+  for (int i = threadIdx.x; i < nt; i += blockDim.x) tap[i] = taps[i];
+  const int rows = kConvTJ + 2 * R;
+  for (int r = gq; r < rows; r += 4) {
+    const int j = wrap_small(j0 - R + r, N);
+    tile[r * 64 + xl] = x < X ? in[obase + (size_t)x + (size_t)S * j] : (T)0;
+  }
+  __syncthreads();
+  if (x >= X) return;
+  if constexpr (RC > 0) {
+    constexpr int W = 8 + 2 * RC, NT = 2 * RC + 1;
+    const int jb = gq * 8;  // this thread's outputs j0 + jb .. j0 + jb + 7
+    T win[W], tp[NT];
+#pragma unroll
+    for (int i = 0; i < W; i++) win[i] = tile[(jb + i) * 64 + xl];
+#pragma unroll
+    for (int t = 0; t < NT; t++) tp[t] = tap[t];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      if (j0 + jb + u < N) {
+        ConvAcc<T> acc;
+        if constexpr (SYM) {
+          acc.add(tp[RC], win[u + RC]);
+#pragma unroll
+          for (int i = 1; i <= RC; i++) acc.add(tp[RC + i], win[u + RC - i] + win[u + RC + i]);
+        } else {
+#pragma unroll
+          for (int t = 0; t < NT; t++) acc.add(tp[t], win[u + t]);
+        }
+        out[obase + (size_t)x + (size_t)S * (j0 + jb + u)] = acc.value();
       }
-      out[p] = s.value();
+    }
+  } else {
+#pragma unroll 2
+    for (int u = 0; u < kConvTJ / 4; u++) {
+      const int jj = gq + 4 * u;
+      if (j0 + jj >= N) break;
+      ConvAcc<T> acc;
+      for (int t = 0; t < nt; t++) acc.add(tap[t], tile[(jj + t) * 64 + xl]);
+      out[obase + (size_t)x + (size_t)S * (j0 + jj)] = acc.value();
     }
   }
 }
@@ -410,21 +527,60 @@ hipError_t launch_efield(bool dbl, int X, int Y, int Z, double bx, double by, do
 }
 
 hipError_t launch_conv_pass(bool dbl, int pass, int X, int Y, int Z, int R, const void* taps,
-                            const void* in, void* out, hipStream_t s) {
-  const size_t n = (size_t)X * Y * Z;
-  const unsigned g = grid_for(n, 256, 16384);
-#define GSE_CONV(T, P) \
-  k_conv<T, P><<<g, 256, 0, s>>>(X, Y, Z, R, (const T*)taps, (const T*)in, (T*)out)
-  if (dbl) {
-    if (pass == 0) GSE_CONV(double, 0);
-    else if (pass == 1) GSE_CONV(double, 1);
-    else GSE_CONV(double, 2);
-  } else {
-    if (pass == 0) GSE_CONV(float, 0);
-    else if (pass == 1) GSE_CONV(float, 1);
-    else GSE_CONV(float, 2);
+                            const void* in, void* out, hipStream_t s, bool sym) {
+  const size_t ts = dbl ? 8 : 4;
+  const int nt = 2 * R + 1;
+  if (pass == 0) {
+    const int nrows = Y * Z;
+    const size_t lds = ts * (size_t)(((nt + 3) & ~3) + 4 * (X + 2 * R));
+    if (lds > 64 * 1024) return hipErrorInvalidConfiguration;
+    const unsigned g = (unsigned)std::min((nrows + 3) / 4, 8192);
+    if (dbl) {
+      if (R == 5)
+        k_conv_x<double, 5, false><<<g, 256, lds, s>>>(X, nrows, R, (const double*)taps,
+                                                       (const double*)in, (double*)out);
+      else
+        k_conv_x<double, 0, false><<<g, 256, lds, s>>>(X, nrows, R, (const double*)taps,
+                                                       (const double*)in, (double*)out);
+    } else {
+      if (R == 5 && sym)
+        k_conv_x<float, 5, true><<<g, 256, lds, s>>>(X, nrows, R, (const float*)taps,
+                                                     (const float*)in, (float*)out);
+      else if (R == 5)
+        k_conv_x<float, 5, false><<<g, 256, lds, s>>>(X, nrows, R, (const float*)taps,
+                                                      (const float*)in, (float*)out);
+      else
+        k_conv_x<float, 0, false><<<g, 256, lds, s>>>(X, nrows, R, (const float*)taps,
+                                                      (const float*)in, (float*)out);
+    }
+    return hipGetLastError();
   }
-#undef GSE_CONV
+  const int N = pass == 1 ? Y : Z;
+  const long long S = pass == 1 ? (long long)X : (long long)X * Y;
+  const int outer = pass == 1 ? Z : Y;
+  const long long So = pass == 1 ? (long long)X * Y : (long long)X;
+  const size_t lds = ts * (size_t)(((nt + 3) & ~3) + (kConvTJ + 2 * R) * 64);
+  if (lds > 64 * 1024 || outer > 65535) return hipErrorInvalidConfiguration;
+  const dim3 grid((unsigned)((X + 63) / 64), (unsigned)((N + kConvTJ - 1) / kConvTJ),
+                  (unsigned)outer);
+  if (dbl) {
+    if (R == 5)
+      k_conv_yz<double, 5, false><<<grid, 256, lds, s>>>(X, N, S, So, R, (const double*)taps,
+                                                         (const double*)in, (double*)out);
+    else
+      k_conv_yz<double, 0, false><<<grid, 256, lds, s>>>(X, N, S, So, R, (const double*)taps,
+                                                         (const double*)in, (double*)out);
+  } else {
+    if (R == 5 && sym)
+      k_conv_yz<float, 5, true><<<grid, 256, lds, s>>>(X, N, S, So, R, (const float*)taps,
+                                                       (const float*)in, (float*)out);
+    else if (R == 5)
+      k_conv_yz<float, 5, false><<<grid, 256, lds, s>>>(X, N, S, So, R, (const float*)taps,
+                                                        (const float*)in, (float*)out);
+    else
+      k_conv_yz<float, 0, false><<<grid, 256, lds, s>>>(X, N, S, So, R, (const float*)taps,
+                                                        (const float*)in, (float*)out);
+  }
   return hipGetLastError();
 }
 

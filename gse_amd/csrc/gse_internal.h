@@ -311,9 +311,10 @@ hipError_t launch_efield(bool dbl, int X, int Y, int Z, double bx, double by,
                          void* ez, hipStream_t s);
 
 // convolution: pass = 0 (x), 1 (y), 2 (z); taps on device (2R+1)
+// sym: the taps are symmetric, k[R-i] == k[R+i] (FP32 sums pairs then)
 hipError_t launch_conv_pass(bool dbl, int pass, int X, int Y, int Z, int R,
                             const void* taps, const void* in, void* out,
-                            hipStream_t s);
+                            hipStream_t s, bool sym = false);
 
 // charge statistics for the closed-form corrections: out[0] = sum q,
 // out[1] = sum q^2 (FP64 partials -> finalize)

@@ -375,3 +375,24 @@ def test_overflow_cluster_parity_and_determinism():
         g32.spreadCharge1(n, x32[perm].contiguous())
         assert torch.equal(g32.getRhoS(), r1)
     assert rel(host(r1), orc.spread(p.dims, p.sigma1, p.rcut1, host(x32), p.box)) <= 1e-6
+
+
+@pytest.mark.parametrize("R", [5, 3, 7])
+def test_convolution_fp32_asymmetric_taps(R):
+    """Taps that are not symmetric (no pairing) and radii other than 5 (no
+    register window) still match the oracle's three-pass convolution."""
+    dims = (24, 20, 28)
+    X, Y, Z = dims
+    rng = np.random.default_rng(20 + R)
+    rho = rng.normal(size=(Z, Y, X))
+    ks = [rng.random(2 * R + 1) for _ in range(3)]
+    c = api.CpuConvolution(F32, *dims)
+    c.setupKernel(R, R, R, *ks)
+    o = grid(dims, F32)
+    c.runXYZ(to_dev(rho, F32), o)
+    k32 = [k.astype(np.float32).astype(np.float64) for k in ks]
+    ref = orc.conv_run_xyz(dims, *k32, rho.astype(np.float32).astype(np.float64))
+    assert rel(host(o), ref) < 1e-6
+    c.run(to_dev(rho, F32), o)  # X pass only (CpuConvolution::run)
+    ref_x = orc.conv_run_x(dims, k32[0], rho.astype(np.float32).astype(np.float64))
+    assert rel(host(o), ref_x) < 1e-6
