@@ -9,9 +9,9 @@
 //                  virial moments, all in FP64, per block
 //  k_dot / k_sum_columns / k_reduce_partials   deterministic FP64 reductions
 //                  (fixed block count, fixed combination order)
-//  k_conv_x / k_conv_yz  one separable periodic convolution pass (gather form,
-//                  CpuGaussCharge.cpp:448-514 / CpuConvolution.cpp:134-148);
-//                  FP32 sums are compensated (TwoProd/TwoSum)
+//  k_conv_x / k_conv_pencil / k_conv_yz  one separable periodic convolution
+//                  pass (gather form, CpuGaussCharge.cpp:448-514 /
+//                  CpuConvolution.cpp:134-148); FP32 sums are Kahan-compensated
 //  k_efield        CpuGaussCharge.cpp:62-99
 //  k_dipole        CpuGaussCharge.cpp:189-226
 #include <math.h>
@@ -239,12 +239,13 @@ __global__ void k_efield(int X, int Y, int Z, T ihx, T ihy, T ihz, const T* __re
 // 448-514, CpuConvolution.cpp:134-148): out[i] = sum_{ii=-R..R} k[ii+R]
 // in[i+ii] along the pass axis, summed in ii order (FP32: compensated).
 // FP32: Kahan-compensated sum of the products (the summation error is carried
-// in c; -ffp-contract=off keeps each step rounded as written).
+// in c; -ffp-contract=off keeps the other steps rounded as written).  The
+// product enters as fma(a, b, -c): one rounding instead of two.
 template <typename T>
 struct ConvAcc {
   T s = 0, c = 0;
   __device__ __forceinline__ void add(T a, T b) {
-    const T y = a * b - c;
+    const T y = __builtin_fmaf(a, b, -c);
     const T t = s + y;
     c = (t - s) - y;
     s = t;
@@ -382,6 +383,58 @@ __global__ void __launch_bounds__(256)
       for (int t = 0; t < nt; t++) acc.add(tap[t], tile[(jj + t) * 64 + xl]);
       out[obase + (size_t)x + (size_t)S * (j0 + jj)] = acc.value();
     }
+  }
+}
+
+// y or z pass as register pencils: lane c owns the column at element offset
+// (c % cw) + (c / cw) * cs and produces outputs j0 .. j0 + TJ - 1 along the
+// axis (length N, element stride S) from a register window of TJ + 2R inputs
+// read straight from HBM; consecutive lanes own consecutive elements, so each
+// row of the window is one coalesced read per wave.  No LDS, no barrier.
+// The row index is wave-uniform: its periodic wrap is scalar work.
+template <typename T, int RC, bool SYM, int TJ>
+__global__ void __launch_bounds__(256)
+    k_conv_pencil(int C, int cw, long long cs, int N, long long S, const T* __restrict__ taps,
+                  const T* __restrict__ in, T* __restrict__ out) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= C) return;
+  constexpr int W = TJ + 2 * RC, NT = 2 * RC + 1;
+  const int j0 = blockIdx.y * TJ;
+  const size_t base = (size_t)(c % cw) + (size_t)(c / cw) * (size_t)cs;
+  T tp[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) tp[t] = taps[t];
+  int j = (j0 - RC) % N;
+  if (j < 0) j += N;
+  const T* src = in + base;
+  long long o = S * j;
+  T win[W];
+#pragma unroll
+  for (int i = 0; i < W; i++) {
+    win[i] = src[o];
+    if (++j == N) {
+      j = 0;
+      o = 0;
+    } else {
+      o += S;
+    }
+  }
+  T* dst = out + base + S * j0;
+#pragma unroll
+  for (int u = 0; u < TJ; u++) {
+    if (j0 + u < N) {
+      ConvAcc<T> acc;
+      if constexpr (SYM) {
+        acc.add(tp[RC], win[u + RC]);
+#pragma unroll
+        for (int i = 1; i <= RC; i++) acc.add(tp[RC + i], win[u + RC - i] + win[u + RC + i]);
+      } else {
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc.add(tp[t], win[u + t]);
+      }
+      *dst = acc.value();
+    }
+    dst += S;
   }
 }
 
@@ -557,6 +610,29 @@ hipError_t launch_conv_pass(bool dbl, int pass, int X, int Y, int Z, int R, cons
   }
   const int N = pass == 1 ? Y : Z;
   const long long S = pass == 1 ? (long long)X : (long long)X * Y;
+  if (R == 5 && (long long)X * Y * Z < (1ll << 31)) {
+    // pencils: y pass columns are (x, z) with x fastest, z pass columns the
+    // whole xy plane; tiles of 32 outputs when that still gives >= 8 waves
+    // per CU on 256 CUs, else 16
+    const int C = pass == 1 ? X * Z : X * Y;
+    const int cw = pass == 1 ? X : X * Y;
+    const long long cs = pass == 1 ? (long long)X * Y : 0;
+    const unsigned gx = (unsigned)((C + 255) / 256);
+    const bool big = (long long)gx * 4 * ((N + 31) / 32) >= 256 * 8;
+    const dim3 grid(gx, (unsigned)(big ? (N + 31) / 32 : (N + 15) / 16));
+#define GSE_PENCIL(TT, SY, TJ)                                                              \
+  k_conv_pencil<TT, 5, SY, TJ><<<grid, 256, 0, s>>>(C, cw, cs, N, S, (const TT*)taps, \
+                                                     (const TT*)in, (TT*)out)
+    if (dbl) {
+      if (big) GSE_PENCIL(double, false, 32); else GSE_PENCIL(double, false, 16);
+    } else if (sym) {
+      if (big) GSE_PENCIL(float, true, 32); else GSE_PENCIL(float, true, 16);
+    } else {
+      if (big) GSE_PENCIL(float, false, 32); else GSE_PENCIL(float, false, 16);
+    }
+#undef GSE_PENCIL
+    return hipGetLastError();
+  }
   const int outer = pass == 1 ? Z : Y;
   const long long So = pass == 1 ? (long long)X * Y : (long long)X;
   const size_t lds = ts * (size_t)(((nt + 3) & ~3) + (kConvTJ + 2 * R) * 64);

@@ -396,3 +396,34 @@ def test_convolution_fp32_asymmetric_taps(R):
     c.run(to_dev(rho, F32), o)  # X pass only (CpuConvolution::run)
     ref_x = orc.conv_run_x(dims, k32[0], rho.astype(np.float32).astype(np.float64))
     assert rel(host(o), ref_x) < 1e-6
+
+
+@pytest.mark.parametrize("dims,dt,sym", [
+    ((9, 4, 7), F64, False),      # axes shorter than the radius: repeated wraps
+    ((9, 4, 7), F32, True),
+    ((40, 33, 50), F32, False),   # ragged lengths, last tile partly outside
+    ((256, 40, 256), F32, True),  # y pass in 32-row tiles, z pass in 16-row tiles
+    ((256, 40, 256), F64, False),
+])
+def test_convolution_pencil_shapes(dims, dt, sym):
+    """Radius-5 y and z passes (register pencils): periodic wraps, ragged
+    tiles and both tile heights match the oracle's three-pass convolution."""
+    X, Y, Z = dims
+    R = 5
+    rng = np.random.default_rng(X * 7 + Y + Z)
+    rho = rng.normal(size=(Z, Y, X))
+    if sym:
+        k = np.exp(-np.arange(-R, R + 1) ** 2 / (2 * 2.0 ** 2))
+        ks = [k, k * 0.5, k * 1.5]
+    else:
+        ks = [rng.random(2 * R + 1) for _ in range(3)]
+    c = api.CpuConvolution(dt, *dims)
+    c.setupKernel(R, R, R, *ks)
+    o = grid(dims, dt)
+    c.runXYZ(to_dev(rho, dt), o)
+    if dt == F32:
+        k32 = [k.astype(np.float32).astype(np.float64) for k in ks]
+        ref = orc.conv_run_xyz(dims, *k32, rho.astype(np.float32).astype(np.float64))
+        assert rel(host(o), ref) < 1e-6
+    else:
+        assert rel(host(o), orc.conv_run_xyz(dims, *ks, rho)) <= TOL
