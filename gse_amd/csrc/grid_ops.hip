@@ -9,12 +9,14 @@
 //                  virial moments, all in FP64, per block
 //  k_dot / k_sum_columns / k_reduce_partials   deterministic FP64 reductions
 //                  (fixed block count, fixed combination order)
-//  k_conv_x / k_conv_pencil / k_conv_yz  one separable periodic convolution
-//                  pass (gather form, CpuGaussCharge.cpp:448-514 /
+//  k_conv_x4 / k_conv_x / k_conv_pencil / k_conv_yz  one separable periodic
+//                  convolution pass (gather form, CpuGaussCharge.cpp:448-514 /
 //                  CpuConvolution.cpp:134-148); FP32 sums are Kahan-compensated
 //  k_efield        CpuGaussCharge.cpp:62-99
 //  k_dipole        CpuGaussCharge.cpp:189-226
 #include <math.h>
+
+#include <stdint.h>
 
 #include <algorithm>
 
@@ -324,6 +326,82 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// x pass for X % 4 == 0, X >= 8, radius RC <= 8: workgroup b takes RW
+// consecutive rows, one contiguous span of RW * X elements, copies it to LDS
+// in 4-element pieces (each row at pitch X + 16, interior at offset 8), fills
+// each row's periodic halo from LDS, then works through items of 4
+// consecutive outputs of one row: 5 aligned 4-element LDS reads give the
+// item's window and one 4-element store writes it.  Items are spread over
+// the whole workgroup, so short rows leave no lanes idle.
+template <typename T, int RC, bool SYM>
+__global__ void __launch_bounds__(256)
+    k_conv_x4(int X, int nrows, int RW, const T* __restrict__ taps, const T* __restrict__ in,
+              T* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char conv_smem[];
+  static_assert(RC <= 8, "halo slots");
+  constexpr int NT = 2 * RC + 1;
+  T* lds = (T*)conv_smem;
+  const int P = X + 16, X4 = X >> 2;
+  // e / X4 as a multiply-high: exact for e < 2^32 / X4
+  const unsigned mg = 0xFFFFFFFFu / (unsigned)X4 + 1u;
+  T tp[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) tp[t] = taps[t];
+  const int r0 = blockIdx.x * RW;
+  const int rw = min(RW, nrows - r0);
+  const int n4 = rw * X4;
+  const V4<T>* src = (const V4<T>*)(in + (size_t)r0 * X);
+  for (int e = threadIdx.x; e < n4; e += 256) {
+    const int row = (int)__umulhi((unsigned)e, mg), q = e - row * X4;
+    *(V4<T>*)(lds + row * P + 8 + 4 * q) = src[e];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < rw * 2 * RC; e += 256) {
+    const int row = e / (2 * RC), h = e - row * (2 * RC);
+    T* L = lds + row * P + 8;
+    if (h < RC)
+      L[h - RC] = L[X - RC + h];  // x = h - RC  <-  X - RC + h
+    else
+      L[X + h - RC] = L[h - RC];  // x = X + h - RC  <-  h - RC
+  }
+  __syncthreads();
+  V4<T>* dst = (V4<T>*)(out + (size_t)r0 * X);
+  for (int e = threadIdx.x; e < n4; e += 256) {
+    const int row = (int)__umulhi((unsigned)e, mg), q = e - row * X4;
+    const T* L = lds + row * P + 4 * q;  // window input i (x = 4q - RC + i) is w[8 - RC + i]
+    T w[20];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      const V4<T> v = *(const V4<T>*)(L + 4 * k);
+      w[4 * k] = v.x;
+      w[4 * k + 1] = v.y;
+      w[4 * k + 2] = v.z;
+      w[4 * k + 3] = v.w;
+    }
+    T res[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      ConvAcc<T> acc;
+      constexpr int c0 = 8 - RC;
+      if constexpr (SYM) {
+        acc.add(tp[RC], w[c0 + u + RC]);
+#pragma unroll
+        for (int i = 1; i <= RC; i++) acc.add(tp[RC + i], w[c0 + u + RC - i] + w[c0 + u + RC + i]);
+      } else {
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc.add(tp[t], w[c0 + u + t]);
+      }
+      res[u] = acc.value();
+    }
+    V4<T> o;
+    o.x = res[0];
+    o.y = res[1];
+    o.z = res[2];
+    o.w = res[3];
+    dst[e] = o;
+  }
+}
+
 // y or z pass: an axis of length N with element stride S; x (contiguous) is
 // tiled by 64 lanes, the axis by kConvTJ outputs, `outer` is the remaining
 // axis (stride So).  The (kConvTJ + 2R) x 64 input tile is staged in LDS with
@@ -585,6 +663,26 @@ hipError_t launch_conv_pass(bool dbl, int pass, int X, int Y, int Z, int R, cons
   const int nt = 2 * R + 1;
   if (pass == 0) {
     const int nrows = Y * Z;
+    const size_t vb = 4 * ts;
+    if (R == 5 && X % 4 == 0 && X >= 8 && (uintptr_t)in % vb == 0 && (uintptr_t)out % vb == 0 &&
+        (size_t)X * nrows < (1ull << 31)) {
+      // about 512 four-output items per workgroup
+      const int RW = std::max(1, 512 / (X / 4));
+      const size_t lds = ts * (size_t)RW * (X + 16);
+      if (lds <= 64 * 1024) {
+        const unsigned g = (unsigned)((nrows + RW - 1) / RW);
+#define GSE_CONVX4(TT, SY) \
+  k_conv_x4<TT, 5, SY><<<g, 256, lds, s>>>(X, nrows, RW, (const TT*)taps, (const TT*)in, (TT*)out)
+        if (dbl)
+          GSE_CONVX4(double, false);
+        else if (sym)
+          GSE_CONVX4(float, true);
+        else
+          GSE_CONVX4(float, false);
+#undef GSE_CONVX4
+        return hipGetLastError();
+      }
+    }
     const size_t lds = ts * (size_t)(((nt + 3) & ~3) + 4 * (X + 2 * R));
     if (lds > 64 * 1024) return hipErrorInvalidConfiguration;
     const unsigned g = (unsigned)std::min((nrows + 3) / 4, 8192);

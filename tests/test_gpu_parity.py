@@ -404,10 +404,13 @@ def test_convolution_fp32_asymmetric_taps(R):
     ((40, 33, 50), F32, False),   # ragged lengths, last tile partly outside
     ((256, 40, 256), F32, True),  # y pass in 32-row tiles, z pass in 16-row tiles
     ((256, 40, 256), F64, False),
+    ((8, 5, 6), F32, True),       # shortest row of the 4-wide x pass (halo from 8 values)
+    ((12, 9, 10), F64, False),
 ])
 def test_convolution_pencil_shapes(dims, dt, sym):
-    """Radius-5 y and z passes (register pencils): periodic wraps, ragged
-    tiles and both tile heights match the oracle's three-pass convolution."""
+    """Radius-5 passes (4-wide x pass, register-pencil y and z passes):
+    periodic wraps, ragged tiles and both tile heights match the oracle's
+    three-pass convolution."""
     X, Y, Z = dims
     R = 5
     rng = np.random.default_rng(X * 7 + Y + Z)
