@@ -1,14 +1,14 @@
 """GSE reciprocal-space benchmark on one MI355X.
 
-One step = bin -> spread sigma1 (int64 fixed point) -> r2c -> G(k) k-pass
-(FP64 energy) -> c2r -> gather (reference force + per-particle potential),
-on device-resident synthetic input, FP32 grids/kernels with FP64 Green's
-table and reductions.  Workload: 16,000 rigid water-like triplets (48,000
-charges) in L = 78.3 A, M = 80, sigma = 2.12.
+One step = bin -> spread sigma1 (fixed point, bit-reproducible) -> brick
+reduce -> r2c -> G(k) k-pass (FP64 energy) -> c2r -> gather (reference force
++ per-particle potential), on device-resident synthetic input, FP32 grids and
+kernels with FP64 Green's table and reductions.  Workload: 16,000 rigid
+water-like triplets (48,000 charges) in L = 78.3 A, M = 80, sigma = 2.12.
 
 Prints ONE JSON line (rank 0, N = 1).  Extra figures (FP64 mode, adjoint
-cost, the 1,000,002-charge M = 216 system, batched systems/s) go into
-"extras".
+cost, the separable convolution at M = 80 and M = 216, the mode with the
+on-grid convolution, batched systems/s) go into "extras".
 """
 import argparse
 import json
@@ -30,9 +30,18 @@ HBM_PEAK_GBS = 8000.0
 # (FETCH x2.000, WRITE x1.000); the commands and per-kernel table are in
 # profiles/k_gather_k_spread_water48k_M80_fp32_hbm_traffic.md.
 MEASURED_TRAFFIC = {
-    ("water_48k_M80", "fp32", "gather"): 8582531,
-    ("water_48k_M80", "fp32", "spread"): 56767795,
+    ("water_48k_M80", "fp32", "gather"): 10610278,
+    ("water_48k_M80", "fp32", "spread"): 41774182,
 }
+# Wave-level VALU instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU, same
+# file).  The gather and the spread issue far more vector instructions than
+# their bytes need; their bound is VALU issue: 256 CUs x 2.4 GHz x one wave64
+# VALU instruction per CU per clock (4 SIMDs, 16 lanes each) = 614.4 G/s.
+MEASURED_VALU = {
+    ("water_48k_M80", "fp32", "gather"): 26385390,
+    ("water_48k_M80", "fp32", "spread"): 11386716,
+}
+VALU_PEAK_GINST = 614.4
 
 
 def parse():
@@ -175,6 +184,11 @@ def main():
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": MEASURED_TRAFFIC.get((args.workload, args.precision, dom)),
             "bytes_per_launch": sb[dom], "avg_ms": round(ours[dom], 5)}
+    nvalu = MEASURED_VALU.get((args.workload, args.precision, dom))
+    if nvalu:
+        g = nvalu / (ours[dom] * 1e-3) / 1e9
+        roof["valu"] = {"insts_per_launch": nvalu, "achieved_Ginst_s": round(g, 1),
+                        "peak_Ginst_s": VALU_PEAK_GINST, "frac": round(g / VALU_PEAK_GINST, 4)}
 
     # accuracy against the FP64 oracle and the CPU baseline (oracle timing)
     torch.cuda.synchronize()
