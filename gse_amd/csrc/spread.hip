@@ -1,20 +1,30 @@
-// Cell-binned Gaussian charge spreading with fixed-point int64 accumulation
+// Cell-binned Gaussian charge spreading with fixed-point accumulation
 // (CpuGaussCharge::spreadChargeToGrid, CpuGaussCharge.cpp:235-384).
 //
 //   k_bin          one pass over the particles: cube origin
 //                  o = wrap(round(x*inv_h) - n), bin of o, slot in the bin by
-//                  atomic count; the particle is copied into its bin's fixed
+//                  atomic count; the particle, its original index and its
+//                  cube origin within the bin are copied into the bin's fixed
 //                  slots (or onto the overflow list); max|q| and rejected
 //                  particles are counted.
-//   k_spread       one workgroup per bin: the (B+2n)^3 brick lives in LDS as
-//                  int64.  The bin's particles are staged into LDS, one wave per
+//   k_spread       one workgroup per bin: the (B+2n)^3 brick lives in LDS
+//                  (int64 at the global scale 2^e, or int32 at a per-bin
+//                  scale 2^eb <= 2^e chosen from the bin's particle count).
+//                  The bin's particles are staged into LDS, one wave per
 //                  particle computes its 3(2n+1) factors, then one wave per
 //                  particle walks its cube with lanes on (tx,ty) columns and
-//                  adds round(q*pref*wx*wy*wz*2^e) with ds_add_u64.  The brick
-//                  is added into the global int64 grid with 64-bit atomics.
-//                  kOverflowWG extra workgroups add the overflow particles
-//                  straight into the global grid.
-//   k_fixed_to_grid  int64 -> T with 2^-e, the int64 grid re-zeroed.
+//                  adds round(q*pref*wx*wy*wz*2^eb) with LDS integer atomics.
+//                  The brick is then either stored whole (with its shift
+//                  e - eb) for the brick reduce, or added into the global
+//                  int64 grid with 64-bit atomics.  kOverflowWG extra
+//                  workgroups add the overflow particles into the global
+//                  int64 grid at their home bin's scale.
+//   k_brick_reduce[_tile|_tile4]  each output cell sums, exactly in int64,
+//                  the stored bricks of the bins whose bricks cover it (each
+//                  shifted to 2^e) plus the int64 overflow grid when the
+//                  overflow list was used, then converts to T with 2^-e.
+//   k_fixed_to_grid  int64 -> T with 2^-e, the int64 grid re-zeroed (atomic
+//                  flush).
 // Integer addition is associative, so the grid is bit-identical run to run
 // whatever order particles and workgroups run in.  The scale 2^e follows from
 // max|q|, the particle count and the support (fixed_exponent).
