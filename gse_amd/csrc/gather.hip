@@ -153,13 +153,15 @@ __host__ __device__ inline int brick_pitch(int ct_bytes, int X, int Bx, int Lx) 
   return brick_v4(ct_bytes, X, Bx) ? (Lx + 3) & ~3 : Lx;
 }
 
-// FL: GatherFlags bit mask; NZC: 2nz+1 known at compile time (0 = runtime)
+// FL: GatherFlags bit mask; NZC: 2nz+1 known at compile time (0 = runtime);
+// PL: the brick's plane size Px*Ly known at compile time (0 = runtime), so
+// the cube walk's LDS reads take their z step as an immediate offset
 //
 // One bin of one system: its cnt particles start at `sorted`/`perm` (slot 0
 // of the bin), the system's potential grid at `phi`; perm holds the output
 // index of each particle.  Ends with a barrier, so a workgroup may call it
 // for several bins in a row.
-template <typename CT, typename AT, int NZC, int FL, int BS>
+template <typename CT, typename AT, int NZC, int FL, int BS, int PL = 0>
 __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt,
                                            const V4<CT>* __restrict__ sorted,
                                            const int* __restrict__ perm,
@@ -308,6 +310,9 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
   // right side kept per z (one compare per point).  FP64 keeps the
   // reference's rounding, (dx^2+dy^2)+dz^2 < rc^2 (CpuGaussCharge.cpp:157).
   constexpr bool FAST_SPH = SPH && !VIR && sizeof(CT) == 4;
+  // FP32 accumulation without the virial: the z walk as FMAs (see below)
+  constexpr bool FMA_FORM = NZC > 0 && !VIR && sizeof(CT) == 4 && sizeof(AT) == 4;
+  const int PLv = PL > 0 ? PL : PxLy;
 
   for (int c0 = s0; c0 < s1; c0 += cap) {
     const int na = min(cap, s1 - c0);
@@ -347,13 +352,14 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
     for (int ai = wid; ai < na; ai += nwv) {
       const CT* w = sW + (size_t)ai * NW * 2;
       const int base = sB[ai];
-      CT wzr[NZA], dzr[NZA], dz2[NZA];
+      CT wzr[NZA], dzr[NZA], dz2[NZA], vzr[NZA];
       if (NZC > 0) {
 #pragma unroll
         for (int t = 0; t < NZA; t++) {
           wzr[t] = w[2 * (NX + NY + t)];
           dzr[t] = w[2 * (NX + NY + t) + 1];
           dz2[t] = FAST_SPH ? rc2 - mul_rn(dzr[t], dzr[t]) : mul_rn(dzr[t], dzr[t]);
+          vzr[t] = wzr[t] * dzr[t];
         }
       }
       AT f[13];
@@ -371,10 +377,24 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
         const CT dxy2 = add_rn(mul_rn(dx, dx), mul_rn(dy, dy));
         const int a0 = base + k_off[k];
         AT B0 = 0, B1 = 0, B2 = 0, A0 = 0, A1 = 0;
-        if (NZC > 0) {
+        if (FMA_FORM) {
+          // each potential value enters every sum through one FMA against a
+          // per-particle z factor (wz, wz*dz), the sphere mask selects it
 #pragma unroll
           for (int t = 0; t < NZA; t++) {
-            const CT ph = brick[a0 + t * PxLy];
+            const CT ph = brick[a0 + t * PLv];
+            if (NEED_B0) B0 = __builtin_fmaf(wzr[t], ph, B0);
+            if (NEED_B1) B1 = __builtin_fmaf(vzr[t], ph, B1);
+            if (SPH) {
+              const CT ms = dxy2 < dz2[t] ? ph : (CT)0;
+              A0 = __builtin_fmaf(wzr[t], ms, A0);
+              A1 = __builtin_fmaf(vzr[t], ms, A1);
+            }
+          }
+        } else if (NZC > 0) {
+#pragma unroll
+          for (int t = 0; t < NZA; t++) {
+            const CT ph = brick[a0 + t * PLv];
             const AT wph = (AT)(wzr[t] * ph);
             const AT dzt = (AT)dzr[t];
             if (NEED_B0) B0 += wph;
@@ -444,7 +464,7 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
 }
 
 
-template <typename CT, typename AT, int NZC, int FL, int BS>
+template <typename CT, typename AT, int NZC, int FL, int BS, int PL = 0>
 __global__ void __launch_bounds__(BS)
     k_gather(GatherArgs a, const V4<CT>* __restrict__ xyzq, const V4<CT>* __restrict__ sorted,
              const int* __restrict__ perm, const int* __restrict__ counts,
@@ -478,7 +498,7 @@ __global__ void __launch_bounds__(BS)
   if (threadIdx.x == 0) counts_other[bin] = 0;
   if (cnt == 0) return;
   const size_t s0 = (size_t)bin * a.cap_bin;
-  gather_bin<CT, AT, NZC, FL, BS>(a, bin, cnt, sorted + s0, perm + s0, loc + s0, phi, o, ds,
+  gather_bin<CT, AT, NZC, FL, BS, PL>(a, bin, cnt, sorted + s0, perm + s0, loc + s0, phi, o, ds,
                                   smem_raw);
 }
 
@@ -683,9 +703,17 @@ static hipError_t gather_nz(GatherArgs a, const Binning& w, const void* phi, con
     k_gather<CT, AT, NZV, FL, 512><<<nwg, 512, lds, s>>>(GSE_G_ARGS);     \
   else                                                                     \
     k_gather<CT, AT, NZV, FL, 256><<<nwg, 256, lds, s>>>(GSE_G_ARGS);
+  const int pxly = brick_pitch(sizeof(CT), a.g.X, a.b.Bx, a.b.Lx) * a.b.Ly;
   switch (2 * a.g.nz + 1) {
     case 11: GSE_G_BS(11) break;
-    case 13: GSE_G_BS(13) break;
+    case 13:
+      // 8^3 bins with n = 6: 20 x 20 planes of the brick
+      if (sizeof(CT) == 4 && bs == 512 && pxly == 400)
+        k_gather<CT, AT, 13, FL, 512, 400><<<nwg, 512, lds, s>>>(GSE_G_ARGS);
+      else {
+        GSE_G_BS(13)
+      }
+      break;
     case 15: GSE_G_BS(15) break;
     case 17: GSE_G_BS(17) break;
     default: GSE_G_BS(0) break;
