@@ -281,12 +281,18 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
   const CT inv2s2 = (CT)a.inv2s2;
   double hsel, ihsel;
   int nsel, jj, axis;
+  // the half-widths as opaque uniform values: a per-lane choice between
+  // fields of the argument struct would otherwise become an indexed copy
+  // of the struct in scratch memory
+  const int gnx = __builtin_amdgcn_readfirstlane(g.nx);
+  const int gny = __builtin_amdgcn_readfirstlane(g.ny);
+  const int gnz = __builtin_amdgcn_readfirstlane(g.nz);
   if (lane < NX) {
-    axis = 0; jj = lane; hsel = g.hx; ihsel = g.ihx; nsel = g.nx;
+    axis = 0; jj = lane; hsel = g.hx; ihsel = g.ihx; nsel = gnx;
   } else if (lane < NX + NY) {
-    axis = 1; jj = lane - NX; hsel = g.hy; ihsel = g.ihy; nsel = g.ny;
+    axis = 1; jj = lane - NX; hsel = g.hy; ihsel = g.ihy; nsel = gny;
   } else {
-    axis = lane < NW ? 2 : -1; jj = lane - NX - NY; hsel = g.hz; ihsel = g.ihz; nsel = g.nz;
+    axis = lane < NW ? 2 : -1; jj = lane - NX - NY; hsel = g.hz; ihsel = g.ihz; nsel = gnz;
   }
   int k_tx[kMaxK], k_ty[kMaxK], k_off[kMaxK];
   bool k_ok[kMaxK];
@@ -313,6 +319,19 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
   // FP32 accumulation without the virial: the z walk as FMAs (see below)
   constexpr bool FMA_FORM = NZC > 0 && !VIR && sizeof(CT) == 4 && sizeof(AT) == 4;
   const int PLv = PL > 0 ? PL : PxLy;
+  // PL > 0 also means a cubic support of NZC points per axis.  The FMA form
+  // then walks two charges per wave, one per 32-lane half: the per-charge
+  // set-up and the cross-lane reduction are shared by two charges, and the
+  // NZC^2 columns fill 32-lane slots instead of 64-lane ones.
+  constexpr bool PAIR = FMA_FORM && PL > 0;
+  constexpr int NXYP = NZA * NZA, KP = (NXYP + 31) / 32;
+  int p_tx[KP], p_ty[KP];
+#pragma unroll
+  for (int k = 0; k < KP; k++) {
+    const int idx = (lane & 31) + 32 * k;
+    p_ty[k] = idx < NXYP ? idx / NZA : 0;
+    p_tx[k] = idx < NXYP ? idx - p_ty[k] * NZA : 0;
+  }
 
   for (int c0 = s0; c0 < s1; c0 += cap) {
     const int na = min(cap, s1 - c0);
@@ -348,6 +367,76 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
       }
     }
     __syncthreads();
+    if constexpr (PAIR) {
+      // phase 2, two charges per wave: half h of wave `wid` takes charge
+      // 2*g2 + h; a half without a charge walks the other half's charge
+      // with its sums discarded (no divergence inside the walk)
+      const int h = lane >> 5, l32 = lane & 31;
+      for (int g2 = wid; 2 * g2 < na; g2 += nwv) {
+        const int ai = 2 * g2 + h;
+        const bool live = ai < na;
+        const int aic = live ? ai : 2 * g2;
+        const CT* w = sW + (size_t)aic * NW * 2;
+        const int base = sB[aic];
+        CT wzr[NZA], dz2[NZA], vzr[NZA];
+#pragma unroll
+        for (int t = 0; t < NZA; t++) {
+          wzr[t] = w[2 * (2 * NZA + t)];
+          const CT dz = w[2 * (2 * NZA + t) + 1];
+          dz2[t] = rc2 - mul_rn(dz, dz);
+          vzr[t] = wzr[t] * dz;
+        }
+        AT f[13];
+#pragma unroll
+        for (int c = 0; c < 13; c++) f[c] = 0;
+#pragma unroll
+        for (int k = 0; k < KP; k++) {
+          if (l32 + 32 * k >= NXYP) continue;
+          const CT wx = w[2 * p_tx[k]];
+          const CT dx = w[2 * p_tx[k] + 1];
+          const CT wy = w[2 * (NZA + p_ty[k])];
+          const CT dy = w[2 * (NZA + p_ty[k]) + 1];
+          const CT dxy2 = add_rn(mul_rn(dx, dx), mul_rn(dy, dy));
+          const int a0 = base + p_tx[k] + Px * p_ty[k];
+          AT B0 = 0, B1 = 0, A0 = 0, A1 = 0;
+#pragma unroll
+          for (int t = 0; t < NZA; t++) {
+            const CT ph = brick[a0 + t * PL];
+            if (NEED_B0) B0 = __builtin_fmaf(wzr[t], ph, B0);
+            if (NEED_B1) B1 = __builtin_fmaf(vzr[t], ph, B1);
+            if (SPH) {
+              const CT ms = dxy2 < dz2[t] ? ph : (CT)0;
+              A0 = __builtin_fmaf(wzr[t], ms, A0);
+              A1 = __builtin_fmaf(vzr[t], ms, A1);
+            }
+          }
+          const AT wxy = wx * wy;
+          if (SPH) {
+            const AT t0 = wxy * A0;
+            f[0] = fmac(t0, dx, f[0]);
+            f[1] = fmac(t0, dy, f[1]);
+            f[2] = fmac(wxy, A1, f[2]);
+          }
+          if (NEED_B0) {
+            const AT t0 = wxy * B0;
+            if (CUB) {
+              f[3] = fmac(t0, dx, f[3]);
+              f[4] = fmac(t0, dy, f[4]);
+              f[5] = fmac(wxy, B1, f[5]);
+            }
+            if (POT) f[6] += t0;
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 7; c++) {
+          const bool used = (c < 3 && SPH) || (c >= 3 && c < 6 && CUB) || (c == 6 && POT);
+          if (used) f[c] = half_total(live ? f[c] : (AT)0);
+        }
+        if (live && l32 == 0) gather_store<AT, FL>(a, o, sP[ai], sQ[ai], ds, f);
+      }
+      __syncthreads();
+      continue;
+    }
     // phase 2: one wave per particle
     for (int ai = wid; ai < na; ai += nwv) {
       const CT* w = sW + (size_t)ai * NW * 2;
@@ -707,12 +796,16 @@ static hipError_t gather_nz(GatherArgs a, const Binning& w, const void* phi, con
   switch (2 * a.g.nz + 1) {
     case 11: GSE_G_BS(11) break;
     case 13:
-      // 8^3 bins with n = 6: 20 x 20 planes of the brick
-      if (sizeof(CT) == 4 && bs == 512 && pxly == 400)
-        k_gather<CT, AT, 13, FL, 512, 400><<<nwg, 512, lds, s>>>(GSE_G_ARGS);
-      else {
-        GSE_G_BS(13)
+      // 8^3 bins with n = 6 on every axis: 20 x 20 planes of the brick (FP32,
+      // no virial: the virial variant keeps a runtime stride, its register
+      // budget has no room for the immediates)
+      if constexpr (sizeof(CT) == 4 && !(FL & GF_VIRIAL)) {
+        if (bs == 512 && pxly == 400 && a.g.nx == a.g.nz && a.g.ny == a.g.nz) {
+          k_gather<CT, AT, 13, FL, 512, 400><<<nwg, 512, lds, s>>>(GSE_G_ARGS);
+          break;
+        }
       }
+      GSE_G_BS(13)
       break;
     case 15: GSE_G_BS(15) break;
     case 17: GSE_G_BS(17) break;

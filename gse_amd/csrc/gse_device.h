@@ -72,6 +72,18 @@ __device__ __forceinline__ T wave_total(T v) {
   return (r0 + r1) + (r2 + r3);
 }
 
+// Sum over each 32-lane half of the wave: lanes 0-31 get the sum of lanes
+// 0-31, lanes 32-63 the sum of lanes 32-63 (fixed order, deterministic).
+template <typename T>
+__device__ __forceinline__ T half_total(T v) {
+  v += dpp_move<0xB1>(v);
+  v += dpp_move<0x4E>(v);
+  v += dpp_move<0x141>(v);
+  v += dpp_move<0x140>(v);
+  const T r0 = readlane(v, 0), r1 = readlane(v, 16), r2 = readlane(v, 32), r3 = readlane(v, 48);
+  return (threadIdx.x & 32) ? r2 + r3 : r0 + r1;
+}
+
 // Non-negative modulo
 __device__ __forceinline__ int wrapi(int i, int m) {
   int r = i % m;

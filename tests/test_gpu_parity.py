@@ -273,6 +273,18 @@ def test_gsek_fp32_water48k_force_rms():
     assert orc.erms(r.force_cube, host(o["force_cube"]).T) <= 1e-5
     E = float(host(o["energy"])[0])
     assert abs(E - r.energy) <= 1e-5 * abs(r.energy)
+    # the output sets without the virial take the two-charges-per-wave walk
+    # on this grid (8^3 bins, n = 6): sphere force, +potential, cube+potential
+    x32 = to_dev(xyzq, F32)
+    o1 = gs.step(x32, mode=0, force=True, potential=False)
+    assert orc.erms(r.force, host(o1["force"]).T) <= 1e-5
+    o2 = gs.step(x32, mode=0, force=True, potential=True)
+    assert orc.erms(r.force, host(o2["force"]).T) <= 1e-5
+    assert rel(host(o2["pot"]), r.pot) <= 1e-5
+    o3 = gs.step(x32, mode=0, force=False, potential=True, cube=True)
+    assert orc.erms(r.force_cube, host(o3["force_cube"]).T) <= 1e-5
+    assert rel(host(o3["pot"]), r.pot) <= 1e-5
+    assert torch.equal(o2["pot"], o3["pot"])
     # (AT,CT) = (double, float): the reference's mixed instantiation
     gm = api.CpuGSEk(F64, F32, M, M, M, sigma, 0.4, 3.0, 3.0, L, L, L)
     om = gm.step(to_dev(xyzq, F32), mode=0, force=True, potential=False)
