@@ -15,6 +15,14 @@ __device__ __forceinline__ long long fixed_round(double v) {
          (long long)__double_as_longlong(magic);
 }
 
+// round-to-nearest(a*b) of the exact product (|a*b| below 2^51): one FMA
+// adds the product to 1.5*2^52 with a single rounding.
+__device__ __forceinline__ long long fixed_round_prod(double a, double b) {
+  const double magic = 6755399441055744.0;  // 1.5 * 2^52
+  return (long long)__double_as_longlong(__builtin_fma(a, b, magic)) -
+         (long long)__double_as_longlong(magic);
+}
+
 // Uniform (wave-wide) broadcast of lane `src`'s value; src must be uniform.
 __device__ __forceinline__ double readlane_d(double v, int src) {
   int lo = __builtin_amdgcn_readlane(__double2loint(v), src);

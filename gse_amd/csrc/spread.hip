@@ -179,10 +179,10 @@ __device__ void spread_one_global(const Geom& g, double inv2s2, double pref, dou
       wy = wy * ((CT)pref1 / sy);
       wzv = wzv * ((CT)pref1 / sz);
     }
-    const double v = ((qps * (double)wx) * (double)wy) * (double)wzv;
+    const double qxy = (qps * (double)wx) * (double)wy;
     const int gx = (wx0 + tx) % g.X, gy = (wy0 + ty) % g.Y, gz = (wz0 + tz) % g.Z;
     atomicAdd(&grid64[gx + (size_t)g.X * (gy + (size_t)g.Y * gz)],
-              (unsigned long long)(fixed_round(v) << shift));
+              (unsigned long long)(fixed_round_prod(qxy, (double)wzv) << shift));
   }
 }
 
@@ -257,7 +257,7 @@ __device__ __forceinline__ void spread_overflow(const SpreadArgs& a, int eg,
 // into brick_out, with its shift to the global scale in shift_out[bin], for
 // k_brick_reduce.  Ends with a barrier, so a workgroup may call it for
 // several bins in a row.
-template <typename CT, int NZC, int BS, typename ACC>
+template <typename CT, int NZC, int BS, typename ACC, int PL = 0>
 __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt, int cnt_all,
                                            int eg, unsigned long long qbits,
                                            const V4<CT>* __restrict__ sorted,
@@ -405,12 +405,14 @@ __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt
           const double qxy = (qps * w[k_tx[k]]) * w[k_ty[k]];
           const int a0 = base + k_off[k];
           if (NZC > 0) {
+            // PL > 0: the plane stride is a compile-time LDS offset
+            const int pl = PL > 0 ? PL : LxLy;
 #pragma unroll
             for (int t = 0; t < (NZC > 0 ? NZC : 1); t++)
-              atomicAdd(&brick[a0 + t * LxLy], (ACC)fixed_round(qxy * wz[t]));
+              atomicAdd(&brick[a0 + t * pl], (ACC)fixed_round_prod(qxy, wz[t]));
           } else {
             for (int t = 0; t < NZ; t++)
-              atomicAdd(&brick[a0 + t * LxLy], (ACC)fixed_round(qxy * w[NX + NY + t]));
+              atomicAdd(&brick[a0 + t * LxLy], (ACC)fixed_round_prod(qxy, w[NX + NY + t]));
           }
         }
       }
@@ -466,7 +468,7 @@ __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt
 
 // 512-thread workgroups: three per CU (6 waves per SIMD, at most 80 VGPRs),
 // so one workgroup's staging overlaps two others' work
-template <typename CT, int NZC, int BS, typename ACC>
+template <typename CT, int NZC, int BS, typename ACC, int PL = 0>
 __global__ void __launch_bounds__(BS, BS == 512 ? 6 : 1)
     k_spread(SpreadArgs a, const V4<CT>* __restrict__ xyzq, const V4<CT>* __restrict__ sorted,
              const int* __restrict__ counts, int* __restrict__ counts_other,
@@ -498,7 +500,7 @@ __global__ void __launch_bounds__(BS, BS == 512 ? 6 : 1)
   }
   if (cnt == 0) return;
   const size_t vol = (size_t)a.b.Lx * a.b.Ly * a.b.Lz;
-  spread_bin<CT, NZC, BS, ACC>(a, bin, cnt, call, eg, st.qmax_bits,
+  spread_bin<CT, NZC, BS, ACC, PL>(a, bin, cnt, call, eg, st.qmax_bits,
                                sorted + (size_t)bin * a.cap_bin, loc + (size_t)bin * a.cap_bin,
                                grid64, spread_smem,
                                bricks ? bricks + (size_t)bin * vol : nullptr, bshift);
@@ -1347,7 +1349,14 @@ static hipError_t spread_t(const Geom& g, const BinShape& b, double sigma, bool 
 #define GSE_SPREAD_NZ(ACCT)                           \
   switch (NZ) {                                       \
     case 11: GSE_SPREAD_BS(11, ACCT) break;           \
-    case 13: GSE_SPREAD_BS(13, ACCT) break;           \
+    case 13:                                          \
+      if (bs == 512 && b.Lx * b.Ly == 400) {          \
+        k_spread<CT, 13, 512, ACCT, 400><<<nwg, 512, lds, s>>>(                             \
+            a, xq, srt, w.counts, w.counts_other, w.stats, w.stats_other, w.ovf, grid64,     \
+            (ACCT*)w.bricks, w.bshift, w.loc);        \
+        break;                                        \
+      }                                               \
+      GSE_SPREAD_BS(13, ACCT) break;                  \
     case 15: GSE_SPREAD_BS(15, ACCT) break;           \
     case 17: GSE_SPREAD_BS(17, ACCT) break;           \
     default: GSE_SPREAD_BS(0, ACCT) break;            \
