@@ -109,6 +109,103 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// The same binning for grids of at most kBinAggMaxBins bins, with the slot
+// claims aggregated per workgroup: a workgroup of 1024 threads takes
+// kBinAggPPT * 1024 consecutive particles, ranks them within their bins with
+// LDS atomics, then claims each touched bin's slots with one global atomic
+// (the particle's slot is that base plus its rank).  A random particle order
+// touches each bin a few times per workgroup, so this issues several times
+// fewer global atomics than one per particle.
+constexpr int kBinAggPPT = 4, kBinAggMaxBins = 16384;
+template <typename CT>
+__global__ void __launch_bounds__(1024)
+    k_bin_agg(Geom g, BinShape b, int n, int cap_bin, const V4<CT>* __restrict__ xyzq,
+              int* __restrict__ counts, BinStats* __restrict__ stats, int* __restrict__ ovf,
+              int* __restrict__ perm, int* __restrict__ loc, V4<CT>* __restrict__ sorted) {
+  __shared__ int hist[kBinAggMaxBins];
+  __shared__ double wmax[16];
+  __shared__ int winv[16];
+  for (int j = threadIdx.x; j < b.nbins; j += 1024) hist[j] = 0;
+  __syncthreads();
+  const int i0 = blockIdx.x * (1024 * kBinAggPPT);
+  int bin_[kBinAggPPT], rank_[kBinAggPPT], loc_[kBinAggPPT];
+  V4<CT> p_[kBinAggPPT];
+  double aq = 0.0;
+  int ninv = 0;
+#pragma unroll
+  for (int u = 0; u < kBinAggPPT; u++) {
+    const int i = i0 + u * 1024 + (int)threadIdx.x;
+    bin_[u] = -1;
+    rank_[u] = 0;
+    loc_[u] = 0;
+    p_[u] = V4<CT>{0, 0, 0, 0};
+    if (i < n) {
+      const V4<CT> p = xyzq[i];
+      p_[u] = p;
+      // x*inv_h in FP64 as CpuGaussCharge.cpp:311-313, as k_bin
+      const double fx = (double)p.x * g.ihx;
+      const double fy = (double)p.y * g.ihy;
+      const double fz = (double)p.z * g.ihz;
+      const double lim = 1.0e9;
+      const bool ok = fabs(fx) < lim && fabs(fy) < lim && fabs(fz) < lim && isfinite((double)p.w);
+      if (ok) {
+        const int ox = wrapi((int)round(fx) - g.nx, g.X);
+        const int oy = wrapi((int)round(fy) - g.ny, g.Y);
+        const int oz = wrapi((int)round(fz) - g.nz, g.Z);
+        const int qx = ox / b.Bx, qy = oy / b.By, qz = oz / b.Bz;
+        bin_[u] = qx + b.nbx * (qy + b.nby * qz);
+        loc_[u] = (ox - qx * b.Bx) | ((oy - qy * b.By) << 8) | ((oz - qz * b.Bz) << 16);
+        rank_[u] = atomicAdd(&hist[bin_[u]], 1);
+        aq = fmax(aq, fabs((double)p.w));
+      } else {
+        ninv++;
+      }
+    }
+  }
+  __syncthreads();
+  // one global claim per touched bin; hist[] becomes the bin's base slot
+  for (int j = threadIdx.x; j < b.nbins; j += 1024) {
+    const int c = hist[j];
+    if (c > 0) hist[j] = atomicAdd(&counts[j], c);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kBinAggPPT; u++) {
+    if (bin_[u] < 0) continue;
+    const int i = i0 + u * 1024 + (int)threadIdx.x;
+    const int slot = hist[bin_[u]] + rank_[u];
+    if (slot < cap_bin) {
+      const size_t pos = (size_t)bin_[u] * cap_bin + slot;
+      sorted[pos] = p_[u];
+      perm[pos] = i;
+      loc[pos] = loc_[u];
+    } else {
+      ovf[atomicAdd(&stats->overflow, 1)] = i;
+    }
+  }
+  // one max and one invalid count per workgroup reach memory
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    aq = fmax(aq, __shfl_xor(aq, m, kWave));
+    ninv += __shfl_xor(ninv, m, kWave);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    wmax[threadIdx.x >> 6] = aq;
+    winv[threadIdx.x >> 6] = ninv;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    int inv = 0;
+    for (int w = 0; w < 16; w++) {
+      m = fmax(m, wmax[w]);
+      inv += winv[w];
+    }
+    if (m > 0.0) atomicMax(&stats->qmax_bits, (unsigned long long)__double_as_longlong(m));
+    if (inv > 0) atomicAdd(&stats->invalid, inv);
+  }
+}
+
 struct LaneRole {
   int axis;  // 0,1,2 or -1
   int j;
@@ -1242,6 +1339,18 @@ hipError_t launch_bin_clear(int* counts, int nbins, BinStats* stats, hipStream_t
 }
 
 hipError_t launch_bin(bool dbl, const Geom& g, const BinShape& b, Binning& w, hipStream_t s) {
+  if (w.n > 0 && b.nbins <= kBinAggMaxBins && tuning().bin_agg) {
+    const unsigned grid = (unsigned)((w.n + 1024 * kBinAggPPT - 1) / (1024 * kBinAggPPT));
+    if (dbl)
+      k_bin_agg<double><<<grid, 1024, 0, s>>>(g, b, w.n, w.cap_bin, (const V4<double>*)w.xyzq,
+                                              w.counts, w.stats, w.ovf, w.perm, w.loc,
+                                              (V4<double>*)w.sorted);
+    else
+      k_bin_agg<float><<<grid, 1024, 0, s>>>(g, b, w.n, w.cap_bin, (const V4<float>*)w.xyzq,
+                                             w.counts, w.stats, w.ovf, w.perm, w.loc,
+                                             (V4<float>*)w.sorted);
+    return hipGetLastError();
+  }
   if (w.n > 0) {
     const unsigned grid = (unsigned)((w.n + 255) / 256);
     if (dbl)

@@ -68,7 +68,9 @@ GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
  *   spread_cols   column-owner register spread (0/1, 0); 0 = LDS-atomic brick
  *   spread_flush  1: each bin stores its brick and one pass sums the bricks
  *                 covering each grid point; 0: bricks added with atomics (0/1, 1)
- *   reduce_tile   that sum by one workgroup per 8^3 bin where the grid allows (0/1, 1) */
+ *   reduce_tile   that sum by one workgroup per 8^3 bin where the grid allows (0/1, 1)
+ *   bin_agg       binning claims bin slots once per workgroup and bin
+ *                 (grids of at most 16384 bins) (0/1, 1) */
 GSE_API gse_status_t gse_set_tuning_param(const char* name, int value);
 GSE_API gse_status_t gse_get_tuning_param(const char* name, int* value);
 
