@@ -639,7 +639,8 @@ gse_status_t gse_set_tuning_param(const char* name, int value) {
   else if (n == "spread_cols" && (value == 0 || value == 1)) t.spread_cols = value;
   else if (n == "spread_flush" && (value == 0 || value == 1)) t.spread_flush = value;
   else if (n == "reduce_tile" && (value == 0 || value == 1)) t.reduce_tile = value;
-  else if (n == "bin_agg" && (value == 0 || value == 1)) t.bin_agg = value;
+  else if (n == "bin_agg" && (value == 0 || value == 1 || value == 2 || value == 4 || value == 8))
+    t.bin_agg = value;
   else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter or value out of range: %s=%d",
                    name, value);
   return GSE_SUCCESS;

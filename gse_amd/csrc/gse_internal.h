@@ -46,7 +46,7 @@ struct Tuning {
   int spread_cols = 0;         // column-owner spread in registers (else LDS-atomic brick)
   int spread_flush = 1;        // 1: bricks stored and summed by k_brick_reduce; 0: atomics
   int reduce_tile = 1;         // tiled brick reduce where the grid allows it
-  int bin_agg = 1;             // binning with slot claims aggregated per workgroup
+  int bin_agg = 4;             // particles per workgroup / 256 of the aggregated binning (0: off)
 };
 Tuning& tuning();
 

@@ -110,31 +110,32 @@ __global__ void __launch_bounds__(256)
 }
 
 // The same binning for grids of at most kBinAggMaxBins bins, with the slot
-// claims aggregated per workgroup: a workgroup of 1024 threads takes
-// kBinAggPPT * 1024 consecutive particles, ranks them within their bins with
+// claims aggregated per workgroup: a workgroup takes BT * kBinAggPPT
+// consecutive particles, ranks them within their bins with
 // LDS atomics, then claims each touched bin's slots with one global atomic
-// (the particle's slot is that base plus its rank).  A random particle order
+// (the particle's slot is that base plus its rank).  (BT threads, kBinAggPPT
+// particles each.)  A random particle order
 // touches each bin a few times per workgroup, so this issues several times
 // fewer global atomics than one per particle.
-constexpr int kBinAggPPT = 4, kBinAggMaxBins = 16384;
-template <typename CT>
-__global__ void __launch_bounds__(1024)
+constexpr int kBinAggMaxBins = 16384;
+template <typename CT, int BT, int kBinAggPPT>
+__global__ void __launch_bounds__(BT)
     k_bin_agg(Geom g, BinShape b, int n, int cap_bin, const V4<CT>* __restrict__ xyzq,
               int* __restrict__ counts, BinStats* __restrict__ stats, int* __restrict__ ovf,
               int* __restrict__ perm, int* __restrict__ loc, V4<CT>* __restrict__ sorted) {
   __shared__ int hist[kBinAggMaxBins];
-  __shared__ double wmax[16];
-  __shared__ int winv[16];
-  for (int j = threadIdx.x; j < b.nbins; j += 1024) hist[j] = 0;
+  __shared__ double wmax[BT / 64];
+  __shared__ int winv[BT / 64];
+  for (int j = threadIdx.x; j < b.nbins; j += BT) hist[j] = 0;
   __syncthreads();
-  const int i0 = blockIdx.x * (1024 * kBinAggPPT);
+  const int i0 = blockIdx.x * (BT * kBinAggPPT);
   int bin_[kBinAggPPT], rank_[kBinAggPPT], loc_[kBinAggPPT];
   V4<CT> p_[kBinAggPPT];
   double aq = 0.0;
   int ninv = 0;
 #pragma unroll
   for (int u = 0; u < kBinAggPPT; u++) {
-    const int i = i0 + u * 1024 + (int)threadIdx.x;
+    const int i = i0 + u * BT + (int)threadIdx.x;
     bin_[u] = -1;
     rank_[u] = 0;
     loc_[u] = 0;
@@ -164,7 +165,7 @@ __global__ void __launch_bounds__(1024)
   }
   __syncthreads();
   // one global claim per touched bin; hist[] becomes the bin's base slot
-  for (int j = threadIdx.x; j < b.nbins; j += 1024) {
+  for (int j = threadIdx.x; j < b.nbins; j += BT) {
     const int c = hist[j];
     if (c > 0) hist[j] = atomicAdd(&counts[j], c);
   }
@@ -172,7 +173,7 @@ __global__ void __launch_bounds__(1024)
 #pragma unroll
   for (int u = 0; u < kBinAggPPT; u++) {
     if (bin_[u] < 0) continue;
-    const int i = i0 + u * 1024 + (int)threadIdx.x;
+    const int i = i0 + u * BT + (int)threadIdx.x;
     const int slot = hist[bin_[u]] + rank_[u];
     if (slot < cap_bin) {
       const size_t pos = (size_t)bin_[u] * cap_bin + slot;
@@ -197,7 +198,7 @@ __global__ void __launch_bounds__(1024)
   if (threadIdx.x == 0) {
     double m = 0.0;
     int inv = 0;
-    for (int w = 0; w < 16; w++) {
+    for (int w = 0; w < BT / 64; w++) {
       m = fmax(m, wmax[w]);
       inv += winv[w];
     }
@@ -1339,16 +1340,30 @@ hipError_t launch_bin_clear(int* counts, int nbins, BinStats* stats, hipStream_t
 }
 
 hipError_t launch_bin(bool dbl, const Geom& g, const BinShape& b, Binning& w, hipStream_t s) {
-  if (w.n > 0 && b.nbins <= kBinAggMaxBins && tuning().bin_agg) {
-    const unsigned grid = (unsigned)((w.n + 1024 * kBinAggPPT - 1) / (1024 * kBinAggPPT));
-    if (dbl)
-      k_bin_agg<double><<<grid, 1024, 0, s>>>(g, b, w.n, w.cap_bin, (const V4<double>*)w.xyzq,
-                                              w.counts, w.stats, w.ovf, w.perm, w.loc,
-                                              (V4<double>*)w.sorted);
-    else
-      k_bin_agg<float><<<grid, 1024, 0, s>>>(g, b, w.n, w.cap_bin, (const V4<float>*)w.xyzq,
-                                             w.counts, w.stats, w.ovf, w.perm, w.loc,
-                                             (V4<float>*)w.sorted);
+  // bin_agg: particles per workgroup / 256
+  const int agg = tuning().bin_agg;
+  if (w.n > 0 && b.nbins <= kBinAggMaxBins && agg > 0) {
+    const int per_wg = 256 * agg;
+    const unsigned grid = (unsigned)((w.n + per_wg - 1) / per_wg);
+#define GSE_BIN_AGG(TT, BT, P)                                                                   \
+  k_bin_agg<TT, BT, P><<<grid, BT, 0, s>>>(g, b, w.n, w.cap_bin, (const V4<TT>*)w.xyzq, w.counts, \
+                                           w.stats, w.ovf, w.perm, w.loc, (V4<TT>*)w.sorted)
+#define GSE_BIN_AGG_T(TT)                  \
+  if (agg == 1)                            \
+    GSE_BIN_AGG(TT, 256, 1);               \
+  else if (agg == 2)                       \
+    GSE_BIN_AGG(TT, 512, 1);               \
+  else if (agg == 4)                       \
+    GSE_BIN_AGG(TT, 1024, 1);              \
+  else                                     \
+    GSE_BIN_AGG(TT, 1024, 2);
+    if (dbl) {
+      GSE_BIN_AGG_T(double)
+    } else {
+      GSE_BIN_AGG_T(float)
+    }
+#undef GSE_BIN_AGG_T
+#undef GSE_BIN_AGG
     return hipGetLastError();
   }
   if (w.n > 0) {

@@ -69,8 +69,9 @@ GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
  *   spread_flush  1: each bin stores its brick and one pass sums the bricks
  *                 covering each grid point; 0: bricks added with atomics (0/1, 1)
  *   reduce_tile   that sum by one workgroup per 8^3 bin where the grid allows (0/1, 1)
- *   bin_agg       binning claims bin slots once per workgroup and bin
- *                 (grids of at most 16384 bins) (0/1, 1) */
+ *   bin_agg       binning claims bin slots once per workgroup and bin, with
+ *                 256 x this many particles per workgroup (grids of at most
+ *                 16384 bins); 0 = one claim per particle (0,1,2,4,8; 4) */
 GSE_API gse_status_t gse_set_tuning_param(const char* name, int value);
 GSE_API gse_status_t gse_get_tuning_param(const char* name, int* value);
 
