@@ -484,6 +484,7 @@ struct BatchGroup {  // systems sharing 2nz+1: one spread and one gather launch
   size_t lds_spread = 0, lds_gather = 0;
   int grid = 0;
   bool tile4 = true;  // every brick row is whole 16-byte pieces (32-bit bricks)
+  bool pair = true;   // every system takes the two-charges-per-wave gather
 };
 
 struct BatchFFT {  // systems with equal grid dimensions, adjacent in the layout
@@ -1360,6 +1361,7 @@ gse_status_t batch_layout(gse_batch_s* h, hipStream_t st) {
       gr.lds_spread = std::max(gr.lds_spread, spread_lds_bytes(dbl, S.g, S.b, nullptr));
       gr.tile4 = gr.tile4 && acc32 && S.b.Lx % 4 == 0;
       gr.lds_gather = std::max(gr.lds_gather, gather_lds_bytes(dbl, S.g, S.b, nullptr));
+      gr.pair = gr.pair && gather_pair_ok(dbl, h->at_dbl, S.g, S.b);
     }
     gr.nitems = (int)h->items.size() - gr.item0;
     const int want = std::min(gr.nitems, 2048);
@@ -1490,7 +1492,7 @@ gse_status_t batch_run(gse_batch_s* h, hipStream_t st, const void* xyzq, void* f
     for (const BatchGroup& g : h->groups)
       HIPCHK(launch_batch_gather(dbl, h->at_dbl, g.nz, batch_block(h->gather_block, dbl),
                                  g.lds_gather, g.grid, d, items + g.item0, g.nitems, h->phi_b.p,
-                                 o, st));
+                                 o, g.pair, st));
     HIPCHK(launch_batch_gather_overflow(dbl, h->at_dbl, d, xyzq, h->phi_b.p, o, st));
   }
   if (energies)

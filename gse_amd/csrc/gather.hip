@@ -647,7 +647,7 @@ __device__ __forceinline__ GatherArgs batch_gather_args(const BatchSys& S) {
   return a;
 }
 
-template <typename CT, typename AT, int NZC, int FL, int BS>
+template <typename CT, typename AT, int NZC, int FL, int BS, int PL = 0>
 __global__ void __launch_bounds__(BS)
     k_batch_gather(BatchDev d, const BatchSys* __restrict__ sysp, const int2* __restrict__ items,
                    int nitems, const CT* __restrict__ phi, GatherOut o) {
@@ -663,7 +663,7 @@ __global__ void __launch_bounds__(BS)
     const int cnt = min(d.counts[S.bofs + bin], S.cap_bin);
     if (cnt == 0) continue;
     const long long s0 = S.slot0 + (long long)bin * S.cap_bin;
-    gather_bin<CT, AT, NZC, FL, BS>(batch_gather_args(S), bin, cnt,
+    gather_bin<CT, AT, NZC, FL, BS, PL>(batch_gather_args(S), bin, cnt,
                                     (const V4<CT>*)d.sorted + s0, d.perm + s0, d.loc + s0,
                                     phi + S.gofs, o, 1.0, smem_raw);
   }
@@ -686,10 +686,16 @@ __global__ void __launch_bounds__(256)
 template <typename CT, typename AT, int FL>
 static hipError_t batch_gather_t(int nz, int bs, size_t lds, int grid, const BatchDev& d,
                                  const int2* items, int nitems, const void* phi,
-                                 const GatherOut& o, hipStream_t s) {
+                                 const GatherOut& o, bool pair, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
   const unsigned g = (unsigned)grid;
   const CT* ph = (const CT*)phi;
+  if constexpr (sizeof(CT) == 4 && sizeof(AT) == 4 && !(FL & GF_VIRIAL)) {
+    if (pair && nz == 13 && bs == 512) {  // every system: 20 x 20 brick planes, cubic support
+      k_batch_gather<CT, AT, 13, FL, 512, 400><<<g, 512, lds, s>>>(d, d.sys, items, nitems, ph, o);
+      return hipGetLastError();
+    }
+  }
 #define GSE_BG_LAUNCH(NZV, BSV) \
   k_batch_gather<CT, AT, NZV, FL, BSV><<<g, BSV, lds, s>>>(d, d.sys, items, nitems, ph, o)
 #define GSE_BG_BS(NZV)       \
@@ -711,22 +717,29 @@ static hipError_t batch_gather_t(int nz, int bs, size_t lds, int grid, const Bat
 template <typename CT, typename AT>
 static hipError_t batch_gather_at(int nz, int bs, size_t lds, int grid, const BatchDev& d,
                                   const int2* items, int nitems, const void* phi,
-                                  const GatherOut& o, hipStream_t s) {
+                                  const GatherOut& o, bool pair, hipStream_t s) {
   if (o.pot)
     return batch_gather_t<CT, AT, GF_SPHERE | GF_POT>(nz, bs, lds, grid, d, items, nitems, phi,
-                                                      o, s);
-  return batch_gather_t<CT, AT, GF_SPHERE>(nz, bs, lds, grid, d, items, nitems, phi, o, s);
+                                                      o, pair, s);
+  return batch_gather_t<CT, AT, GF_SPHERE>(nz, bs, lds, grid, d, items, nitems, phi, o, pair, s);
 }
 
 hipError_t launch_batch_gather(bool ct_dbl, bool at_dbl, int nz, int block, size_t lds, int grid,
                                const BatchDev& d, const int2* items, int nitems, const void* phi,
-                               const GatherOut& o, hipStream_t s) {
+                               const GatherOut& o, bool pair, hipStream_t s) {
   if (lds > 150 * 1024) return hipErrorInvalidConfiguration;
   if (ct_dbl)
-    return batch_gather_at<double, double>(nz, block, lds, grid, d, items, nitems, phi, o, s);
+    return batch_gather_at<double, double>(nz, block, lds, grid, d, items, nitems, phi, o, false,
+                                           s);
   if (at_dbl)
-    return batch_gather_at<float, double>(nz, block, lds, grid, d, items, nitems, phi, o, s);
-  return batch_gather_at<float, float>(nz, block, lds, grid, d, items, nitems, phi, o, s);
+    return batch_gather_at<float, double>(nz, block, lds, grid, d, items, nitems, phi, o, false,
+                                          s);
+  return batch_gather_at<float, float>(nz, block, lds, grid, d, items, nitems, phi, o, pair, s);
+}
+
+bool gather_pair_ok(bool ct_dbl, bool at_dbl, const Geom& g, const BinShape& b) {
+  return !ct_dbl && !at_dbl && g.nz == 6 && g.nx == 6 && g.ny == 6 &&
+         brick_pitch(4, g.X, b.Bx, b.Lx) * b.Ly == 400;
 }
 
 template <typename CT, typename AT>
