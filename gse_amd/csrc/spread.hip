@@ -1177,6 +1177,95 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// The batched binning with slot claims aggregated per workgroup (as
+// k_bin_agg): 1024 consecutive particles span few systems, so the bins they
+// touch lie in a short range of the concatenated bin numbering; each touched
+// bin is claimed with one global atomic.  A workgroup spanning 32 systems or
+// more, or a bin range over kBatchAggBins, claims one slot per particle.
+constexpr int kBatchAggBins = 8192;
+template <typename CT>
+__global__ void __launch_bounds__(1024) k_batch_bin_agg(BatchDev d, const V4<CT>* __restrict__ xyzq) {
+  __shared__ int hist[kBatchAggBins];
+  const int i0 = blockIdx.x * 1024;
+  const int i = i0 + (int)threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  // systems sa..sb hold this workgroup's particles; their bins (numbered in
+  // the internal system order) lie in [base, base + range)
+  const int sa = batch_system(d.pofs, d.nsys, i0);
+  const int sb = batch_system(d.pofs, d.nsys, min(d.ntot, i0 + 1024) - 1);
+  int base = 0, range = kBatchAggBins + 1;
+  if (sb - sa < 32) {
+    int lo = d.sys[sa].bofs, hi = lo + d.sys[sa].b.nbins;
+    for (int t = sa + 1; t <= sb; t++) {
+      lo = min(lo, d.sys[t].bofs);
+      hi = max(hi, d.sys[t].bofs + d.sys[t].b.nbins);
+    }
+    base = lo;
+    range = hi - lo;
+  }
+  const bool agg = range <= kBatchAggBins;
+  if (agg)
+    for (int j = threadIdx.x; j < range; j += 1024) hist[j] = 0;
+  __syncthreads();
+  int s = -1, gb = -1, rank = 0, lc = 0;
+  double aq = 0.0;
+  V4<CT> p{0, 0, 0, 0};
+  if (i < d.ntot) {
+    s = batch_system(d.pofs, d.nsys, i);
+    const BatchSys& S = d.sys[s];
+    const Geom& g = S.g;
+    p = xyzq[i];
+    const double fx = (double)p.x * g.ihx;
+    const double fy = (double)p.y * g.ihy;
+    const double fz = (double)p.z * g.ihz;
+    const double lim = 1.0e9;
+    const bool ok = fabs(fx) < lim && fabs(fy) < lim && fabs(fz) < lim && isfinite((double)p.w);
+    if (ok) {
+      const int ox = wrapi((int)round(fx) - g.nx, g.X);
+      const int oy = wrapi((int)round(fy) - g.ny, g.Y);
+      const int oz = wrapi((int)round(fz) - g.nz, g.Z);
+      const BinShape& b = S.b;
+      const int qx = ox / b.Bx, qy = oy / b.By, qz = oz / b.Bz;
+      gb = S.bofs + qx + b.nbx * (qy + b.nby * qz);
+      lc = (ox - qx * b.Bx) | ((oy - qy * b.By) << 8) | ((oz - qz * b.Bz) << 16);
+      if (agg) rank = atomicAdd(&hist[gb - base], 1);
+      aq = fabs((double)p.w);
+    } else {
+      atomicAdd(&d.stats[s].invalid, 1);
+    }
+  }
+  __syncthreads();
+  if (agg)
+    for (int j = threadIdx.x; j < range; j += 1024) {
+      const int c = hist[j];
+      if (c > 0) hist[j] = atomicAdd(&d.counts[base + j], c);
+    }
+  __syncthreads();
+  if (gb >= 0) {
+    const BatchSys& S = d.sys[s];
+    const int slot = agg ? hist[gb - base] + rank : atomicAdd(&d.counts[gb], 1);
+    if (slot < S.cap_bin) {
+      const long long pos = S.slot0 + (long long)(gb - S.bofs) * S.cap_bin + slot;
+      ((V4<CT>*)d.sorted)[pos] = p;
+      d.perm[pos] = i;
+      d.loc[pos] = lc;
+    } else {
+      d.ovf[atomicAdd(d.novf, 1)] = i;
+    }
+  }
+  // max |q| per system: one atomic per wave when the wave holds one system
+  const int s0 = __builtin_amdgcn_readfirstlane(s);
+  const bool uniform = __all(s == s0);
+  if (uniform) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) aq = fmax(aq, __shfl_xor(aq, m, kWave));
+    if (lane == 0 && s0 >= 0 && aq > 0.0)
+      atomicMax(&d.stats[s0].qmax_bits, (unsigned long long)__double_as_longlong(aq));
+  } else if (s >= 0 && aq > 0.0) {
+    atomicMax(&d.stats[s].qmax_bits, (unsigned long long)__double_as_longlong(aq));
+  }
+}
+
 // sysp and items are read-only for the whole launch (separate restrict
 // arguments, so their uniform loads go to scalar registers)
 template <typename CT, int NZC, int BS, typename ACC>
@@ -1633,6 +1722,14 @@ hipError_t launch_batch_clear(const BatchDev& d, hipStream_t s) {
 
 hipError_t launch_batch_bin(bool dbl, const BatchDev& d, const void* xyzq, hipStream_t s) {
   if (d.ntot <= 0) return hipSuccess;
+  if (tuning().bin_agg) {
+    const unsigned grid = (unsigned)((d.ntot + 1023) / 1024);
+    if (dbl)
+      k_batch_bin_agg<double><<<grid, 1024, 0, s>>>(d, (const V4<double>*)xyzq);
+    else
+      k_batch_bin_agg<float><<<grid, 1024, 0, s>>>(d, (const V4<float>*)xyzq);
+    return hipGetLastError();
+  }
   const unsigned grid = (unsigned)((d.ntot + 255) / 256);
   if (dbl)
     k_batch_bin<double><<<grid, 256, 0, s>>>(d, (const V4<double>*)xyzq);
