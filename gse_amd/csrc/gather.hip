@@ -316,8 +316,8 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
   // right side kept per z (one compare per point).  FP64 keeps the
   // reference's rounding, (dx^2+dy^2)+dz^2 < rc^2 (CpuGaussCharge.cpp:157).
   constexpr bool FAST_SPH = SPH && !VIR && sizeof(CT) == 4;
-  // FP32 accumulation without the virial: the z walk as FMAs (see below)
-  constexpr bool FMA_FORM = NZC > 0 && !VIR && sizeof(CT) == 4 && sizeof(AT) == 4;
+  // FP32 accumulation: the z walk as FMAs (see below)
+  constexpr bool FMA_FORM = NZC > 0 && sizeof(CT) == 4 && sizeof(AT) == 4;
   const int PLv = PL > 0 ? PL : PxLy;
   // PL > 0 also means a cubic support of NZC points per axis.  The FMA form
   // then walks two charges per wave, one per 32-lane half: the per-charge
@@ -378,13 +378,16 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
         const int aic = live ? ai : 2 * g2;
         const CT* w = sW + (size_t)aic * NW * 2;
         const int base = sB[aic];
-        CT wzr[NZA], dz2[NZA], vzr[NZA];
+        // dz2: rc^2 - dz^2 (no virial: one compare per point), else dz^2
+        // (the virial weight, and the exact sphere test); w2r = wz dz^2
+        CT wzr[NZA], dz2[NZA], vzr[NZA], w2r[VIR ? NZA : 1];
 #pragma unroll
         for (int t = 0; t < NZA; t++) {
           wzr[t] = w[2 * (2 * NZA + t)];
           const CT dz = w[2 * (2 * NZA + t) + 1];
-          dz2[t] = rc2 - mul_rn(dz, dz);
+          dz2[t] = VIR ? mul_rn(dz, dz) : rc2 - mul_rn(dz, dz);
           vzr[t] = wzr[t] * dz;
+          if constexpr (VIR) w2r[t] = wzr[t] * dz2[t];
         }
         AT f[13];
 #pragma unroll
@@ -398,14 +401,16 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
           const CT dy = w[2 * (NZA + p_ty[k]) + 1];
           const CT dxy2 = add_rn(mul_rn(dx, dx), mul_rn(dy, dy));
           const int a0 = base + p_tx[k] + Px * p_ty[k];
-          AT B0 = 0, B1 = 0, A0 = 0, A1 = 0;
+          AT B0 = 0, B1 = 0, B2 = 0, A0 = 0, A1 = 0;
 #pragma unroll
           for (int t = 0; t < NZA; t++) {
             const CT ph = brick[a0 + t * PL];
             if (NEED_B0) B0 = __builtin_fmaf(wzr[t], ph, B0);
             if (NEED_B1) B1 = __builtin_fmaf(vzr[t], ph, B1);
+            if constexpr (VIR) B2 = __builtin_fmaf(w2r[t], ph, B2);
             if (SPH) {
-              const CT ms = dxy2 < dz2[t] ? ph : (CT)0;
+              const bool in = VIR ? add_rn(dxy2, dz2[t]) < rc2 : dxy2 < dz2[t];
+              const CT ms = in ? ph : (CT)0;
               A0 = __builtin_fmaf(wzr[t], ms, A0);
               A1 = __builtin_fmaf(vzr[t], ms, A1);
             }
@@ -425,11 +430,20 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
               f[5] = fmac(wxy, B1, f[5]);
             }
             if (POT) f[6] += t0;
+            if (VIR) {
+              f[7] = fmac(t0 * dx, dx, f[7]);
+              f[8] = fmac(t0 * dx, dy, f[8]);
+              f[9] = fmac(wxy * dx, B1, f[9]);
+              f[10] = fmac(t0 * dy, dy, f[10]);
+              f[11] = fmac(wxy * dy, B1, f[11]);
+              f[12] = fmac(wxy, B2, f[12]);
+            }
           }
         }
 #pragma unroll
-        for (int c = 0; c < 7; c++) {
-          const bool used = (c < 3 && SPH) || (c >= 3 && c < 6 && CUB) || (c == 6 && POT);
+        for (int c = 0; c < 13; c++) {
+          const bool used = (c < 3 && SPH) || (c >= 3 && c < 6 && CUB) || (c == 6 && POT) ||
+                            (c >= 7 && VIR);
           if (used) f[c] = half_total(live ? f[c] : (AT)0);
         }
         if (live && l32 == 0) gather_store<AT, FL>(a, o, sP[ai], sQ[ai], ds, f);
@@ -466,7 +480,7 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
         const CT dxy2 = add_rn(mul_rn(dx, dx), mul_rn(dy, dy));
         const int a0 = base + k_off[k];
         AT B0 = 0, B1 = 0, B2 = 0, A0 = 0, A1 = 0;
-        if (FMA_FORM) {
+        if (FMA_FORM && !VIR) {
           // each potential value enters every sum through one FMA against a
           // per-particle z factor (wz, wz*dz), the sphere mask selects it
 #pragma unroll
@@ -554,7 +568,10 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
 
 
 template <typename CT, typename AT, int NZC, int FL, int BS, int PL = 0>
-__global__ void __launch_bounds__(BS)
+// FP32 with supports up to 13: at most 128 VGPRs, so two 512-thread
+// workgroups share a CU
+__global__ void __launch_bounds__(
+    BS, (BS == 512 && sizeof(CT) == 4 && sizeof(AT) == 4 && NZC <= 13) ? 4 : 1)
     k_gather(GatherArgs a, const V4<CT>* __restrict__ xyzq, const V4<CT>* __restrict__ sorted,
              const int* __restrict__ perm, const int* __restrict__ counts,
              int* __restrict__ counts_other, const BinStats* __restrict__ stats,
@@ -809,10 +826,9 @@ static hipError_t gather_nz(GatherArgs a, const Binning& w, const void* phi, con
   switch (2 * a.g.nz + 1) {
     case 11: GSE_G_BS(11) break;
     case 13:
-      // 8^3 bins with n = 6 on every axis: 20 x 20 planes of the brick (FP32,
-      // no virial: the virial variant keeps a runtime stride, its register
-      // budget has no room for the immediates)
-      if constexpr (sizeof(CT) == 4 && !(FL & GF_VIRIAL)) {
+      // 8^3 bins with n = 6 on every axis: 20 x 20 planes of the brick (FP32;
+      // not the full output set, whose sums do not fit the register budget)
+      if constexpr (sizeof(CT) == 4 && FL != (GF_SPHERE | GF_CUBE | GF_POT | GF_VIRIAL)) {
         if (bs == 512 && pxly == 400 && a.g.nx == a.g.nz && a.g.ny == a.g.nz) {
           k_gather<CT, AT, 13, FL, 512, 400><<<nwg, 512, lds, s>>>(GSE_G_ARGS);
           break;
