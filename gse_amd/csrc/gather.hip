@@ -708,8 +708,13 @@ static hipError_t batch_gather_t(int nz, int bs, size_t lds, int grid, const Bat
   const unsigned g = (unsigned)grid;
   const CT* ph = (const CT*)phi;
   if constexpr (sizeof(CT) == 4 && sizeof(AT) == 4 && !(FL & GF_VIRIAL)) {
-    if (pair && nz == 13 && bs == 512) {  // every system: 20 x 20 brick planes, cubic support
+    // every system of the group: cubic support, 8^3 bins, 4-aligned brick rows
+    if (pair && nz == 13 && bs == 512) {  // 20 x 20 brick planes
       k_batch_gather<CT, AT, 13, FL, 512, 400><<<g, 512, lds, s>>>(d, d.sys, items, nitems, ph, o);
+      return hipGetLastError();
+    }
+    if (pair && nz == 15 && bs == 512) {  // 24 x 22 brick planes
+      k_batch_gather<CT, AT, 15, FL, 512, 528><<<g, 512, lds, s>>>(d, d.sys, items, nitems, ph, o);
       return hipGetLastError();
     }
   }
@@ -755,8 +760,9 @@ hipError_t launch_batch_gather(bool ct_dbl, bool at_dbl, int nz, int block, size
 }
 
 bool gather_pair_ok(bool ct_dbl, bool at_dbl, const Geom& g, const BinShape& b) {
-  return !ct_dbl && !at_dbl && g.nz == 6 && g.nx == 6 && g.ny == 6 &&
-         brick_pitch(4, g.X, b.Bx, b.Lx) * b.Ly == 400;
+  if (ct_dbl || at_dbl || g.nx != g.nz || g.ny != g.nz) return false;
+  const int pl = brick_pitch(4, g.X, b.Bx, b.Lx) * b.Ly;
+  return (g.nz == 6 && pl == 400) || (g.nz == 7 && pl == 528);
 }
 
 template <typename CT, typename AT>

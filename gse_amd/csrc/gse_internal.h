@@ -261,8 +261,8 @@ hipError_t launch_batch_energy(const BatchDev& d, const double* partials, double
 hipError_t launch_batch_gather(bool ct_dbl, bool at_dbl, int nz, int block, size_t lds,
                                int grid, const BatchDev& d, const int2* items, int nitems,
                                const void* phi, const GatherOut& o, bool pair, hipStream_t s);
-// FP32 gather of a cubic 13^3 support over 20 x 20 brick planes: the
-// two-charges-per-wave walk applies
+// FP32 gather of a cubic 13^3 support over 20 x 20 brick planes, or 15^3
+// over 24 x 22: the two-charges-per-wave walk applies
 bool gather_pair_ok(bool ct_dbl, bool at_dbl, const Geom& g, const BinShape& b);
 hipError_t launch_batch_gather_overflow(bool ct_dbl, bool at_dbl, const BatchDev& d,
                                         const void* xyzq, const void* phi, const GatherOut& o,
