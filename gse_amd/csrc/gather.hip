@@ -317,13 +317,13 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
   // reference's rounding, (dx^2+dy^2)+dz^2 < rc^2 (CpuGaussCharge.cpp:157).
   constexpr bool FAST_SPH = SPH && !VIR && sizeof(CT) == 4;
   // FP32 accumulation: the z walk as FMAs (see below)
-  constexpr bool FMA_FORM = NZC > 0 && sizeof(CT) == 4 && sizeof(AT) == 4;
+  constexpr bool FMA_FORM = NZC > 0 && sizeof(CT) == sizeof(AT);
   const int PLv = PL > 0 ? PL : PxLy;
   // PL > 0 also means a cubic support of NZC points per axis.  The FMA form
   // then walks two charges per wave, one per 32-lane half: the per-charge
   // set-up and the cross-lane reduction are shared by two charges, and the
   // NZC^2 columns fill 32-lane slots instead of 64-lane ones.
-  constexpr bool PAIR = FMA_FORM && PL > 0;
+  constexpr bool PAIR = FMA_FORM && PL > 0 && sizeof(CT) == 4;
   constexpr int NXYP = NZA * NZA, KP = (NXYP + 31) / 32;
   int p_tx[KP], p_ty[KP];
 #pragma unroll
@@ -482,16 +482,18 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
         AT B0 = 0, B1 = 0, B2 = 0, A0 = 0, A1 = 0;
         if (FMA_FORM && !VIR) {
           // each potential value enters every sum through one FMA against a
-          // per-particle z factor (wz, wz*dz), the sphere mask selects it
+          // per-particle z factor (wz, wz*dz), the sphere mask selects it;
+          // FP64 keeps the reference's exact sphere test
 #pragma unroll
           for (int t = 0; t < NZA; t++) {
             const CT ph = brick[a0 + t * PLv];
-            if (NEED_B0) B0 = __builtin_fmaf(wzr[t], ph, B0);
-            if (NEED_B1) B1 = __builtin_fmaf(vzr[t], ph, B1);
+            if (NEED_B0) B0 = fmac((AT)wzr[t], (AT)ph, B0);
+            if (NEED_B1) B1 = fmac((AT)vzr[t], (AT)ph, B1);
             if (SPH) {
-              const CT ms = dxy2 < dz2[t] ? ph : (CT)0;
-              A0 = __builtin_fmaf(wzr[t], ms, A0);
-              A1 = __builtin_fmaf(vzr[t], ms, A1);
+              const bool in = FAST_SPH ? dxy2 < dz2[t] : add_rn(dxy2, dz2[t]) < rc2;
+              const AT ms = in ? (AT)ph : (AT)0;
+              A0 = fmac((AT)wzr[t], ms, A0);
+              A1 = fmac((AT)vzr[t], ms, A1);
             }
           }
         } else if (NZC > 0) {
@@ -837,6 +839,12 @@ static hipError_t gather_nz(GatherArgs a, const Binning& w, const void* phi, con
       if constexpr (sizeof(CT) == 4 && FL != (GF_SPHERE | GF_CUBE | GF_POT | GF_VIRIAL)) {
         if (bs == 512 && pxly == 400 && a.g.nx == a.g.nz && a.g.ny == a.g.nz) {
           k_gather<CT, AT, 13, FL, 512, 400><<<nwg, 512, lds, s>>>(GSE_G_ARGS);
+          break;
+        }
+      }
+      if constexpr (sizeof(CT) == 8 && sizeof(AT) == 8) {  // FP64: the z step as an immediate
+        if (bs == 256 && pxly == 400) {
+          k_gather<CT, AT, 13, FL, 256, 400><<<nwg, 256, lds, s>>>(GSE_G_ARGS);
           break;
         }
       }
