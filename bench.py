@@ -8,7 +8,8 @@ water-like triplets (48,000 charges) in L = 78.3 A, M = 80, sigma = 2.12.
 
 Prints ONE JSON line (rank 0, N = 1).  Extra figures (FP64 mode, adjoint
 cost, the separable convolution at M = 80 and M = 216, the mode with the
-on-grid convolution, batched systems/s) go into "extras".
+on-grid convolution, batched systems/s, the 1,000,002-charge M = 216 system)
+go into "extras".
 """
 import argparse
 import json
@@ -110,6 +111,21 @@ def bench_conv(api, torch, dev):
                         "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
                         "frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     return out
+
+
+def bench_large(api, systems, torch, dev):
+    """The 1,000,002-charge water box at M = 216 (FP32, same step as the
+    headline): forward steps/s and the adjoint sweep's cost relative to it
+    (the adjoint recomputes bin, spread and the Poisson solve)."""
+    xyzq_np, L, M, sigma = systems.make("water_1M_M216")
+    x = torch.from_numpy(xyzq_np).to(device=dev, dtype=torch.float32).contiguous()
+    gs = api.CpuGSEk(torch.float32, torch.float32, M, M, M, sigma, 0.4, 3.0, 3.0, L, L, L)
+    out = {}
+    sf = time_steps(lambda: gs.step(x, mode=0, force=True, potential=True, out=out), 20, 3, torch)
+    sa = time_steps(lambda: gs.adjoint(x), 20, 3, torch)
+    return {"workload": "water_1M_M216", "n_charges": len(xyzq_np), "grid": M,
+            "steps_per_s": round(1.0 / sf, 2), "ms_per_step": round(sf * 1e3, 4),
+            "adjoint_ms": round(sa * 1e3, 4), "adjoint_over_forward": round(sa / sf, 4)}
 
 
 def bench_batched(api, systems, torch, dev, nsys=256):
@@ -251,6 +267,7 @@ def main():
                         max(20, args.steps // 5), 5, torch)
         extras["mode_R_steps_per_s"] = round(1.0 / sm, 3)
         extras["batched"] = bench_batched(api, systems, torch, dev)
+        extras["large"] = bench_large(api, systems, torch, dev)
         line["extras"] = extras
 
     print(json.dumps(line), flush=True)
