@@ -56,6 +56,8 @@ def parse():
                     help="budget for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--conv-only", action="store_true",
+                    help="only the separable convolution timing (for counter passes)")
     return ap.parse_args()
 
 
@@ -172,6 +174,9 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.conv_only:
+        print(json.dumps({"conv_roofline": bench_conv(api, torch, dev)}), flush=True)
+        return
     ct = torch.float32 if args.precision == "fp32" else torch.float64
     xyzq_np, L, M, sigma = systems.make(args.workload)
     N = len(xyzq_np)
