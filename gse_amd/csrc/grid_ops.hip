@@ -470,14 +470,20 @@ __global__ void __launch_bounds__(256)
 // read straight from HBM; consecutive lanes own consecutive elements, so each
 // row of the window is one coalesced read per wave.  No LDS, no barrier.
 // The row index is wave-uniform: its periodic wrap is scalar work.
+// Workgroups are numbered so that the ntile row tiles of one column block
+// run on one XCD (workgroups go to the 8 XCDs round-robin), where the rows
+// that neighbouring tiles' windows share can come from that XCD's L2.
 template <typename T, int RC, bool SYM, int TJ>
 __global__ void __launch_bounds__(256)
-    k_conv_pencil(int C, int cw, long long cs, int N, long long S, const T* __restrict__ taps,
-                  const T* __restrict__ in, T* __restrict__ out) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
+    k_conv_pencil(int C, int cw, long long cs, int N, long long S, int ncb, int ntile,
+                  const T* __restrict__ taps, const T* __restrict__ in, T* __restrict__ out) {
+  const int r = (int)(blockIdx.x >> 3);
+  const int cb = (r / ntile) * 8 + (int)(blockIdx.x & 7);
+  if (cb >= ncb) return;
+  const int c = cb * 256 + threadIdx.x;
   if (c >= C) return;
   constexpr int W = TJ + 2 * RC, NT = 2 * RC + 1;
-  const int j0 = blockIdx.y * TJ;
+  const int j0 = (r % ntile) * TJ;
   const size_t base = (size_t)(c % cw) + (size_t)(c / cw) * (size_t)cs;
   T tp[NT];
 #pragma unroll
@@ -717,10 +723,11 @@ hipError_t launch_conv_pass(bool dbl, int pass, int X, int Y, int Z, int R, cons
     const long long cs = pass == 1 ? (long long)X * Y : 0;
     const unsigned gx = (unsigned)((C + 255) / 256);
     const bool big = (long long)gx * 4 * ((N + 31) / 32) >= 256 * 8;
-    const dim3 grid(gx, (unsigned)(big ? (N + 31) / 32 : (N + 15) / 16));
-#define GSE_PENCIL(TT, SY, TJ)                                                              \
-  k_conv_pencil<TT, 5, SY, TJ><<<grid, 256, 0, s>>>(C, cw, cs, N, S, (const TT*)taps, \
-                                                     (const TT*)in, (TT*)out)
+    const int ntile = big ? (N + 31) / 32 : (N + 15) / 16;
+    const unsigned grid = ((gx + 7) / 8) * 8 * (unsigned)ntile;
+#define GSE_PENCIL(TT, SY, TJ)                                                                 \
+  k_conv_pencil<TT, 5, SY, TJ><<<grid, 256, 0, s>>>(C, cw, cs, N, S, (int)gx, ntile,       \
+                                                     (const TT*)taps, (const TT*)in, (TT*)out)
     if (dbl) {
       if (big) GSE_PENCIL(double, false, 32); else GSE_PENCIL(double, false, 16);
     } else if (sym) {

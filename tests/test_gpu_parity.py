@@ -418,6 +418,8 @@ def test_convolution_fp32_asymmetric_taps(R):
     ((256, 40, 256), F64, False),
     ((8, 5, 6), F32, True),       # shortest row of the 4-wide x pass (halo from 8 values)
     ((12, 9, 10), F64, False),
+    ((256, 130, 256), F32, True),  # y pass in 32-row tiles, last tile 2 rows
+    ((256, 130, 256), F64, False),
 ])
 def test_convolution_pencil_shapes(dims, dt, sym):
     """Radius-5 passes (4-wide x pass, register-pencil y and z passes):
