@@ -635,7 +635,8 @@ gse_status_t gse_set_tuning_param(const char* name, int value) {
     t.spread_block = value;
   else if (n == "gather_block" && (value == 0 || value == 256 || value == 512)) t.gather_block = value;
   else if (n == "spread_stage" && value >= 1024 && value <= 96 * 1024) t.spread_stage = value;
-  else if (n == "gather_stage" && value >= 1024 && value <= 96 * 1024) t.gather_stage = value;
+  else if (n == "gather_stage" && (value == 0 || (value >= 1024 && value <= 96 * 1024)))
+    t.gather_stage = value;
   else if (n == "spread_lds32" && (value == 0 || value == 1)) t.spread_lds32 = value;
   else if (n == "spread_cols" && (value == 0 || value == 1)) t.spread_cols = value;
   else if (n == "spread_flush" && (value == 0 || value == 1)) t.spread_flush = value;

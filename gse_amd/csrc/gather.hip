@@ -791,7 +791,7 @@ size_t gather_lds_bytes(bool dbl, const Geom& g, const BinShape& b, int* cap_out
   const int NW = 2 * (g.nx + g.ny + g.nz) + 3;
   const int ct = dbl ? 8 : 4;
   const int vol = brick_pitch(ct, g.X, b.Bx, b.Lx) * b.Ly * b.Lz;
-  const size_t lds_total = (size_t)vol * ct + (size_t)tuning().gather_stage;
+  const size_t lds_total = (size_t)vol * ct + (size_t)gather_stage_bytes(dbl);
   int cap = 256;
   while (cap > 1 && gather_lds(vol, cap, NW, ct).total > lds_total) cap--;
   if (cap_out) *cap_out = cap;
@@ -814,7 +814,7 @@ static hipError_t gather_nz(GatherArgs a, const Binning& w, const void* phi, con
   const int NW = 2 * (a.g.nx + a.g.ny + a.g.nz) + 3;
   const int vol = brick_pitch(sizeof(CT), a.g.X, a.b.Bx, a.b.Lx) * a.b.Ly * a.b.Lz;
   const Tuning& tu = tuning();
-  const size_t lds_total = (size_t)vol * sizeof(CT) + (size_t)tu.gather_stage;
+  const size_t lds_total = (size_t)vol * sizeof(CT) + (size_t)gather_stage_bytes(sizeof(CT) == 8);
   int cap = 256;
   while (cap > 1 && gather_lds(vol, cap, NW, sizeof(CT)).total > lds_total) cap--;
   a.cap = cap;

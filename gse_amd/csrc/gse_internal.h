@@ -41,7 +41,7 @@ struct Tuning {
   int spread_block = 0;        // threads per spread workgroup (0: 512 FP32 / 256 FP64)
   int gather_block = 0;        // threads per gather workgroup (0: 512 FP32 / 256 FP64)
   int spread_stage = 16 * 1024;  // LDS bytes for staged particles beside the brick
-  int gather_stage = 16 * 1024;
+  int gather_stage = 0;        // 0: 24 KiB for FP32 grids, 16 KiB for FP64
   int spread_lds32 = 1;        // FP32 grids: 32-bit accumulators with a per-bin scale
   int spread_cols = 0;         // column-owner spread in registers (else LDS-atomic brick)
   int spread_flush = 1;        // 1: bricks stored and summed by k_brick_reduce; 0: atomics
@@ -49,6 +49,13 @@ struct Tuning {
   int bin_agg = 4;             // particles per workgroup / 256 of the aggregated binning (0: off)
 };
 Tuning& tuning();
+// LDS bytes for the gather's staged particles (the gather_stage knob, or its
+// default): the FP32 brick (32 KB) leaves room for a bin's charges in one
+// chunk; the FP64 brick (64 KB) keeps two workgroups per CU at 16 KiB
+inline int gather_stage_bytes(bool dbl) {
+  const int v = tuning().gather_stage;
+  return v ? v : (dbl ? 16 * 1024 : 24 * 1024);
+}
 
 // Per-binning statistics, accumulated by the binning kernel and read by its
 // consumers (spread, conversion, gather).  Two copies alternate between

@@ -299,7 +299,7 @@ struct SpreadArgs {
 
 // LDS layout of k_spread: int64 brick at 0, then per staged particle (cap
 // of them) the 4-vector (CT, 32-byte aligned block), its 3(2n+1) factors
-// (double), q*pref*2^e (double) and its local brick offset (int).
+// (CT, as computed), q*pref*2^e (double) and its local brick offset (int).
 struct SpreadLds {
   size_t oX, oW, oQ, oB, total;
 };
@@ -309,7 +309,7 @@ __host__ __device__ inline SpreadLds spread_lds(int vol, int cap, int nw, int ct
   l.oX = o;
   o += (size_t)cap * 4 * ct_bytes;
   l.oW = o;
-  o += (size_t)cap * nw * 8;
+  o += ((size_t)cap * nw * ct_bytes + 7) & ~(size_t)7;
   l.oQ = o;
   o += (size_t)cap * 8;
   l.oB = o;
@@ -385,7 +385,7 @@ __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt
   const SpreadLds lay = spread_lds(vol * (int)sizeof(ACC) / 8 + 1, cap, NW, sizeof(CT));
   char* lds = (char*)brick;
   V4<CT>* sX = (V4<CT>*)(lds + lay.oX);  // [cap]
-  double* sW = (double*)(lds + lay.oW);  // [cap][NW]
+  CT* sW = (CT*)(lds + lay.oW);  // [cap][NW] factors (values of type CT)
   double* sQ = (double*)(lds + lay.oQ);  // [cap]
   int* sB = (int*)(lds + lay.oB);        // [cap]
   // the first chunk's particle (cap <= 256 <= BS: at most one per thread) is
@@ -468,7 +468,7 @@ __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt
         // t = (c*h - x) + j*h, rounded as lane_weight does (CpuGaussCharge.cpp:314-329)
         const double t = add_rn(sub_rn(mul_rn((double)c, lh), xv), mul_rn(lane_j, lh));
         const CT tc = (CT)t;
-        sW[(size_t)ai * NW + lane] = (double)gexp(mul_rn(mul_rn(-tc, tc), inv2s2_c));
+        sW[(size_t)ai * NW + lane] = gexp(mul_rn(mul_rn(-tc, tc), inv2s2_c));
       }
     }
     __syncthreads();
@@ -477,30 +477,30 @@ __device__ __forceinline__ void spread_bin(const SpreadArgs& a, int bin, int cnt
         const int ai = e / 3, ax = e - 3 * ai;
         const int j0 = ax == 0 ? 0 : ax == 1 ? NX : NX + NY;
         const int nj = ax == 0 ? NX : ax == 1 ? NY : NZ;
-        double* w = sW + (size_t)ai * NW + j0;
+        CT* w = sW + (size_t)ai * NW + j0;
         CT sum = 0;
-        for (int j = 0; j < nj; j++) sum += (CT)w[j];
+        for (int j = 0; j < nj; j++) sum += w[j];
         const CT nrm = (CT)a.pref1 / sum;
-        for (int j = 0; j < nj; j++) w[j] = (double)((CT)w[j] * nrm);
+        for (int j = 0; j < nj; j++) w[j] = w[j] * nrm;
       }
       __syncthreads();
     }
     // phase 2: one wave per particle, lanes on (tx,ty) columns, walk tz
     for (int ai = wid; ai < na; ai += nwv) {
-      const double* w = sW + (size_t)ai * NW;
+      const CT* w = sW + (size_t)ai * NW;
       const int base = sB[ai];
       const double qps = sQ[ai];
       double wz[NZC > 0 ? NZC : 1];
       if (NZC > 0) {
 #pragma unroll
-        for (int t = 0; t < (NZC > 0 ? NZC : 1); t++) wz[t] = w[NX + NY + t];
+        for (int t = 0; t < (NZC > 0 ? NZC : 1); t++) wz[t] = (double)w[NX + NY + t];
       }
 #pragma unroll
       for (int k = 0; k < kMaxK; k++) {
         if (k >= KI) break;
         if (k_off[k] >= 0) {
           // ((q pref 2^e) wx) wy: the reference's product order, scaled exactly
-          const double qxy = (qps * w[k_tx[k]]) * w[k_ty[k]];
+          const double qxy = (qps * (double)w[k_tx[k]]) * (double)w[k_ty[k]];
           const int a0 = base + k_off[k];
           if (NZC > 0) {
             // PL > 0: the plane stride is a compile-time LDS offset
@@ -744,7 +744,7 @@ __global__ void __launch_bounds__(BS)
   const int cap = a.cap;
   const SpreadLds lay = spread_lds(0, cap, NW, sizeof(CT));
   V4<CT>* sX = (V4<CT>*)(cols_smem + lay.oX);
-  double* sW = (double*)(cols_smem + lay.oW);
+  CT* sW = (CT*)(cols_smem + lay.oW);
   double* sQ = (double*)(cols_smem + lay.oQ);
   int* sO = (int*)(cols_smem + lay.oB);  // packed local origin ox | oy<<10 | oz<<20
   (void)NW_MAX;
@@ -781,7 +781,7 @@ __global__ void __launch_bounds__(BS)
       const int cy = (int)round(y * g.ihy) - g.ny;
       const int cz = (int)round(z * g.ihz) - g.nz;
       if (role.axis >= 0)
-        sW[(size_t)ai * NW + lane] = (double)lane_weight<CT>(role, cx, cy, cz, x, y, z, g, a.inv2s2);
+        sW[(size_t)ai * NW + lane] = lane_weight<CT>(role, cx, cy, cz, x, y, z, g, a.inv2s2);
       if (lane == 0) {
         const int ox = wrapi(cx, g.X) - bxi * b.Bx;  // in [0, Bx)
         const int oy = wrapi(cy, g.Y) - byi * b.By;
@@ -796,11 +796,11 @@ __global__ void __launch_bounds__(BS)
         const int ai = e / 3, ax = e - 3 * ai;
         const int j0 = ax == 0 ? 0 : ax == 1 ? NX : NX + NY;
         const int nj = ax == 0 ? NX : ax == 1 ? NY : NZC;
-        double* w = sW + (size_t)ai * NW + j0;
+        CT* w = sW + (size_t)ai * NW + j0;
         CT sum = 0;
-        for (int j = 0; j < nj; j++) sum += (CT)w[j];
+        for (int j = 0; j < nj; j++) sum += w[j];
         const CT nrm = (CT)a.pref1 / sum;
-        for (int j = 0; j < nj; j++) w[j] = (double)((CT)w[j] * nrm);
+        for (int j = 0; j < nj; j++) w[j] = w[j] * nrm;
       }
       __syncthreads();
     }
@@ -809,12 +809,12 @@ __global__ void __launch_bounds__(BS)
       const int tx = cxl - (o & 1023);
       const int ty = cyl - ((o >> 10) & 1023);
       if (has_col && (unsigned)tx < (unsigned)NX && (unsigned)ty < (unsigned)NY) {
-        const double* w = sW + (size_t)ai * NW;
+        const CT* w = sW + (size_t)ai * NW;
         // ((q pref 2^e) wx) wy: the reference's product order, scaled exactly
-        const double qxy = (sQ[ai] * w[tx]) * w[NX + ty];
+        const double qxy = (sQ[ai] * (double)w[tx]) * (double)w[NX + ty];
         double wz[NZC];
 #pragma unroll
-        for (int t = 0; t < NZC; t++) wz[t] = w[NX + NY + t];
+        for (int t = 0; t < NZC; t++) wz[t] = (double)w[NX + NY + t];
         switch (o >> 20) {
 #define GSE_COL_CASE(OZ) \
   case OZ:               \

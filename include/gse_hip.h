@@ -63,7 +63,8 @@ GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
  *   spread_block  threads per spread workgroup, 0 = automatic (0,256,512,1024)
  *   gather_block  threads per gather workgroup, 0 = automatic (0,256,512)
  *   spread_stage  LDS bytes for staged particles in the spread (16 KiB)
- *   gather_stage  LDS bytes for staged particles in the gather (16 KiB)
+ *   gather_stage  LDS bytes for staged particles in the gather, 0 = automatic
+ *                 (24 KiB for FP32 grids, 16 KiB for FP64) (0)
  *   spread_lds32  FP32 grids: 32-bit accumulators, per-bin scale (0/1, 1)
  *   spread_cols   column-owner register spread (0/1, 0); 0 = LDS-atomic brick
  *   spread_flush  1: each bin stores its brick and one pass sums the bricks
