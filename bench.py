@@ -164,8 +164,8 @@ def bench_batched(api, systems, torch, dev, nsys=256):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build measures the single-GPU path "
-              "(see DESIGN.md for how it would shard); use --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: it measures the single-GPU step of one "
+              "system; use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
