@@ -46,6 +46,7 @@ struct Tuning {
   int spread_cols = 0;         // column-owner spread in registers (else LDS-atomic brick)
   int spread_flush = 1;        // 1: bricks stored and summed by k_brick_reduce; 0: atomics
   int reduce_tile = 1;         // tiled brick reduce where the grid allows it
+  int reduce_nt = 0;           // FP32 tiled reduce: 1 non-temporal brick loads
   int bin_agg = 4;             // particles per workgroup / 256 of the aggregated binning (0: off)
 };
 Tuning& tuning();

@@ -991,12 +991,13 @@ __global__ void __launch_bounds__(512)
 // k_brick_reduce_tile with 32-bit bricks whose rows are whole 16-byte
 // pieces (Lx a multiple of 4): 128 threads per bin interior, each thread four
 // consecutive x points read and written as 16-byte pieces, the 27 candidate
-// bricks taken one z layer (9 bricks) at a time.  Every brick word is read
-// exactly once; with NT the brick loads are non-temporal.  That pays while
-// the brick array of one system is cache-resident (a 32 MB array reduces
-// about 18% faster) and costs when it is not (a 630 MB array, or the bricks
-// of a batch of systems, reduce 30-60% slower), so the host picks NT by the
-// array's size.
+// bricks taken one z layer (9 bricks) at a time.  Each load fills a
+// zero-initialised vector under `if (ok)`; selecting between a load and a
+// zero vector instead took 94 VGPRs (5 waves per SIMD) against 54 (8).
+// Every brick word is read exactly once; NT makes the brick loads
+// non-temporal, which measured slower at every grid size from 80^3 to 216^3
+// (a brick row's three pieces go to three tiles, and the later pieces then
+// miss), so it is off by default.
 typedef unsigned BrickWord4 __attribute__((ext_vector_type(4)));
 template <typename T, bool NT>
 __device__ __forceinline__ void reduce_tile4_bin(const Geom& g, const BinShape& b, int bin,
@@ -1659,11 +1660,8 @@ hipError_t launch_brick_reduce(bool dbl, bool acc32, const Geom& g, const BinSha
   if (brick_tile_ok(g, b) && tuning().reduce_tile) {
     const int nwg = (b.nbins + 7) / 8 * 8;
     if (!dbl && acc32 && b.Lx % 4 == 0) {
-      // non-temporal brick loads while the brick array fits well inside the
-      // 256 MB last-level cache
-      const size_t brick_bytes = (size_t)b.nbins * b.Lx * b.Ly * b.Lz * sizeof(unsigned);
-      auto k = brick_bytes <= ((size_t)128 << 20) ? k_brick_reduce_tile4<float, true>
-                                                  : k_brick_reduce_tile4<float, false>;
+      auto k = tuning().reduce_nt ? k_brick_reduce_tile4<float, true>
+                                  : k_brick_reduce_tile4<float, false>;
       k<<<nwg, 128, 0, s>>>(g, b, nwg, (const unsigned*)w.bricks, w.bshift, w.stats, w.n,
                             w.images, w.pref_bound, grid64, (float*)out);
       return hipGetLastError();

@@ -493,3 +493,30 @@ def test_binning_variants_give_identical_grids(dt):
                 assert torch.equal(a, b)
     finally:
         api.set_tuning(bin_agg=keys["bin_agg"])
+
+
+def test_reduce_load_modes_give_identical_grids():
+    """The FP32 tiled brick reduce with ordinary or non-temporal brick loads
+    sums the same integers, so the grids and energies are bit-identical;
+    forces match the FP64 oracle to Erms 1e-5."""
+    xyzq = systems.water_box(2000, 40.0, seed=29)
+    M = 48
+    p = orc.Params(M, 40.0, 2.12)
+    ref = orc.gse_step(p, xyzq.astype(np.float32).astype(np.float64))
+    keys = api.get_tuning()
+    try:
+        outs = []
+        for mode in (0, 1):
+            api.set_tuning(reduce_nt=mode)
+            g = api.CpuGaussCharge(F32, F32, M, M, M)
+            rho = grid(p.dims, F32)
+            g.spreadChargeToGrid(p.sigma1, p.rcut1, len(xyzq), to_dev(xyzq, F32), *p.box, rho)
+            gs = api.CpuGSEk(F32, F32, M, M, M, 2.12, 0.4, 3.0, 3.0, 40.0, 40.0, 40.0)
+            o = gs.step(to_dev(xyzq, F32), force=True, potential=True)
+            assert orc.erms(ref.force, host(o["force"]).T) <= 1e-5
+            outs.append((rho, o["energy"].clone()))
+        for r in outs[1:]:
+            for a, b in zip(outs[0], r):
+                assert torch.equal(a, b)
+    finally:
+        api.set_tuning(reduce_nt=keys["reduce_nt"])
