@@ -936,36 +936,34 @@ __device__ __forceinline__ void reduce_tile_bin(const Geom& g, const BinShape& b
   const int Lx = b.Lx, Ly = b.Ly, Lz = b.Lz;
   const size_t vol = (size_t)Lx * Ly * Lz;
   // the 27 candidate bricks (the bin and its neighbours at -1, -2 bins on
-  // each axis): every shift and every value is loaded before any is summed,
-  // so the loads overlap instead of each waiting for the previous one
-  int sh[27];
-  ACC raw[27];
-#pragma unroll
-  for (int k = 0; k < 27; k++) {
-    const int dx = k % 3, dy = (k / 3) % 3, dz = k / 9;
-    int bx = cx - dx, by = cy - dy, bz = cz - dz;
-    bx += bx < 0 ? b.nbx : 0;
-    by += by < 0 ? b.nby : 0;
-    bz += bz < 0 ? b.nbz : 0;
-    sh[k] = bshift[bx + b.nbx * (by + b.nby * bz)];
-  }
-#pragma unroll
-  for (int k = 0; k < 27; k++) {
-    const int dx = k % 3, dy = (k / 3) % 3, dz = k / 9;
-    int bx = cx - dx, by = cy - dy, bz = cz - dz;
-    bx += bx < 0 ? b.nbx : 0;
-    by += by < 0 ? b.nby : 0;
-    bz += bz < 0 ? b.nbz : 0;
-    const int lx = tx + 8 * dx, ly = ty + 8 * dy, lz = tz + 8 * dz;
-    const bool ok = sh[k] >= 0 && lx < Lx && ly < Ly && lz < Lz;
-    raw[k] = ok ? bricks[(size_t)(bx + b.nbx * (by + b.nby * bz)) * vol + lx + Lx * (ly + Ly * lz)]
-                : (ACC)0;
-  }
+  // each axis), one z layer of 9 at a time: a layer's shifts and values are
+  // all loaded before any is summed, so its loads overlap; each load fills a
+  // zeroed value under `if (ok)`, which keeps the register count low
   long long sum = 0;
 #pragma unroll
-  for (int k = 0; k < 27; k++)
-    if (sh[k] >= 0)
-      sum += sizeof(ACC) == 4 ? ((long long)(int)raw[k] << sh[k]) : (long long)raw[k];
+  for (int dz = 0; dz < 3; dz++) {
+    int sh[9];
+    ACC raw[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const int dx = k % 3, dy = k / 3;
+      int bx = cx - dx, by = cy - dy, bz = cz - dz;
+      bx += bx < 0 ? b.nbx : 0;
+      by += by < 0 ? b.nby : 0;
+      bz += bz < 0 ? b.nbz : 0;
+      const int nbin = bx + b.nbx * (by + b.nby * bz);
+      sh[k] = bshift[nbin];
+      const int lx = tx + 8 * dx, ly = ty + 8 * dy, lz = tz + 8 * dz;
+      const bool ok = sh[k] >= 0 && lx < Lx && ly < Ly && lz < Lz;
+      ACC v = 0;
+      if (ok) v = bricks[(size_t)nbin * vol + lx + Lx * (ly + Ly * lz)];
+      raw[k] = v;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+      if (sh[k] >= 0)
+        sum += sizeof(ACC) == 4 ? ((long long)(int)raw[k] << sh[k]) : (long long)raw[k];
+  }
   const size_t p = (size_t)(8 * cx + tx) + (size_t)g.X * ((8 * cy + ty) + (size_t)g.Y * (8 * cz + tz));
   if (ovf) {
     sum += (long long)grid64[p];
