@@ -56,3 +56,32 @@ def test_argument_validation_without_device(lib):
     from oracle import oracle
     assert n.value == oracle.support_size((20, 20, 20), 5.6921, (20.0,) * 3)
     assert lib.gse_gauss_charge_destroy(h) == 0
+
+
+def test_tuning_knobs_round_trip_and_reject_bad_values(lib):
+    """Every named launch-shape knob reads back what was set, rejects values
+    outside its documented range with a message naming it, and leaves its
+    value unchanged on rejection.  Setting knobs touches no device."""
+    from gse_amd import api
+    keys = api.get_tuning()
+    assert set(keys) == set(api.TUNING_KEYS)
+    good = {"bin_edge": 6, "brick_budget": 32768, "spread_block": 256, "gather_block": 256,
+            "spread_stage": 8192, "gather_stage": 16384, "spread_lds32": 0, "spread_cols": 1,
+            "spread_flush": 0, "reduce_tile": 0, "reduce_nt": 1, "bin_agg": 2}
+    bad = {"bin_edge": 0, "spread_block": 384, "gather_block": 1024, "spread_stage": 512,
+           "gather_stage": 200000, "spread_lds32": 2, "reduce_tile": -1, "reduce_nt": 2,
+           "bin_agg": 3}
+    assert set(good) == set(keys)
+    try:
+        for k, v in good.items():
+            api.set_tuning(**{k: v})
+            assert api.get_tuning()[k] == v, k
+        for k, v in bad.items():
+            before = api.get_tuning()[k]
+            assert lib.gse_set_tuning_param(k.encode(), v) == 1, k
+            assert k.encode() in lib.gse_last_error(), k
+            assert api.get_tuning()[k] == before, k
+        assert lib.gse_set_tuning_param(b"no_such_knob", 1) == 1
+    finally:
+        api.set_tuning(**keys)
+    assert api.get_tuning() == keys
