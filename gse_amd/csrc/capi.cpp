@@ -642,6 +642,7 @@ gse_status_t gse_set_tuning_param(const char* name, int value) {
   else if (n == "spread_flush" && (value == 0 || value == 1)) t.spread_flush = value;
   else if (n == "reduce_tile" && (value == 0 || value == 1)) t.reduce_tile = value;
   else if (n == "reduce_nt" && (value == 0 || value == 1)) t.reduce_nt = value;
+  else if (n == "reduce_tile2" && (value == 0 || value == 1)) t.reduce_tile2 = value;
   else if (n == "bin_agg" && (value == 0 || value == 1 || value == 2 || value == 4 || value == 8))
     t.bin_agg = value;
   else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter or value out of range: %s=%d",
@@ -664,6 +665,7 @@ gse_status_t gse_get_tuning_param(const char* name, int* value) {
   else if (n == "spread_flush") *value = t.spread_flush;
   else if (n == "reduce_tile") *value = t.reduce_tile;
   else if (n == "reduce_nt") *value = t.reduce_nt;
+  else if (n == "reduce_tile2") *value = t.reduce_tile2;
   else if (n == "bin_agg") *value = t.bin_agg;
   else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter: %s", name);
   return GSE_SUCCESS;

@@ -47,6 +47,7 @@ struct Tuning {
   int spread_flush = 1;        // 1: bricks stored and summed by k_brick_reduce; 0: atomics
   int reduce_tile = 1;         // tiled brick reduce where the grid allows it
   int reduce_nt = 0;           // FP32 tiled reduce: 1 non-temporal brick loads
+  int reduce_tile2 = 1;        // 64-bit bricks: tiled reduce with 16-byte pieces
   int bin_agg = 4;             // particles per workgroup / 256 of the aggregated binning (0: off)
 };
 Tuning& tuning();

@@ -1072,6 +1072,72 @@ __global__ void __launch_bounds__(128)
   reduce_tile4_bin<T, NT>(g, b, bin, bricks, bshift, inv, st.overflow > 0, grid64, out);
 }
 
+// k_brick_reduce_tile with 64-bit bricks whose rows have an even length: 256
+// threads per bin interior, each thread two consecutive x points read as one
+// 16-byte piece per brick, the 27 candidate bricks taken one z layer (9
+// bricks) at a time as in reduce_tile4_bin.
+template <typename T>
+__device__ __forceinline__ void reduce_tile2_bin(const Geom& g, const BinShape& b, int bin,
+                                                 const unsigned long long* __restrict__ bricks,
+                                                 const int* __restrict__ bshift, double inv,
+                                                 bool ovf,
+                                                 unsigned long long* __restrict__ grid64,
+                                                 T* __restrict__ out) {
+  typedef unsigned long long BrickWord2 __attribute__((ext_vector_type(2)));
+  const int cx = bin % b.nbx, cy = (bin / b.nbx) % b.nby, cz = bin / (b.nbx * b.nby);
+  const int tq = threadIdx.x & 3, ty = (threadIdx.x >> 2) & 7, tz = threadIdx.x >> 5;
+  const int Lx = b.Lx, Ly = b.Ly, Lz = b.Lz;
+  const size_t vol = (size_t)Lx * Ly * Lz;
+  long long sum[2] = {0, 0};
+#pragma unroll
+  for (int dz = 0; dz < 3; dz++) {
+    BrickWord2 raw[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const int dx = k % 3, dy = k / 3;
+      int bx = cx - dx, by = cy - dy, bz = cz - dz;
+      bx += bx < 0 ? b.nbx : 0;
+      by += by < 0 ? b.nby : 0;
+      bz += bz < 0 ? b.nbz : 0;
+      const int nbin = bx + b.nbx * (by + b.nby * bz);
+      const int lx = 2 * tq + 8 * dx, ly = ty + 8 * dy, lz = tz + 8 * dz;
+      const bool ok = bshift[nbin] >= 0 && lx < Lx && ly < Ly && lz < Lz;
+      BrickWord2 v = {0ull, 0ull};
+      if (ok) v = *(const BrickWord2*)&bricks[(size_t)nbin * vol + lx + Lx * (ly + Ly * lz)];
+      raw[k] = v;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      sum[0] += (long long)raw[k].x;
+      sum[1] += (long long)raw[k].y;
+    }
+  }
+  const size_t p = (size_t)(8 * cx + 2 * tq) + (size_t)g.X * ((8 * cy + ty) + (size_t)g.Y * (8 * cz + tz));
+  if (ovf) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      sum[c] += (long long)grid64[p + c];
+      grid64[p + c] = 0ull;
+    }
+  }
+  out[p] = (T)((double)sum[0] * inv);
+  out[p + 1] = (T)((double)sum[1] * inv);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_brick_reduce_tile2(Geom g, BinShape b, int nwg, const unsigned long long* __restrict__ bricks,
+                         const int* __restrict__ bshift, const BinStats* __restrict__ stats,
+                         int n, int images, double pref_bound,
+                         unsigned long long* __restrict__ grid64, T* __restrict__ out) {
+  const int bin = xcd_remap(blockIdx.x, nwg);
+  if (bin >= b.nbins) return;
+  const BinStats st = *stats;
+  const double inv =
+      ldexp(1.0, -fixed_exponent(st.qmax_bits, n - st.invalid, images, pref_bound));
+  reduce_tile2_bin<T>(g, b, bin, bricks, bshift, inv, st.overflow > 0, grid64, out);
+}
+
 // ---------------------------------------------------------------------------
 // Batched independent systems: the same per-bin bodies, one launch for all
 // systems.  Spread and gather walk a table of (system, bin) items with a
@@ -1662,6 +1728,17 @@ hipError_t launch_brick_reduce(bool dbl, bool acc32, const Geom& g, const BinSha
                                   : k_brick_reduce_tile4<float, false>;
       k<<<nwg, 128, 0, s>>>(g, b, nwg, (const unsigned*)w.bricks, w.bshift, w.stats, w.n,
                             w.images, w.pref_bound, grid64, (float*)out);
+      return hipGetLastError();
+    }
+    if (!acc32 && b.Lx % 2 == 0 && tuning().reduce_tile2) {
+      if (dbl)
+        k_brick_reduce_tile2<double><<<nwg, 256, 0, s>>>(
+            g, b, nwg, (const unsigned long long*)w.bricks, w.bshift, w.stats, w.n, w.images,
+            w.pref_bound, grid64, (double*)out);
+      else
+        k_brick_reduce_tile2<float><<<nwg, 256, 0, s>>>(
+            g, b, nwg, (const unsigned long long*)w.bricks, w.bshift, w.stats, w.n, w.images,
+            w.pref_bound, grid64, (float*)out);
       return hipGetLastError();
     }
     if (dbl)

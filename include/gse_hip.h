@@ -71,6 +71,7 @@ GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
  *                 covering each grid point; 0: bricks added with atomics (0/1, 1)
  *   reduce_tile   that sum by one workgroup per 8^3 bin where the grid allows (0/1, 1)
  *   reduce_nt     brick loads of that FP32 sum: 0 ordinary, 1 non-temporal (0/1, 0)
+ *   reduce_tile2  that sum over 64-bit bricks with two x points per thread (0/1, 1)
  *   bin_agg       binning claims bin slots once per workgroup and bin, with
  *                 256 x this many particles per workgroup (grids of at most
  *                 16384 bins); 0 = one claim per particle (0,1,2,4,8; 4) */

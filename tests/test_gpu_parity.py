@@ -520,3 +520,37 @@ def test_reduce_load_modes_give_identical_grids():
                 assert torch.equal(a, b)
     finally:
         api.set_tuning(reduce_nt=keys["reduce_nt"])
+
+
+@pytest.mark.parametrize("dt,lds32", [(F64, 1), (F32, 0)], ids=["F64", "F32-int64"])
+def test_reduce_tile2_gives_identical_grids(dt, lds32):
+    """The tiled reduce of 64-bit bricks with two x points per thread and the
+    one-point form sum the same integers: grids and energies bit-identical,
+    FP64 grid and forces within 1e-10 of the oracle (FP32: Erms 1e-5)."""
+    xyzq = systems.water_box(2000, 40.0, seed=31)
+    M = 48
+    p = orc.Params(M, 40.0, 2.12)
+    src = xyzq if dt == F64 else xyzq.astype(np.float32).astype(np.float64)
+    ref = orc.gse_step(p, src)
+    keys = api.get_tuning()
+    try:
+        api.set_tuning(spread_lds32=lds32)
+        outs = []
+        for mode in (1, 0):
+            api.set_tuning(reduce_tile2=mode)
+            g = api.CpuGaussCharge(dt, dt, M, M, M)
+            rho = grid(p.dims, dt)
+            g.spreadChargeToGrid(p.sigma1, p.rcut1, len(xyzq), to_dev(xyzq, dt), *p.box, rho)
+            gs = api.CpuGSEk(dt, dt, M, M, M, 2.12, 0.4, 3.0, 3.0, 40.0, 40.0, 40.0)
+            o = gs.step(to_dev(xyzq, dt), force=True, potential=True)
+            f = host(o["force"]).T
+            if dt == F64:
+                assert rel(host(rho), orc.spread(p.dims, p.sigma1, p.rcut1, xyzq, p.box)) <= TOL
+                assert rel(f, ref.force) <= TOL
+            else:
+                assert orc.erms(ref.force, f) <= 1e-5
+            outs.append((rho, o["energy"].clone()))
+        for a, b in zip(outs[0], outs[1]):
+            assert torch.equal(a, b)
+    finally:
+        api.set_tuning(reduce_tile2=keys["reduce_tile2"], spread_lds32=keys["spread_lds32"])
