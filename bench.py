@@ -152,13 +152,74 @@ def bench_batched(api, systems, torch, dev, nsys=256):
             h.step(x, force=True, potential=False, out=o)
 
     t_loop = time_steps(loop, 10, 2, torch)
+    # the batched forces against the looped single-system forces, every system
+    f_loop = torch.cat([o["force"] for o in outs], dim=1).double().cpu().numpy()
+    f_batch = ob["force"].double().cpu().numpy()
+    e_loop = torch.cat([o["energy"] for o in outs]).cpu().numpy()
+    e_batch = ob["energy"].cpu().numpy()
+    erms = float(np.sqrt(np.sum((f_batch - f_loop) ** 2) / np.sum(f_loop ** 2)))
+    de = float(np.max(np.abs(e_batch - e_loop) / np.abs(e_loop)))
     return {"systems": nsys, "charges": int(xc.shape[0]),
+            "force_erms_batched_vs_looped": erms, "max_energy_rel_batched_vs_looped": de,
             "systems_per_s_graph": round(nsys / t_graph, 1),
             "systems_per_s_eager": round(nsys / t_eager, 1),
             "systems_per_s_looped": round(nsys / t_loop, 1),
             "speedup_graph_vs_looped": round(t_loop / t_graph, 3),
             "ms_per_batched_step": round(t_graph * 1e3, 4),
             "autotuned": cfg}
+
+
+def host_info():
+    """Core count and CPU model of this host (for the baseline record)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    return {"nproc": affinity, "logical_cpus": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_baseline(args, M, L, sigma, xyzq_np, f_gpu, e_gpu, N):
+    """The oracle's GSE-k step (spread, FFT solve, energy, reference force
+    and phi_i: the GPU step's work) timed on this host: one core, the
+    reference's own threading, and OpenMP across the cores the process may
+    use (OMP_NUM_THREADS when set).  Each leg runs up to 5 steps within
+    --cpu-seconds and reports the median.  Accuracy of the GPU step is taken
+    against the serial FP64 oracle."""
+    from oracle import oracle
+    p = oracle.Params(M, L, sigma)
+    info = host_info()
+    omp_n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or info["nproc"]
+
+    def timed(threads):
+        oracle.set_omp_threads(threads)
+        ts = []
+        t_end = time.perf_counter() + args.cpu_seconds
+        while not ts or (time.perf_counter() < t_end and len(ts) < 5):
+            t0 = time.perf_counter()
+            oracle.step_omp(p, xyzq_np)
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), len(ts)
+
+    t1, n1 = timed(1)
+    tn, nn = timed(omp_n)
+    ref = oracle.gse_step(p, xyzq_np, mode=0)
+    cpu = {"value": round(1.0 / t1, 5), "unit": "steps/s", "cores": 1, "kind": "port",
+           "sample": f"{n1} FP64 oracle GSE-k step(s) of the {N}-charge M={M} box on one "
+                     f"host core, median {t1:.3f} s",
+           "all_cores": {"value": round(1.0 / tn, 5), "unit": "steps/s", "cores": omp_n,
+                         "sample": f"{nn} step(s), OpenMP over {omp_n} threads, "
+                                   f"median {tn:.3f} s"},
+           **info}
+    return cpu, oracle.erms(ref.force, f_gpu), abs(e_gpu - ref.energy) / abs(ref.energy)
 
 
 def main():
@@ -219,21 +280,7 @@ def main():
     erms = None
     de = None
     if not args.no_cpu:
-        from oracle import oracle
-        p = oracle.Params(M, L, sigma)
-        ts = []
-        t_end = time.perf_counter() + args.cpu_seconds
-        ref = None
-        while not ts or (time.perf_counter() < t_end and len(ts) < 5):
-            t0 = time.perf_counter()
-            ref = oracle.gse_step(p, xyzq_np, mode=0)
-            ts.append(time.perf_counter() - t0)
-        t_med = float(np.median(ts))
-        cpu = {"value": round(1.0 / t_med, 5), "unit": "steps/s", "cores": 1, "kind": "port",
-               "sample": f"{len(ts)} full FP64 oracle step(s) of {args.workload} "
-                         f"({N} charges, M={M}) on one host core, median {t_med:.2f} s"}
-        erms = oracle.erms(ref.force, f_gpu)
-        de = abs(e_gpu - ref.energy) / abs(ref.energy)
+        cpu, erms, de = cpu_baseline(args, M, L, sigma, xyzq_np, f_gpu, e_gpu, N)
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "steps/s", "n_gpus": 1,

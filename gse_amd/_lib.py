@@ -20,6 +20,11 @@ GATHER_CUBE_FORCE = 2
 GATHER_POTENTIAL = 4
 GATHER_VIRIAL = 8
 
+LAYOUT_FIELDS = 10
+# conversion kernel of the last spread (gse_*_get_layout field 8)
+REDUCE_PATHS = {-1: "none", 0: "atomic", 1: "generic", 2: "tile4", 3: "tile4_nt", 4: "tile2",
+                5: "tile"}
+
 _lib = None
 
 V = ctypes.c_void_p
@@ -63,6 +68,7 @@ _SIGS = {
     "gse_gauss_charge_calc_dipole_sum": [V, D, D, D, V, PD],
     "gse_gauss_charge_gather": [V, D, D, I, V, D, D, D, V, ctypes.POINTER(GatherDesc)],
     "gse_gauss_charge_invalid_count": [V, PI],
+    "gse_gauss_charge_get_layout": [V, PI],
     "gse_convolution_create": [PV, I, I, I, I],
     "gse_convolution_destroy": [V],
     "gse_convolution_set_stream": [V, V],
@@ -86,6 +92,7 @@ _SIGS = {
     "gse_gsek_solve_poisson": [V],
     "gse_gsek_calculate_energy": [V, PD],
     "gse_gsek_interpolate_force": [V, I, V, V, V, V],
+    "gse_gsek_get_layout": [V, PI],
     "gse_gsek_get_rho_s": [V, PV],
     "gse_gsek_get_phi": [V, PV],
     "gse_gsek_copy_grid": [V, I, V],

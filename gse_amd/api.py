@@ -60,6 +60,21 @@ def _check_xyzq(xyzq, ct):
         raise TypeError(f"xyzq must be {ct}, got {xyzq.dtype}")
 
 
+def _check_count(numCoord, xyzq):
+    """numCoord may not exceed the records xyzq holds: the kernels read
+    numCoord records from the device pointer."""
+    if not isinstance(numCoord, int) or numCoord < 0 or numCoord > xyzq.shape[0]:
+        raise ValueError(f"numCoord must be in [0, {xyzq.shape[0]}] (got {numCoord})")
+
+
+def _layout(fn, h):
+    info = (ctypes.c_int * _lib.LAYOUT_FIELDS)()
+    check(fn(h, info))
+    v = list(info)
+    return {"bin": tuple(v[0:3]), "nbins": v[3], "brick": tuple(v[4:7]), "direct": v[7],
+            "reduce": _lib.REDUCE_PATHS[v[8]], "cap_bin": v[9]}
+
+
 def _check_grid(g, dims, dt):
     X, Y, Z = dims
     if tuple(g.shape) != (Z, Y, X):
@@ -98,6 +113,7 @@ class CpuGaussCharge:
     def spreadChargeToGrid(self, sigma, rcut, numCoord, xyzq, boxx, boxy, boxz, rho,
                            normalize=False):
         _check_xyzq(xyzq, self.ct)
+        _check_count(numCoord, xyzq)
         _check_grid(rho, self.dims, self.ct)
         self._s()
         check(self._L.gse_gauss_charge_spread_charge_to_grid(
@@ -114,6 +130,7 @@ class CpuGaussCharge:
     def interpolateForce(self, sigma, rcut, numCoord, xyzq, boxx, boxy, boxz, phi, forceX,
                          forceY, forceZ):
         _check_xyzq(xyzq, self.ct)
+        _check_count(numCoord, xyzq)
         _check_grid(phi, self.dims, self.ct)
         for f in (forceX, forceY, forceZ):
             if f.dtype != self.at or f.numel() < numCoord:
@@ -169,6 +186,11 @@ class CpuGaussCharge:
         self._s()
         check(self._L.gse_gauss_charge_invalid_count(self._h, ctypes.byref(c)))
         return c.value
+
+    def layout(self):
+        """Bin layout of the last spread or gather and the kernel that
+        converted the last spread's fixed-point sums (host only)."""
+        return _layout(self._L.gse_gauss_charge_get_layout, self._h)
 
 
 class CpuConvolution:
@@ -308,6 +330,7 @@ class CpuGSEk:
 
     def spreadCharge1(self, numCoord, xyzq):
         _check_xyzq(xyzq, self.ct)
+        _check_count(numCoord, xyzq)
         self._s()
         check(self._L.gse_gsek_spread_charge1(self._h, numCoord, _ptr(xyzq)))
 
@@ -328,6 +351,10 @@ class CpuGSEk:
 
     def interpolateForce(self, numCoord, xyzq, forceX, forceY, forceZ):
         _check_xyzq(xyzq, self.ct)
+        _check_count(numCoord, xyzq)
+        for f in (forceX, forceY, forceZ):
+            if f.dtype != self.at or f.numel() < numCoord:
+                raise ValueError("force arrays must be AT with numCoord entries")
         self._s()
         check(self._L.gse_gsek_interpolate_force(self._h, numCoord, _ptr(xyzq), _ptr(forceX),
                                                  _ptr(forceY), _ptr(forceZ)))
@@ -338,6 +365,10 @@ class CpuGSEk:
         self._s()
         check(self._L.gse_gsek_copy_grid(self._h, which, _ptr(out)))
         return out
+
+    def layout(self):
+        """Bin layout and conversion kernel of the handle's last spread."""
+        return _layout(self._L.gse_gsek_get_layout, self._h)
 
     def getRhoS(self):
         """Copy of rhoS (CpuGSE.h:63)."""

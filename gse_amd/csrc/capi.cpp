@@ -10,7 +10,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -120,6 +122,7 @@ struct Engine {
   int X = 0, Y = 0, Z = 0;
   DevBuf counts[2], stats[2], ovf, perm, loc, sorted, grid64, vir6, part, bricks, bshift;
   bool brick_flush = false;     // the last spread stored bricks (k_brick_reduce converts)
+  int reduce_path = -1;         // ReducePath of the last conversion (-1: none yet)
   int cur = 1;                  // buffer of the latest binning
   long long clean[2] = {0, 0};  // leading counts (and the stats) known zero
   bool grid64_clean = false;
@@ -222,13 +225,23 @@ struct Engine {
   }
 
   gse_status_t convert(void* rho, hipStream_t s) {
-    if (brick_flush)
+    if (brick_flush) {
+      reduce_path = brick_reduce_path(ct_dbl, acc32(), geom, shape);
       HIPCHK(launch_brick_reduce(ct_dbl, acc32(), geom, shape, bin,
                                  grid64.as<unsigned long long>(), rho, s));
-    else
+    } else {
+      reduce_path = kReduceAtomic;
       HIPCHK(launch_fixed_to_grid(ct_dbl, grid64.as<unsigned long long>(), rho,
                                   (size_t)X * Y * Z, bin, true, s));
+    }
     return GSE_SUCCESS;
+  }
+
+  // [Bx, By, Bz, nbins, Lx, Ly, Lz, direct, reduce path, slots per bin]
+  void layout(int* info) const {
+    const int v[GSE_LAYOUT_FIELDS] = {shape.Bx, shape.By, shape.Bz, shape.nbins, shape.Lx,
+                                      shape.Ly, shape.Lz, shape.direct, reduce_path, bin.cap_bin};
+    std::memcpy(info, v, sizeof(v));
   }
 
   gse_status_t gather_binned(double sigma, double rcut, int flags, const void* phi,
@@ -268,10 +281,99 @@ struct Engine {
 // ---------------------------------------------------------------------------
 std::once_flag g_rocfft_once;
 
+// Process-wide cache of rocFFT plan pairs keyed by (precision, X, Y, Z,
+// batch).  Handles of equal shape share one plan pair (reference counted);
+// each handle keeps its own execution info (stream) and work buffer.  A plan
+// pair is destroyed when its last handle releases it.
+struct PlanPair {
+  rocfft_plan fwd = nullptr, bwd = nullptr;
+  size_t work_f = 0, work_b = 0;
+  int refs = 0;
+};
+struct PlanKey {
+  int dbl, X, Y, Z, batch;
+  bool operator<(const PlanKey& o) const {
+    return std::tie(dbl, X, Y, Z, batch) < std::tie(o.dbl, o.X, o.Y, o.Z, o.batch);
+  }
+};
+std::mutex g_plan_mu;
+std::map<PlanKey, PlanPair> g_plans;
+
+gse_status_t create_plan_pair(const PlanKey& k, PlanPair& pp) {
+  const int X = k.X, Y = k.Y, Z = k.Z, batch = k.batch;
+  const size_t lengths[3] = {(size_t)X, (size_t)Y, (size_t)Z};
+  const rocfft_precision prec = k.dbl ? rocfft_precision_double : rocfft_precision_single;
+  rocfft_plan_description df = nullptr, db = nullptr;
+  if (batch > 1) {
+    const size_t Xh = (size_t)X / 2 + 1;
+    const size_t rs[3] = {1, (size_t)X, (size_t)X * Y};
+    const size_t cs[3] = {1, Xh, Xh * Y};
+    const size_t rd = (size_t)X * Y * Z, cd = Xh * Y * Z;
+    if (rocfft_plan_description_create(&df) != rocfft_status_success ||
+        rocfft_plan_description_create(&db) != rocfft_status_success ||
+        rocfft_plan_description_set_data_layout(df, rocfft_array_type_real,
+                                                rocfft_array_type_hermitian_interleaved,
+                                                nullptr, nullptr, 3, rs, rd, 3, cs, cd) !=
+            rocfft_status_success ||
+        rocfft_plan_description_set_data_layout(db, rocfft_array_type_hermitian_interleaved,
+                                                rocfft_array_type_real, nullptr, nullptr, 3,
+                                                cs, cd, 3, rs, rd) != rocfft_status_success) {
+      if (df) rocfft_plan_description_destroy(df);
+      if (db) rocfft_plan_description_destroy(db);
+      return fail(GSE_ERROR_FFT, "rocfft plan description (%dx%dx%d x %d) failed", X, Y, Z,
+                  batch);
+    }
+  }
+  const rocfft_status sf =
+      rocfft_plan_create(&pp.fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                         prec, 3, lengths, (size_t)batch, df);
+  const rocfft_status sb =
+      rocfft_plan_create(&pp.bwd, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                         prec, 3, lengths, (size_t)batch, db);
+  if (df) rocfft_plan_description_destroy(df);
+  if (db) rocfft_plan_description_destroy(db);
+  if (sf != rocfft_status_success || sb != rocfft_status_success) {
+    if (sf == rocfft_status_success) rocfft_plan_destroy(pp.fwd);
+    if (sb == rocfft_status_success) rocfft_plan_destroy(pp.bwd);
+    pp.fwd = pp.bwd = nullptr;
+    return fail(GSE_ERROR_FFT, "rocfft_plan_create (%s %dx%dx%d x %d) failed",
+                sf != rocfft_status_success ? "r2c" : "c2r", X, Y, Z, batch);
+  }
+  rocfft_plan_get_work_buffer_size(pp.fwd, &pp.work_f);
+  rocfft_plan_get_work_buffer_size(pp.bwd, &pp.work_b);
+  return GSE_SUCCESS;
+}
+
+gse_status_t acquire_plans(const PlanKey& k, PlanPair* out) {
+  std::call_once(g_rocfft_once, [] { rocfft_setup(); });
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  auto it = g_plans.find(k);
+  if (it == g_plans.end()) {
+    PlanPair pp;
+    GSECHK(create_plan_pair(k, pp));
+    it = g_plans.emplace(k, pp).first;
+  }
+  it->second.refs++;
+  *out = it->second;
+  return GSE_SUCCESS;
+}
+
+void release_plans(const PlanKey& k) {
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  auto it = g_plans.find(k);
+  if (it == g_plans.end()) return;
+  if (--it->second.refs == 0) {
+    rocfft_plan_destroy(it->second.fwd);
+    rocfft_plan_destroy(it->second.bwd);
+    g_plans.erase(it);
+  }
+}
+
 struct FFT3 {
   bool dbl = true;
   int X = 0, Y = 0, Z = 0;
-  rocfft_plan fwd = nullptr, bwd = nullptr;
+  PlanKey key{};
+  rocfft_plan fwd = nullptr, bwd = nullptr;  // shared, owned by the cache
   rocfft_execution_info info = nullptr;
   DevBuf work;
   size_t work_f = 0, work_b = 0;
@@ -283,44 +385,13 @@ struct FFT3 {
     X = x;
     Y = y;
     Z = z;
-    std::call_once(g_rocfft_once, [] { rocfft_setup(); });
-    const size_t lengths[3] = {(size_t)X, (size_t)Y, (size_t)Z};
-    const rocfft_precision prec = dbl ? rocfft_precision_double : rocfft_precision_single;
-    rocfft_plan_description df = nullptr, db = nullptr;
-    if (batch > 1) {
-      const size_t Xh = (size_t)X / 2 + 1;
-      const size_t rs[3] = {1, (size_t)X, (size_t)X * Y};
-      const size_t cs[3] = {1, Xh, Xh * Y};
-      const size_t rd = (size_t)X * Y * Z, cd = Xh * Y * Z;
-      if (rocfft_plan_description_create(&df) != rocfft_status_success ||
-          rocfft_plan_description_create(&db) != rocfft_status_success ||
-          rocfft_plan_description_set_data_layout(df, rocfft_array_type_real,
-                                                  rocfft_array_type_hermitian_interleaved,
-                                                  nullptr, nullptr, 3, rs, rd, 3, cs, cd) !=
-              rocfft_status_success ||
-          rocfft_plan_description_set_data_layout(db, rocfft_array_type_hermitian_interleaved,
-                                                  rocfft_array_type_real, nullptr, nullptr, 3,
-                                                  cs, cd, 3, rs, rd) != rocfft_status_success) {
-        if (df) rocfft_plan_description_destroy(df);
-        if (db) rocfft_plan_description_destroy(db);
-        return fail(GSE_ERROR_FFT, "rocfft plan description (%dx%dx%d x %d) failed", X, Y, Z,
-                    batch);
-      }
-    }
-    const rocfft_status sf =
-        rocfft_plan_create(&fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
-                           prec, 3, lengths, (size_t)batch, df);
-    const rocfft_status sb =
-        rocfft_plan_create(&bwd, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
-                           prec, 3, lengths, (size_t)batch, db);
-    if (df) rocfft_plan_description_destroy(df);
-    if (db) rocfft_plan_description_destroy(db);
-    if (sf != rocfft_status_success)
-      return fail(GSE_ERROR_FFT, "rocfft_plan_create (r2c %dx%dx%d x %d) failed", X, Y, Z, batch);
-    if (sb != rocfft_status_success)
-      return fail(GSE_ERROR_FFT, "rocfft_plan_create (c2r %dx%dx%d x %d) failed", X, Y, Z, batch);
-    rocfft_plan_get_work_buffer_size(fwd, &work_f);
-    rocfft_plan_get_work_buffer_size(bwd, &work_b);
+    key = PlanKey{d ? 1 : 0, x, y, z, batch};
+    PlanPair pp;
+    GSECHK(acquire_plans(key, &pp));
+    fwd = pp.fwd;
+    bwd = pp.bwd;
+    work_f = pp.work_f;
+    work_b = pp.work_b;
     const size_t w = work_f > work_b ? work_f : work_b;
     if (w) GSECHK(work.ensure(w));
     if (rocfft_execution_info_create(&info) != rocfft_status_success)
@@ -329,8 +400,7 @@ struct FFT3 {
   }
 
   void release() {
-    if (fwd) rocfft_plan_destroy(fwd);
-    if (bwd) rocfft_plan_destroy(bwd);
+    if (fwd) release_plans(key);
     if (info) rocfft_execution_info_destroy(info);
     fwd = bwd = nullptr;
     info = nullptr;
@@ -853,6 +923,12 @@ gse_status_t gse_gauss_charge_invalid_count(gse_gauss_charge_t h, int* count) {
   return h->eng.invalid_count(h->stream, count);
 }
 
+gse_status_t gse_gauss_charge_get_layout(gse_gauss_charge_t h, int* info) {
+  if (!h || !info) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  h->eng.layout(info);
+  return GSE_SUCCESS;
+}
+
 // ------------------------------- CpuConvolution ----------------------------
 gse_status_t gse_convolution_create(gse_convolution_t* out, gse_dtype_t t, int X, int Y, int Z) {
   if (!out) return fail(GSE_ERROR_INVALID_VALUE, "out is NULL");
@@ -1124,6 +1200,12 @@ gse_status_t gse_gsek_interpolate_force(gse_gsek_t h, int n, const void* xyzq, v
   o.fz = fz;
   o.fscale = h->ccelec;
   return h->eng.gather_binned(h->sigma1, h->rcut1(), GF_SPHERE, h->phiM.p, o, nullptr, h->stream);
+}
+
+gse_status_t gse_gsek_get_layout(gse_gsek_t h, int* info) {
+  if (!h || !info) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  h->eng.layout(info);
+  return GSE_SUCCESS;
 }
 
 gse_status_t gse_gsek_get_rho_s(gse_gsek_t h, void** p) {

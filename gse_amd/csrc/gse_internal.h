@@ -147,6 +147,16 @@ hipError_t launch_fixed_to_grid(bool ct_double, const unsigned long long* grid64
 
 // Brick-store flush: usable for this shape (LDS path, bounded brick overlap)?
 bool brick_reduce_ok(const Geom& g, const BinShape& b);
+// Which kernel converts a spread's fixed-point sums into the grid:
+enum ReducePath : int {
+  kReduceAtomic = 0,   // bricks added with 64-bit atomics, then k_fixed_to_grid
+  kReduceGeneric = 1,  // k_brick_reduce, one thread per grid point
+  kReduceTile4 = 2,    // k_brick_reduce_tile4 (32-bit bricks, 16-byte pieces)
+  kReduceTile4NT = 3,  // the same with non-temporal brick loads
+  kReduceTile2 = 4,    // k_brick_reduce_tile2 (64-bit bricks, two x points per thread)
+  kReduceTile = 5,     // k_brick_reduce_tile (one x point per thread)
+};
+int brick_reduce_path(bool ct_double, bool acc32, const Geom& g, const BinShape& b);
 // rho = 2^-e (sum of the bricks covering each point + the overflow int64 grid)
 hipError_t launch_brick_reduce(bool ct_double, bool acc32, const Geom& g, const BinShape& b,
                                const Binning& w, unsigned long long* grid64, void* out,

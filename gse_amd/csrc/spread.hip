@@ -1718,19 +1718,27 @@ bool brick_tile_ok(const Geom& g, const BinShape& b) {
          b.nbx >= 3 && b.nby >= 3 && b.nbz >= 3 && b.Lx <= 24 && b.Ly <= 24 && b.Lz <= 24;
 }
 
+int brick_reduce_path(bool dbl, bool acc32, const Geom& g, const BinShape& b) {
+  if (!(brick_tile_ok(g, b) && tuning().reduce_tile)) return kReduceGeneric;
+  if (!dbl && acc32 && b.Lx % 4 == 0) return tuning().reduce_nt ? kReduceTile4NT : kReduceTile4;
+  if (!acc32 && b.Lx % 2 == 0 && tuning().reduce_tile2) return kReduceTile2;
+  return kReduceTile;
+}
+
 hipError_t launch_brick_reduce(bool dbl, bool acc32, const Geom& g, const BinShape& b,
                                const Binning& w, unsigned long long* grid64, void* out,
                                hipStream_t s) {
-  if (brick_tile_ok(g, b) && tuning().reduce_tile) {
+  const int path = brick_reduce_path(dbl, acc32, g, b);
+  if (path != kReduceGeneric) {
     const int nwg = (b.nbins + 7) / 8 * 8;
-    if (!dbl && acc32 && b.Lx % 4 == 0) {
-      auto k = tuning().reduce_nt ? k_brick_reduce_tile4<float, true>
-                                  : k_brick_reduce_tile4<float, false>;
+    if (path == kReduceTile4 || path == kReduceTile4NT) {
+      auto k = path == kReduceTile4NT ? k_brick_reduce_tile4<float, true>
+                                      : k_brick_reduce_tile4<float, false>;
       k<<<nwg, 128, 0, s>>>(g, b, nwg, (const unsigned*)w.bricks, w.bshift, w.stats, w.n,
                             w.images, w.pref_bound, grid64, (float*)out);
       return hipGetLastError();
     }
-    if (!acc32 && b.Lx % 2 == 0 && tuning().reduce_tile2) {
+    if (path == kReduceTile2) {
       if (dbl)
         k_brick_reduce_tile2<double><<<nwg, 256, 0, s>>>(
             g, b, nwg, (const unsigned long long*)w.bricks, w.bshift, w.stats, w.n, w.images,

@@ -171,6 +171,19 @@ GSE_API gse_status_t gse_gauss_charge_gather(gse_gauss_charge_t h, double sigma,
  * |x/h| >= 1e9).  Synchronises the stream. */
 GSE_API gse_status_t gse_gauss_charge_invalid_count(gse_gauss_charge_t h, int* count);
 
+/* Bin layout of the handle's last spread or gather and the kernel that
+ * converted its fixed-point sums (build-defined diagnostic, no reference
+ * counterpart).  info receives GSE_LAYOUT_FIELDS host ints:
+ *   [0..2] bin edge Bx, By, Bz (cells)   [3] number of bins
+ *   [4..6] brick edge Lx, Ly, Lz          [7] 1: one bin, global atomics
+ *   [8] conversion kernel of the last spread: -1 none yet, 0 atomic flush,
+ *       1 generic brick reduce, 2 tiled 32-bit, 3 tiled 32-bit with
+ *       non-temporal loads, 4 tiled 64-bit two points per thread, 5 tiled
+ *       one point per thread
+ *   [9] particle slots per bin (the rest goes to the overflow list) */
+#define GSE_LAYOUT_FIELDS 10
+GSE_API gse_status_t gse_gauss_charge_get_layout(gse_gauss_charge_t h, int* info);
+
 /* ------------------------------------------------------------------------
  * CpuConvolution<T>  (CpuConvolution.h:9-38)
  * ---------------------------------------------------------------------- */
@@ -229,14 +242,14 @@ GSE_API gse_status_t gse_greens_func_mul(gse_dtype_t t, gse_greens_kind_t kind, 
  * ---------------------------------------------------------------------- */
 typedef struct gse_gsek_s* gse_gsek_t;
 
-/* CpuGSEk.h:26-29  CpuGSEk(sizeX, sizeY, sizeZ, sigma, kappa, lambdaSigma,
+/* CpuGSEk.h:28-31  CpuGSEk(sizeX, sizeY, sizeZ, sigma, kappa, lambdaSigma,
  * lambdaSigma1, boxx, boxy, boxz, ccelec); asserts of CpuGSE.cpp:33-44
  * (0 <= kappa <= 0.5) become GSE_ERROR_INVALID_VALUE. */
 GSE_API gse_status_t gse_gsek_create(gse_gsek_t* out, gse_dtype_t at, gse_dtype_t ct,
                                      int sizeX, int sizeY, int sizeZ, double sigma,
                                      double kappa, double lambdaSigma, double lambdaSigma1,
                                      double boxx, double boxy, double boxz, double ccelec);
-/* CpuGSEk.h:30 */
+/* CpuGSEk.h:32 */
 GSE_API gse_status_t gse_gsek_destroy(gse_gsek_t h);
 GSE_API gse_status_t gse_gsek_set_stream(gse_gsek_t h, void* hip_stream);
 /* CpuGSE.h:58  setBoxSize */
@@ -244,13 +257,13 @@ GSE_API gse_status_t gse_gsek_set_box_size(gse_gsek_t h, double boxx, double box
 /* CpuGSE.h:65-67  getSigma1, getSigma2, getLambdaSigma1 (host) */
 GSE_API gse_status_t gse_gsek_get_params(gse_gsek_t h, double* sigma1, double* sigma2,
                                          double* lambdaSigma1);
-/* CpuGSEk.h:31  spreadCharge1(numCoord, xyzq): sigma1 spread into rhoS */
+/* CpuGSEk.h:33  spreadCharge1(numCoord, xyzq): sigma1 spread into rhoS */
 GSE_API gse_status_t gse_gsek_spread_charge1(gse_gsek_t h, int numCoord, const void* xyzq);
-/* CpuGSEk.h:32  spreadCharge2(rhoOut): sigma2 on-grid spread of rhoS */
+/* CpuGSEk.h:34  spreadCharge2(rhoOut): sigma2 on-grid spread of rhoS */
 GSE_API gse_status_t gse_gsek_spread_charge2(gse_gsek_t h, void* rhoOut);
-/* CpuGSEk.h:33  solvePoisson(): phiM = FFT solve of rhoS with G_GSE(sigma2) */
+/* CpuGSEk.h:35  solvePoisson(): phiM = FFT solve of rhoS with G_GSE(sigma2) */
 GSE_API gse_status_t gse_gsek_solve_poisson(gse_gsek_t h);
-/* CpuGSEk.h:34  calculateEnergy(): 0.5*ccelec*h^3*sum(rhoS*phiM)
+/* CpuGSEk.h:36  calculateEnergy(): 0.5*ccelec*h^3*sum(rhoS*phiM)
  * (CpuGSE.cpp:86-104).  Synchronises; host result. */
 GSE_API gse_status_t gse_gsek_calculate_energy(gse_gsek_t h, double* energy);
 /* CpuGSE.h:61  interpolateForce(numCoord, xyzq, forceX, forceY, forceZ):
@@ -260,6 +273,8 @@ GSE_API gse_status_t gse_gsek_interpolate_force(gse_gsek_t h, int numCoord, cons
 /* CpuGSE.h:63-64  getRhoS / getPhi: device pointers owned by the handle */
 GSE_API gse_status_t gse_gsek_get_rho_s(gse_gsek_t h, void** rhoS);
 GSE_API gse_status_t gse_gsek_get_phi(gse_gsek_t h, void** phi);
+/* As gse_gauss_charge_get_layout, for the handle's own spread and gather. */
+GSE_API gse_status_t gse_gsek_get_layout(gse_gsek_t h, int* info);
 /* Asynchronous device copy of rhoS (which = 0) or phiM (which = 1) into dst. */
 GSE_API gse_status_t gse_gsek_copy_grid(gse_gsek_t h, int which, void* dst);
 

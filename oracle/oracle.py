@@ -16,6 +16,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 _REF = None
+_OMP = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i = ctypes.c_int
@@ -42,6 +43,38 @@ def lib():
         L.orc_ewald_recip.restype = _d
         _LIB = L
     return _LIB
+
+
+def omp_lib():
+    """OpenMP timing build of the step (oracle/liboracle_omp.so)."""
+    global _OMP
+    if _OMP is None:
+        path = os.path.join(_HERE, "liboracle_omp.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.orc_step_omp.restype = _d
+        L.orc_omp_threads.restype = _i
+        _OMP = L
+    return _OMP
+
+
+def set_omp_threads(n):
+    omp_lib().orc_omp_set_threads(int(n))
+
+
+def step_omp(params, xyzq):
+    """One GSE-k step on all OpenMP threads (timing build; sums in another
+    order than gse_step).  Returns (energy, force (n, 3) x ccelec, pot x
+    ccelec, threads)."""
+    a, n = _xyzq(xyzq)
+    m = max(n, 1)
+    f = np.zeros((3, m))
+    pot = np.zeros(m)
+    L = omp_lib()
+    c = params._c()
+    e = L.orc_step_omp(ctypes.byref(c), n, _p(a), _p(f[0]), _p(f[1]), _p(f[2]), _p(pot))
+    return e, f[:, :n].T.copy(), pot[:n].copy(), L.orc_omp_threads()
 
 
 def ref_lib():

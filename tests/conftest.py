@@ -24,7 +24,7 @@ def orc():
 def ref(orc):
     """The reference's own compiled CPU classes, when they could be built."""
     L = orc.ref_lib()
-    if L is None:
+    if L is None and os.path.isdir("/root/reference"):
         try:
             orc.build(ref=True)
         except Exception:

@@ -199,3 +199,77 @@ def test_batch_fp64_brick_path_with_overflow():
     o = b.step(torch.from_numpy(concat(sysl)).cuda(), force=True, potential=True)
     torch.cuda.synchronize()
     check_against(sysl, refs, o, 1e-10, 1e-10)
+
+
+def _rounded_f32(sysl):
+    xc = torch.from_numpy(concat(sysl)).to("cuda", F32)
+    xr = xc.double().cpu().numpy()
+    rounded, p0 = [], 0
+    for x, _, _ in sysl:
+        rounded.append(xr[p0:p0 + len(x)])
+        p0 += len(x)
+    return xc, rounded
+
+
+def test_batch_every_candidate_shape():
+    """Each launch shape the autotuner may pick (bin edge 6, 8, 10 x spread
+    block 256, 512 x gather block 256, 512) gives FP32 results within Erms
+    1e-5 of the oracle.  Edges 6 and 10 leave the brick store for the
+    atomic flush, so both conversion paths of the batch are covered."""
+    bs = systems.batched_systems(12, seed=9)
+    sysl = [(x, (M, M, M), (L, L, L)) for x, L, M in bs]
+    xc, rounded = _rounded_f32(sysl)
+    refs = run_oracle(sysl, rounded)
+    b = make_batch(F32, sysl)
+    for edge in (6, 8, 10):
+        for sb in (256, 512):
+            for gb in (256, 512):
+                print(f"shape edge={edge} spread={sb} gather={gb}", flush=True)
+                b.set_config(bin_edge=edge, spread_block=sb, gather_block=gb)
+                assert b.get_config() == dict(bin_edge=edge, spread_block=sb, gather_block=gb)
+                o = b.step(xc, force=True, graph=(sb == gb))
+                torch.cuda.synchronize()
+                check_against(sysl, refs, o, 1e-5, 1e-5, erms=True)
+
+
+NSAMPLE_PER_SYS = 64
+
+
+def test_batch_256_water_boxes_graph_vs_oracle():
+    """The batched workload at full size: 256 water boxes of 600..8000
+    charges (1,130,376 in all), M in {24, 32, 40}, FP32, autotuned, one
+    HIP-graph replay per step.  Every system is checked against the FP64
+    oracle on the same float32-rounded particles: its energy (|dE|/E <=
+    1e-5) and its reference forces on a fixed sample of 64 charges (Erms
+    <= 1e-5, cpu_gse.cpp:131-146).  The oracle's spread, FFT solve and
+    energy run on every system; its gather only on the sample."""
+    bs = systems.batched_systems(256, seed=4)
+    sysl = [(x, (M, M, M), (L, L, L)) for x, L, M in bs]
+    xc, rounded = _rounded_f32(sysl)
+    b = make_batch(F32, sysl)
+    cfg, ms = b.autotune(xc, reps=2)
+    print("autotuned", cfg, f"{ms:.3f} ms/step", flush=True)
+    g = {}
+    for _ in range(3):
+        b.step(xc, force=True, graph=True, out=g)
+    torch.cuda.synchronize()
+    f = g["force"].double().cpu().numpy().T
+    e = g["energy"].cpu().numpy()
+    rng = np.random.default_rng(256)
+    p0, worst_f, worst_e = 0, 0.0, 0.0
+    for k, (x, dims, box) in enumerate(sysl):
+        n = len(x)
+        p = orc.Params(dims, box, SIG)
+        xr = rounded[k]
+        rho = orc.spread(p.dims, p.sigma1, p.rcut1, xr, p.box)
+        phi = orc.fft_solve(0, p.dims, p.sigma2, p.box, rho)
+        E = orc.energy(p.dims, p.ccelec, p.box, rho, phi)
+        idx = np.sort(rng.choice(n, min(n, NSAMPLE_PER_SYS), replace=False))
+        fs = orc.interpolate(p.dims, p.sigma1, p.rcut1, xr[idx], p.box, phi) * p.ccelec
+        ef = orc.erms(fs, f[p0 + idx])
+        ee = abs(e[k] - E) / abs(E)
+        worst_f, worst_e = max(worst_f, ef), max(worst_e, ee)
+        assert ef <= 1e-5, (k, ef)
+        assert ee <= 1e-5, (k, ee)
+        p0 += n
+    print(f"256 systems: worst Erms {worst_f:.2e}, worst |dE|/E {worst_e:.2e}", flush=True)

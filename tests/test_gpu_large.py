@@ -95,3 +95,27 @@ def test_parseval_real_space_energy(big):
     er = g.calculateEnergy()
     print(f"1M FP32: k-space E {ek:.6f} real-space E {er:.6f}", flush=True)
     assert abs(ek - er) / abs(er) <= 1e-5
+
+
+def test_water48k_fp64_vs_oracle():
+    """The 48,000-charge water box (16,000 triplets, L = 78.3, M = 80) in FP64
+    mode, the drop-in's own precision: energy, reference forces (sphere
+    support, x ccelec) and per-atom potentials within 1e-10 of the FP64
+    oracle's full step on the same particles."""
+    xyzq, L, M, sigma = systems.make("water_48k_M80")
+    p = orc.Params(M, L, sigma)
+    print("oracle: FP64 step of 48,000 charges at M=80", flush=True)
+    ref = orc.gse_step(p, xyzq, mode=0)
+    g = api.CpuGSEk(torch.float64, torch.float64, M, M, M, sigma, 0.4, 3.0, 3.0, L, L, L)
+    o = g.step(torch.from_numpy(xyzq).cuda(), force=True, potential=True)
+    torch.cuda.synchronize()
+    e = float(o["energy"].cpu()[0])
+    f = o["force"].cpu().numpy().T
+    pot = o["pot"].cpu().numpy()
+    de = abs(e - ref.energy) / abs(ref.energy)
+    df = np.max(np.abs(f - ref.force)) / np.max(np.abs(ref.force))
+    dp = np.max(np.abs(pot - ref.pot)) / np.max(np.abs(ref.pot))
+    print(f"48k FP64: dE/E {de:.2e} max dF {df:.2e} max dphi_i {dp:.2e}", flush=True)
+    assert de <= 1e-10
+    assert df <= 1e-10
+    assert dp <= 1e-10

@@ -495,24 +495,53 @@ def test_binning_variants_give_identical_grids(dt):
         api.set_tuning(bin_agg=keys["bin_agg"])
 
 
+def _tiling_system(seed):
+    """A 48^3 grid in a 46 A box: h = 0.958, n = 6, so 8^3 bins with 20^3
+    bricks (64-bit bricks fit the LDS budget) and 6 bins per axis: every
+    tiled reduce applies.  A dense cluster overfills one bin, so the
+    reduce also adds the overflow particles' int64 grid."""
+    rng = np.random.default_rng(seed)
+    xyzq = systems.water_box(2000, 46.0, seed=seed)
+    cl = np.empty((500, 4))
+    cl[:, :3] = 21.0 + rng.random((500, 3)) * 0.9
+    cl[:, 3] = 0.1 * rng.standard_normal(500)
+    return np.concatenate([xyzq, cl]), 48, 46.0
+
+
+def _overflowed(layout, xyzq, p):
+    """True when some bin received more particles than its slots (checked on
+    the host from the cube origins, as the binning kernel assigns them)."""
+    B = layout["bin"][0]
+    o = np.floor(xyzq[:, :3] / (p.box[0] / p.X) + 0.5).astype(np.int64) - 6
+    o = np.mod(o, p.X) // B
+    nb = p.X // B
+    _, cnt = np.unique(o[:, 0] + nb * (o[:, 1] + nb * o[:, 2]), return_counts=True)
+    return cnt.max() > layout["cap_bin"]
+
+
 def test_reduce_load_modes_give_identical_grids():
     """The FP32 tiled brick reduce with ordinary or non-temporal brick loads
     sums the same integers, so the grids and energies are bit-identical;
-    forces match the FP64 oracle to Erms 1e-5."""
-    xyzq = systems.water_box(2000, 40.0, seed=29)
-    M = 48
-    p = orc.Params(M, 40.0, 2.12)
+    forces match the FP64 oracle to Erms 1e-5.  The layout getter confirms
+    that each mode ran its own kernel, with the overflow grid in use."""
+    xyzq, M, L = _tiling_system(29)
+    p = orc.Params(M, L, 2.12)
     ref = orc.gse_step(p, xyzq.astype(np.float32).astype(np.float64))
     keys = api.get_tuning()
     try:
         outs = []
-        for mode in (0, 1):
+        for mode, path in ((0, "tile4"), (1, "tile4_nt")):
             api.set_tuning(reduce_nt=mode)
             g = api.CpuGaussCharge(F32, F32, M, M, M)
             rho = grid(p.dims, F32)
             g.spreadChargeToGrid(p.sigma1, p.rcut1, len(xyzq), to_dev(xyzq, F32), *p.box, rho)
-            gs = api.CpuGSEk(F32, F32, M, M, M, 2.12, 0.4, 3.0, 3.0, 40.0, 40.0, 40.0)
+            lay = g.layout()
+            assert lay["reduce"] == path, lay
+            assert lay["bin"] == (8, 8, 8) and lay["brick"] == (20, 20, 20), lay
+            assert _overflowed(lay, xyzq, p)
+            gs = api.CpuGSEk(F32, F32, M, M, M, 2.12, 0.4, 3.0, 3.0, L, L, L)
             o = gs.step(to_dev(xyzq, F32), force=True, potential=True)
+            assert gs.layout()["reduce"] == path
             assert orc.erms(ref.force, host(o["force"]).T) <= 1e-5
             outs.append((rho, o["energy"].clone()))
         for r in outs[1:]:
@@ -526,26 +555,31 @@ def test_reduce_load_modes_give_identical_grids():
 def test_reduce_tile2_gives_identical_grids(dt, lds32):
     """The tiled reduce of 64-bit bricks with two x points per thread and the
     one-point form sum the same integers: grids and energies bit-identical,
-    FP64 grid and forces within 1e-10 of the oracle (FP32: Erms 1e-5)."""
-    xyzq = systems.water_box(2000, 40.0, seed=31)
-    M = 48
-    p = orc.Params(M, 40.0, 2.12)
+    FP64 grid and forces within 1e-10 of the oracle (FP32: Erms 1e-5).  The
+    layout getter confirms each kernel ran, with the overflow grid in use."""
+    xyzq, M, L = _tiling_system(31)
+    p = orc.Params(M, L, 2.12)
     src = xyzq if dt == F64 else xyzq.astype(np.float32).astype(np.float64)
     ref = orc.gse_step(p, src)
+    rho_ref = orc.spread(p.dims, p.sigma1, p.rcut1, xyzq, p.box) if dt == F64 else None
     keys = api.get_tuning()
     try:
         api.set_tuning(spread_lds32=lds32)
         outs = []
-        for mode in (1, 0):
+        for mode, path in ((1, "tile2"), (0, "tile")):
             api.set_tuning(reduce_tile2=mode)
             g = api.CpuGaussCharge(dt, dt, M, M, M)
             rho = grid(p.dims, dt)
             g.spreadChargeToGrid(p.sigma1, p.rcut1, len(xyzq), to_dev(xyzq, dt), *p.box, rho)
-            gs = api.CpuGSEk(dt, dt, M, M, M, 2.12, 0.4, 3.0, 3.0, 40.0, 40.0, 40.0)
+            lay = g.layout()
+            assert lay["reduce"] == path, lay
+            assert _overflowed(lay, xyzq, p)
+            gs = api.CpuGSEk(dt, dt, M, M, M, 2.12, 0.4, 3.0, 3.0, L, L, L)
             o = gs.step(to_dev(xyzq, dt), force=True, potential=True)
+            assert gs.layout()["reduce"] == path
             f = host(o["force"]).T
             if dt == F64:
-                assert rel(host(rho), orc.spread(p.dims, p.sigma1, p.rcut1, xyzq, p.box)) <= TOL
+                assert rel(host(rho), rho_ref) <= TOL
                 assert rel(f, ref.force) <= TOL
             else:
                 assert orc.erms(ref.force, f) <= 1e-5
@@ -554,3 +588,21 @@ def test_reduce_tile2_gives_identical_grids(dt, lds32):
             assert torch.equal(a, b)
     finally:
         api.set_tuning(reduce_tile2=keys["reduce_tile2"], spread_lds32=keys["spread_lds32"])
+
+
+def test_numcoord_beyond_xyzq_is_rejected():
+    """numCoord larger than the particle tensor is refused on the host, before
+    any kernel could read past the allocation."""
+    x = to_dev(systems.water_box(10, 16.0, seed=2))
+    g = api.CpuGaussCharge(F64, F64, 16, 16, 16)
+    rho = grid((16, 16, 16))
+    with pytest.raises(ValueError):
+        g.spreadChargeToGrid(1.34, 5.69, 31, x, 16.0, 16.0, 16.0, rho)
+    f = torch.zeros(40, dtype=F64, device=DEV)
+    with pytest.raises(ValueError):
+        g.interpolateForce(1.34, 5.69, 31, x, 16.0, 16.0, 16.0, rho, f, f, f)
+    gs = api.CpuGSEk(F64, F64, 16, 16, 16, 2.12, 0.4, 3.0, 3.0, 16.0, 16.0, 16.0)
+    with pytest.raises(ValueError):
+        gs.spreadCharge1(-1, x)
+    with pytest.raises(ValueError):
+        gs.interpolateForce(31, x, f, f, f)

@@ -474,3 +474,20 @@ def test_efield_and_dipole_vs_reference(orc, ref):
     d = np.zeros(3)
     ref.ref_dipole_dd(X, Y, Z, *map(ctypes.c_double, box), P(g), P(d))
     assert np.array_equal(orc.dipole_sum(dims, box, g), d)
+
+
+def test_openmp_timing_step_matches_serial_oracle(orc):
+    """The OpenMP timing build of the step (the all-core CPU baseline) gives
+    the serial oracle's energy, reference forces and phi_i to rounding, on
+    one thread and on several."""
+    from gse_amd import systems
+    x = systems.water_box(400, 24.0, seed=12)
+    p = orc.Params(24, 24.0, 2.12)
+    r = orc.gse_step(p, x)
+    for nt in (1, 3):
+        orc.set_omp_threads(nt)
+        e, f, pot, used = orc.step_omp(p, x)
+        assert used == nt
+        assert abs(e - r.energy) <= 1e-12 * abs(r.energy)
+        assert np.max(np.abs(f - r.force)) <= 1e-12 * np.max(np.abs(r.force))
+        assert np.max(np.abs(pot - r.pot)) <= 1e-12 * np.max(np.abs(r.pot))
