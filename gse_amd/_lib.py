@@ -62,6 +62,7 @@ _SIGS = {
     "gse_gauss_charge_set_stream": [V, V],
     "gse_gauss_charge_calc_support_size": [V, D, D, D, D, PI],
     "gse_gauss_charge_spread_charge_to_grid": [V, D, D, I, V, D, D, D, V, I],
+    "gse_gauss_charge_spread_charge_to_grid_noalloc": [V, D, D, I, V, D, D, D, V, I],
     "gse_gauss_charge_spread_charge_on_grid": [V, D, D, D, D, D, V, V],
     "gse_gauss_charge_interpolate_force": [V, D, D, I, V, D, D, D, V, V, V, V],
     "gse_gauss_charge_calc_electric_field_on_grid": [V, D, D, D, V, V, V, V],

@@ -842,6 +842,92 @@ gse_status_t gse_gauss_charge_spread_charge_to_grid(gse_gauss_charge_t h, double
   return h->eng.convert(rho, h->stream);
 }
 
+// Allocation-free variant of gse_gauss_charge_spread_charge_to_grid: the
+// same binning, spread and conversion kernels, with every buffer beyond the
+// caller's particles and grid carved from the device's `scratch` array
+// (noalloc.hip).  No hipMalloc/hipFree and no host allocation; the scratch
+// is cleared here where the kernels need zeros.
+gse_status_t gse_gauss_charge_spread_charge_to_grid_noalloc(gse_gauss_charge_t h, double sigma,
+                                                            double rcut, int n,
+                                                            const void* xyzq, double bx,
+                                                            double by, double bz, void* rho,
+                                                            int normalize) {
+  if (!h || !rho) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  if (n < 0) return fail(GSE_ERROR_INVALID_VALUE, "numCoord must be >= 0");
+  if (n > 0 && !xyzq) return fail(GSE_ERROR_INVALID_VALUE, "xyzq is NULL");
+  GSECHK(check_box(bx, by, bz));
+  GSECHK(check_sigma_rcut(sigma, rcut));
+  const Engine& e = h->eng;
+  const bool dbl = e.ct_dbl;
+  const bool acc32 = !dbl && g_tuning.spread_lds32;
+  const Geom g = make_geom(e.X, e.Y, e.Z, bx, by, bz, rcut);
+  const BinShape b = choose_bins(g, 8, (size_t)g_tuning.brick_budget, g_tuning.bin_edge);
+  const int nb = b.nbins;
+  int cap_bin;
+  if (b.direct) {
+    cap_bin = n > 0 ? n : 1;
+  } else {
+    const long long c = 2LL * n / nb + 32;
+    cap_bin = (int)((c + 15) / 16 * 16);
+  }
+  const bool flush = g_tuning.spread_flush && !g_tuning.spread_cols && brick_reduce_ok(g, b);
+  const size_t slots = (size_t)nb * cap_bin, npts = (size_t)e.X * e.Y * e.Z;
+  // scratch layout, each region 256-byte aligned
+  size_t off = 0;
+  auto take = [&off](size_t bytes) {
+    const size_t at = off;
+    off += (bytes + 255) / 256 * 256;
+    return at;
+  };
+  const size_t o_cnt = take(sizeof(int) * (size_t)nb), o_cnt2 = take(sizeof(int) * (size_t)nb),
+               o_st = take(sizeof(BinStats)), o_st2 = take(sizeof(BinStats)),
+               o_ovf = take(sizeof(int) * (size_t)(n > 0 ? n : 1)),
+               o_perm = take(sizeof(int) * slots), o_loc = take(sizeof(int) * slots),
+               o_sorted = take(4 * tsize(dbl) * slots), o_g64 = take(8 * npts),
+               o_bricks = flush ? take((acc32 ? 4 : 8) * (size_t)b.Lx * b.Ly * b.Lz * nb) : 0,
+               o_bshift = flush ? take(sizeof(int) * (size_t)nb) : 0;
+  if (off > kNoAllocScratchBytes)
+    return fail(GSE_ERROR_INVALID_VALUE,
+                "the allocation-free spread needs %zu bytes of scratch for %d particles on a "
+                "%dx%dx%d grid; its limit is %zu bytes (use "
+                "gse_gauss_charge_spread_charge_to_grid)",
+                off, n, e.X, e.Y, e.Z, kNoAllocScratchBytes);
+  void* base = nullptr;
+  HIPCHK(noalloc_scratch(&base));
+  char* sp = (char*)base;
+  hipStream_t s = h->stream;
+  Binning w;
+  w.n = n;
+  w.cap_bin = cap_bin;
+  w.xyzq = xyzq;
+  w.counts = (int*)(sp + o_cnt);
+  w.counts_other = (int*)(sp + o_cnt2);
+  w.stats = (BinStats*)(sp + o_st);
+  w.stats_other = (BinStats*)(sp + o_st2);
+  w.ovf = (int*)(sp + o_ovf);
+  w.perm = (int*)(sp + o_perm);
+  w.loc = (int*)(sp + o_loc);
+  w.sorted = sp + o_sorted;
+  w.images = ((2 * g.nx + 1 + e.X - 1) / e.X) * ((2 * g.ny + 1 + e.Y - 1) / e.Y) *
+             ((2 * g.nz + 1 + e.Z - 1) / e.Z);
+  w.pref_bound = std::pow(2.0 * M_PI * sigma * sigma, -1.5) *
+                 spread_weight_bound(g, sigma, normalize != 0);
+  w.bricks = flush ? sp + o_bricks : nullptr;
+  w.bshift = flush ? (int*)(sp + o_bshift) : nullptr;
+  unsigned long long* g64 = (unsigned long long*)(sp + o_g64);
+  // zeros the kernels rely on: the int64 grid (overflow / atomic sums), the
+  // bin counts and statistics (fill kernels, not memsets: safe to capture)
+  HIPCHK(launch_fill_u64(g64, npts, s));
+  HIPCHK(launch_bin_clear(w.counts, nb, w.stats, s));
+  HIPCHK(launch_bin(dbl, g, b, w, s));
+  HIPCHK(launch_spread(dbl, g, b, sigma, normalize != 0, w, g64, s));
+  if (flush)
+    HIPCHK(launch_brick_reduce(dbl, acc32, g, b, w, g64, rho, s));
+  else
+    HIPCHK(launch_fixed_to_grid(dbl, g64, rho, npts, w, true, s));
+  return GSE_SUCCESS;
+}
+
 gse_status_t gse_gauss_charge_spread_charge_on_grid(gse_gauss_charge_t h, double sigma,
                                                     double rcut, double bx, double by, double bz,
                                                     const void* rho, void* rhoS) {

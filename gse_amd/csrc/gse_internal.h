@@ -126,6 +126,12 @@ __host__ __device__ inline int fixed_exponent_bin(unsigned long long qbits, int 
   return 30 - k;
 }
 
+// Device scratch of the allocation-free spread: one file-scope array per GPU
+// (noalloc.hip).  noalloc_scratch returns the current device's copy, looked
+// up once per device.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+hipError_t noalloc_scratch(void** out);
+
 // ---- kernel launchers (defined in the .hip units) ------------------------
 // ct_double: particle/grid compute type is double (else float)
 // Clears counts[0..nbins) and the statistics of one buffer.

@@ -114,6 +114,24 @@ GSE_API gse_status_t gse_gauss_charge_spread_charge_to_grid(gse_gauss_charge_t h
                                                             double boxy, double boxz, void* rho,
                                                             int normalize);
 
+/* No counterpart in the reference: spreadChargeToGrid for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, kernels and
+ * bit-identical result as gse_gauss_charge_spread_charge_to_grid, but every
+ * buffer beyond xyzq and rho lives in one static device array of
+ * GSE_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends); the call
+ * makes no hipMalloc/hipFree and no host allocation.  It needs about
+ * 8*X*Y*Z (int64 grid) + 4*n + slots*(8 + 4*sizeof(CT)) + the bricks
+ * (8^3 bins: (8+2n_sup)^3 x 4 or 8 bytes per bin) bytes, e.g. 39 MB for
+ * 48,000 FP32 charges on an 80^3 grid; a call needing more returns
+ * GSE_ERROR_INVALID_VALUE with the size in gse_last_error().  All calls on
+ * one GPU share that GPU's array: calls that overlap in time on one GPU
+ * (other streams, other threads) race and must be ordered by the caller. */
+#define GSE_NOALLOC_SCRATCH_BYTES (64u << 20)
+GSE_API gse_status_t gse_gauss_charge_spread_charge_to_grid_noalloc(
+    gse_gauss_charge_t h, double sigma, double rcut, int numCoord, const void* xyzq, double boxx,
+    double boxy, double boxz, void* rho, int normalize);
+
 /* CpuGaussCharge.h:46-48  spreadChargeOnGrid(sigma, rcut, box, rho, rhoS):
  * separable X->Y->Z periodic Gaussian convolution with radius ceil(rcut/h)
  * and the (2 pi sigma^2)^-3/2 h^3 prefactor on the Z taps. */
