@@ -57,6 +57,9 @@ _SIGS = {
     "gse_device_free": [V],
     "gse_memcpy_to_device": [V, V, ctypes.c_size_t],
     "gse_memcpy_to_host": [V, V, ctypes.c_size_t],
+    "gse_memcpy_device_to_device": [V, V, ctypes.c_size_t],
+    "gse_device_clear": [V, ctypes.c_size_t],
+    "gse_device_synchronize": [],
     "gse_gauss_charge_create": [PV, I, I, I, I, I],
     "gse_gauss_charge_destroy": [V],
     "gse_gauss_charge_set_stream": [V, V],
@@ -110,6 +113,22 @@ _SIGS = {
     "gse_batch_set_config": [V, I, I, I],
     "gse_batch_get_config": [V, PI],
     "gse_batch_autotune": [V, V, I, PD],
+    "gse_pme_recip_create": [PV, I, I, I, I, I, I],
+    "gse_pme_recip_destroy": [V],
+    "gse_pme_recip_set_stream": [V, V],
+    "gse_pme_recip_spread_charge": [V, V, I, PD],
+    "gse_pme_recip_r2c_fft": [V],
+    "gse_pme_recip_c2r_fft": [V],
+    "gse_pme_recip_scalar_sum": [V, PD, D, I, I, V, V],
+    "gse_pme_recip_gather_force": [V, V, I, PD, I, V],
+    "gse_pme_recip_calc_self_energy": [V, V, I, D, V],
+    "gse_pme_recip_step": [V, V, I, D, D, D, D, V, V, V, V, V],
+    "gse_pme_recip_get_grid": [V, PV],
+    "gse_ewald_direct_create": [PV, D, D, D, D, D, D],
+    "gse_ewald_direct_destroy": [V],
+    "gse_ewald_direct_set_stream": [V, V],
+    "gse_ewald_direct_setup_exclusions": [V, I, PI, PI],
+    "gse_ewald_direct_calc_force_energy": [V, I, V, V, V, V, V],
 }
 _RET = {"gse_status_string": ctypes.c_char_p, "gse_last_error": ctypes.c_char_p,
         "gse_version": I}

@@ -561,3 +561,156 @@ class GSEBatch:
         self._s()
         check(self._L.gse_batch_autotune(self._h, _ptr(xyzq), int(reps), ctypes.byref(ms)))
         return self.get_config(), ms.value
+
+
+def _recip9(recip):
+    r = (ctypes.c_double * 9)()
+    vals = list(recip)
+    if len(vals) == 3:
+        vals = [1.0 / vals[0], 0, 0, 0, 1.0 / vals[1], 0, 0, 0, 1.0 / vals[2]]
+    for k in range(9):
+        r[k] = float(vals[k])
+    return r
+
+
+class CudaPMERecip:
+    """CudaPMERecip<AT,CT,CT2> (cuda/CudaPMERecip.h:138-180) on the GPU: the
+    SPME reciprocal comparator, order 4, 6 or 8, BOX FFT on one GPU.  recip is
+    the 9-entry reciprocal box (orthorhombic) or the three box lengths."""
+
+    def __init__(self, at, ct, nfftx, nffty, nfftz, order):
+        L = _lib.load()
+        self.at = _TD[_code(at)]
+        self.ct = _TD[_code(ct)]
+        self.dims = (nfftx, nffty, nfftz)
+        self.order = order
+        h = ctypes.c_void_p()
+        check(L.gse_pme_recip_create(ctypes.byref(h), _code(at), _code(ct), nfftx, nffty, nfftz,
+                                     order))
+        self._h = h
+        self._L = L
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.gse_pme_recip_destroy(self._h)
+            self._h = None
+
+    def _s(self):
+        check(self._L.gse_pme_recip_set_stream(self._h, _stream()))
+
+    def spread_charge(self, xyzq, ncoord, recip):
+        _check_xyzq(xyzq, self.ct)
+        _check_count(ncoord, xyzq)
+        self._s()
+        check(self._L.gse_pme_recip_spread_charge(self._h, _ptr(xyzq), ncoord, _recip9(recip)))
+
+    def r2c_fft(self):
+        self._s()
+        check(self._L.gse_pme_recip_r2c_fft(self._h))
+
+    def c2r_fft(self):
+        self._s()
+        check(self._L.gse_pme_recip_c2r_fft(self._h))
+
+    def scalar_sum(self, recip, kappa, calc_energy=True, calc_virial=True):
+        """Returns dict with 'energy' (1,) and 'virial' (9,) device float64."""
+        o = {"energy": torch.zeros(1, dtype=torch.float64, device="cuda"),
+             "virial": torch.zeros(9, dtype=torch.float64, device="cuda")}
+        self._s()
+        check(self._L.gse_pme_recip_scalar_sum(self._h, _recip9(recip), float(kappa),
+                                               int(bool(calc_energy)), int(bool(calc_virial)),
+                                               _ptr(o["energy"]), _ptr(o["virial"])))
+        return o
+
+    def gather_force(self, xyzq, ncoord, recip, stride=None):
+        """dE/dr (the reference's sign) as a (3, stride) AT tensor."""
+        _check_xyzq(xyzq, self.ct)
+        _check_count(ncoord, xyzq)
+        stride = ncoord if stride is None else stride
+        f = torch.zeros((3, max(stride, 1)), dtype=self.at, device=xyzq.device)
+        self._s()
+        check(self._L.gse_pme_recip_gather_force(self._h, _ptr(xyzq), ncoord, _recip9(recip),
+                                                 stride, _ptr(f)))
+        return f
+
+    def calc_self_energy(self, xyzq, ncoord, kappa):
+        _check_xyzq(xyzq, self.ct)
+        _check_count(ncoord, xyzq)
+        e = torch.zeros(1, dtype=torch.float64, device=xyzq.device)
+        self._s()
+        check(self._L.gse_pme_recip_calc_self_energy(self._h, _ptr(xyzq), ncoord, float(kappa),
+                                                     _ptr(e)))
+        return e
+
+    def grid(self):
+        """Copy of the handle's grid, (Z, Y, X) of CT."""
+        X, Y, Z = self.dims
+        out = torch.empty((Z, Y, X), dtype=self.ct, device="cuda")
+        p = ctypes.c_void_p()
+        check(self._L.gse_pme_recip_get_grid(self._h, ctypes.byref(p)))
+        torch.cuda.synchronize()
+        check(self._L.gse_memcpy_device_to_device(_ptr(out), p, out.numel() * out.element_size()))
+        return out
+
+    def step(self, xyzq, box, kappa, force=True, virial=True, out=None):
+        """spread -> r2c -> scalar_sum -> c2r -> gather: energy (1,), virial
+        (9,) and forces -dE/dr (3, N)."""
+        _check_xyzq(xyzq, self.ct)
+        n = xyzq.shape[0]
+        o = {} if out is None else out
+        o.setdefault("energy", torch.empty(1, dtype=torch.float64, device=xyzq.device))
+        if virial:
+            o.setdefault("virial", torch.empty(9, dtype=torch.float64, device=xyzq.device))
+        if force:
+            o.setdefault("force", torch.empty((3, n), dtype=self.at, device=xyzq.device))
+        f = o.get("force") if force else None
+        self._s()
+        check(self._L.gse_pme_recip_step(
+            self._h, _ptr(xyzq), n, *_box(box), float(kappa), _ptr(o["energy"]),
+            _ptr(o["virial"]) if virial else None, _ptr(f[0]) if f is not None else None,
+            _ptr(f[1]) if f is not None else None, _ptr(f[2]) if f is not None else None))
+        return o
+
+
+class CpuEwaldDirect:
+    """CpuEwaldDirect (CpuEwaldDirect.h:11-44) on the GPU, FP64: direct-space
+    erfc pair sum within rcut (minimum image) plus the self energy."""
+
+    def __init__(self, sigma, rcut, boxx, boxy, boxz, ccelec=332.0716):
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        check(L.gse_ewald_direct_create(ctypes.byref(h), float(sigma), float(rcut), float(boxx),
+                                        float(boxy), float(boxz), float(ccelec)))
+        self._h = h
+        self._L = L
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.gse_ewald_direct_destroy(self._h)
+            self._h = None
+
+    def setupExclusions(self, n_iblo14, iblo14, inb14):
+        import numpy as np
+        ib = np.ascontiguousarray(iblo14, dtype=np.int32)
+        nb = np.ascontiguousarray(inb14 if len(inb14) else [0], dtype=np.int32)
+        if ib.size < n_iblo14:
+            raise ValueError("iblo14 needs n_iblo14 entries")
+        self._excl = (ib, nb)
+        ip = ctypes.POINTER(ctypes.c_int)
+        check(self._L.gse_ewald_direct_setup_exclusions(self._h, int(n_iblo14),
+                                                        ib.ctypes.data_as(ip),
+                                                        nb.ctypes.data_as(ip)))
+
+    def calcForceEnergy(self, n, xyzq, forceX, forceY, forceZ):
+        """Writes dE/dr (the reference's sign) into the three float64 device
+        arrays and returns E_direct + E_self (synchronises)."""
+        _check_xyzq(xyzq, torch.float64)
+        _check_count(n, xyzq)
+        for f in (forceX, forceY, forceZ):
+            if f.dtype != torch.float64 or f.numel() < n:
+                raise ValueError("force arrays must be float64 with n entries")
+        e = torch.empty(1, dtype=torch.float64, device=xyzq.device)
+        check(self._L.gse_ewald_direct_set_stream(self._h, _stream()))
+        check(self._L.gse_ewald_direct_calc_force_energy(self._h, n, _ptr(xyzq), _ptr(forceX),
+                                                         _ptr(forceY), _ptr(forceZ), _ptr(e)))
+        return float(e.cpu()[0])

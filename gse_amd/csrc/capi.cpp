@@ -423,6 +423,10 @@ struct FFT3 {
 };
 
 }  // namespace
+// rocFFT pair of one SPME handle (the plans come from the shared cache)
+struct KSolverFFTOnly {
+  FFT3 fft;
+};
 struct gse_gsek_s;
 static gse_status_t stage_mark(gse_gsek_s* h);
 namespace {
@@ -614,6 +618,26 @@ struct gse_batch_s {
   }
 };
 
+// SPME reciprocal comparator (cuda/CudaPMERecip.h:138-180, BOX FFT, one GPU)
+struct gse_pme_recip_s {
+  bool at_dbl = true, ct_dbl = true;
+  int X = 0, Y = 0, Z = 0, order = 4;
+  double recip3[3] = {0, 0, 0};  // of the last spread (gather and scalar_sum reuse it)
+  int n_last = 0;
+  KSolverFFTOnly* fft = nullptr;
+  DevBuf grid64, grid, spec, prefac, partials, stats, tmp2;
+  hipStream_t stream = nullptr;
+};
+
+// Ewald direct space + self energy (CpuEwaldDirect.h:11-44)
+struct gse_ewald_direct_s {
+  double sigma = 0, rcut = 0, ccelec = 332.0716, box[3] = {0, 0, 0};
+  DevBuf xofs, xlist, cell_of, counts, starts, fill, tmp, sidx, sx, partials, self2, mom;
+  bool have_excl = false;
+  int n_excl_atoms = 0;
+  hipStream_t stream = nullptr;
+};
+
 namespace {
 
 // separable sigma taps of spreadChargeOnGrid (CpuGaussCharge.cpp:406-433)
@@ -765,6 +789,26 @@ gse_status_t gse_memcpy_to_host(void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return GSE_SUCCESS;
   if (!dst || !src) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
   HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_memcpy_device_to_device(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return GSE_SUCCESS;
+  if (!dst || !src) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_device_clear(void* ptr, size_t bytes) {
+  if (bytes == 0) return GSE_SUCCESS;
+  if (!ptr) return fail(GSE_ERROR_INVALID_VALUE, "ptr is NULL");
+  HIPCHK(hipMemsetAsync(ptr, 0, bytes, nullptr));
+  HIPCHK(hipStreamSynchronize(nullptr));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_device_synchronize(void) {
+  HIPCHK(hipDeviceSynchronize());
   return GSE_SUCCESS;
 }
 
@@ -1442,6 +1486,379 @@ gse_status_t gse_gsek_corrections(gse_gsek_t h, int n, const void* xyzq, double*
   HIPCHK(launch_charge_moments(h->ct_dbl, xyzq, n, h->part.as<double>(), h->stream));
   HIPCHK(launch_corrections_finalize(h->part.as<double>(), kRedBlocks, h->sigma, h->bx, h->by,
                                      h->bz, h->ccelec, out, h->stream));
+  return GSE_SUCCESS;
+}
+
+// ------------------------------- SPME (CudaPMERecip) ----------------------
+}  // extern "C"
+
+namespace {
+
+// calc_prefac / dftmod (CudaPMERecip.cu:3385-3478): 1/|sum_j M_n(j) e^{2 pi i k j/M}|^2
+// per axis, with the reference's replacement of a vanishing modulus by the
+// mean of its neighbours; FP64 on the host
+void pme_moduli(int order, int nfft, double* out) {
+  double th[16];
+  th[order - 1] = 0.0;
+  th[1] = 0.0;
+  th[0] = 1.0;
+  for (int k = 3; k <= order - 1; k++) {
+    const double div = 1.0 / (double)(k - 1);
+    th[k - 1] = 0.0;
+    for (int j = 1; j <= k - 2; j++) th[k - j - 1] = div * (j * th[k - j - 2] + (k - j) * th[k - j - 1]);
+    th[0] = div * th[0];
+  }
+  {
+    const double div = 1.0 / (double)(order - 1);
+    th[order - 1] = 0.0;
+    for (int j = 1; j <= order - 2; j++)
+      th[order - j - 1] = div * (j * th[order - j - 2] + (order - j) * th[order - j - 1]);
+    th[0] = div * th[0];
+  }
+  std::vector<double> arr((size_t)nfft + order + 2, 0.0);
+  for (int i = 2; i <= order + 1; i++) arr[i - 1] = th[i - 2];
+  const double r = 2.0 * M_PI / (double)nfft;
+  for (int k = 1; k <= nfft; k++) {
+    double s1 = 0.0, s2 = 0.0;
+    const double a1 = (k - 1) * r;
+    for (int j = 1; j < nfft; j++) {
+      const double a = a1 * (j - 1);
+      s1 += arr[j - 1] * std::cos(a);
+      s2 += arr[j - 1] * std::sin(a);
+    }
+    out[k - 1] = s1 * s1 + s2 * s2;
+  }
+  for (int k = 1; k <= nfft; k++)
+    if (out[k - 1] < 1.0e-10) out[k - 1] = 0.5 * (out[(k - 2 + nfft) % nfft] + out[k % nfft]);
+  for (int k = 0; k < nfft; k++) out[k] = 1.0 / out[k];
+}
+
+gse_status_t pme_recip3(const double* recip, double* r3) {
+  if (!recip) return fail(GSE_ERROR_INVALID_VALUE, "recip is NULL");
+  if (recip[1] != 0.0 || recip[2] != 0.0 || recip[3] != 0.0 || recip[5] != 0.0 ||
+      recip[6] != 0.0 || recip[7] != 0.0)
+    return fail(GSE_ERROR_NOT_SUPPORTED, "only orthorhombic boxes are supported (as the reference)");
+  if (!(recip[0] > 0) || !(recip[4] > 0) || !(recip[8] > 0) || !std::isfinite(recip[0]) ||
+      !std::isfinite(recip[4]) || !std::isfinite(recip[8]))
+    return fail(GSE_ERROR_INVALID_VALUE, "recip diagonal must be positive and finite");
+  r3[0] = recip[0];
+  r3[1] = recip[4];
+  r3[2] = recip[8];
+  return GSE_SUCCESS;
+}
+
+// the int64 -> grid conversion reads max |q| and the count from a Binning
+Binning pme_binning(gse_pme_recip_s* h, int n) {
+  Binning w;
+  w.n = n;
+  w.stats = h->stats.as<BinStats>();
+  w.images = 1;
+  w.pref_bound = 1.0;
+  return w;
+}
+
+constexpr double kPmeCcelec = 332.0716;  // CudaPMERecip.h:25
+
+}  // namespace
+
+extern "C" {
+
+gse_status_t gse_pme_recip_create(gse_pme_recip_t* out, gse_dtype_t at, gse_dtype_t ct,
+                                  int nfftx, int nffty, int nfftz, int order) {
+  if (!out) return fail(GSE_ERROR_INVALID_VALUE, "out is NULL");
+  *out = nullptr;
+  if (!valid_pair(at, ct)) return fail(GSE_ERROR_INVALID_VALUE, "unsupported (AT, CT) pair");
+  GSECHK(check_dims(nfftx, nffty, nfftz));
+  if (order != 4 && order != 6 && order != 8)
+    return fail(GSE_ERROR_INVALID_VALUE, "order must be 4, 6 or 8 (CudaPMERecip.cu:2841-2893)");
+  if (nfftx < order || nffty < order || nfftz < order)
+    return fail(GSE_ERROR_INVALID_VALUE, "every grid dimension must be at least the order");
+  auto* h = new gse_pme_recip_s();
+  h->at_dbl = at == GSE_FLOAT64;
+  h->ct_dbl = ct == GSE_FLOAT64;
+  h->X = nfftx;
+  h->Y = nffty;
+  h->Z = nfftz;
+  h->order = order;
+  h->fft = new KSolverFFTOnly();
+  const size_t np = (size_t)nfftx * nffty * nfftz;
+  const size_t nk = (size_t)(nfftx / 2 + 1) * nffty * nfftz;
+  gse_status_t st = h->fft->fft.init(h->ct_dbl, nfftx, nffty, nfftz);
+  if (st == GSE_SUCCESS) st = h->grid64.ensure(8 * np);
+  if (st == GSE_SUCCESS) st = h->grid.ensure(tsize(h->ct_dbl) * np);
+  if (st == GSE_SUCCESS) st = h->spec.ensure(2 * tsize(h->ct_dbl) * nk);
+  if (st == GSE_SUCCESS) st = h->prefac.ensure(sizeof(double) * (size_t)(nfftx + nffty + nfftz));
+  if (st == GSE_SUCCESS) st = h->partials.ensure(sizeof(double) * 7 * kPmeSumBlocks);
+  if (st == GSE_SUCCESS) st = h->stats.ensure(sizeof(BinStats));
+  if (st == GSE_SUCCESS) st = h->tmp2.ensure(sizeof(double) * (kRedBlocks * 2 + 2));
+  if (st == GSE_SUCCESS) {
+    std::vector<double> pf((size_t)nfftx + nffty + nfftz);
+    pme_moduli(order, nfftx, pf.data());
+    pme_moduli(order, nffty, pf.data() + nfftx);
+    pme_moduli(order, nfftz, pf.data() + nfftx + nffty);
+    hipError_t e = hipMemcpy(h->prefac.p, pf.data(), sizeof(double) * pf.size(),
+                             hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_fill_u64(h->grid64.as<unsigned long long>(), np, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) st = fail(GSE_ERROR_HIP, "SPME set-up failed: %s", hipGetErrorString(e));
+  }
+  if (st != GSE_SUCCESS) {
+    gse_pme_recip_destroy(h);
+    return st;
+  }
+  *out = h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_pme_recip_destroy(gse_pme_recip_t h) {
+  if (!h) return GSE_SUCCESS;
+  if (h->fft) {
+    h->fft->fft.release();
+    delete h->fft;
+  }
+  for (DevBuf* b : {&h->grid64, &h->grid, &h->spec, &h->prefac, &h->partials, &h->stats, &h->tmp2})
+    b->release();
+  delete h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_pme_recip_set_stream(gse_pme_recip_t h, void* s) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  h->stream = (hipStream_t)s;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_pme_recip_spread_charge(gse_pme_recip_t h, const void* xyzq, int ncoord,
+                                         const double* recip) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  if (ncoord < 0 || (ncoord > 0 && !xyzq)) return fail(GSE_ERROR_INVALID_VALUE, "bad particles");
+  double r3[3];
+  GSECHK(pme_recip3(recip, r3));
+  std::memcpy(h->recip3, r3, sizeof(r3));
+  h->n_last = ncoord;
+  hipStream_t s = h->stream;
+  HIPCHK(launch_pme_qmax(h->ct_dbl, xyzq, ncoord, r3, h->stats.as<BinStats>(), s));
+  HIPCHK(launch_pme_spread(h->ct_dbl, h->order, xyzq, ncoord, h->X, h->Y, h->Z, r3,
+                           h->stats.as<BinStats>(), h->grid64.as<unsigned long long>(), s));
+  const Binning w = pme_binning(h, ncoord);
+  HIPCHK(launch_fixed_to_grid(h->ct_dbl, h->grid64.as<unsigned long long>(), h->grid.p,
+                              (size_t)h->X * h->Y * h->Z, w, true, s));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_pme_recip_r2c_fft(gse_pme_recip_t h) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  return h->fft->fft.exec(true, h->grid.p, h->spec.p, h->stream);
+}
+
+gse_status_t gse_pme_recip_scalar_sum(gse_pme_recip_t h, const double* recip, double kappa,
+                                      int calc_energy, int calc_virial, double* energy,
+                                      double* virial) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  if (!(kappa > 0) || !std::isfinite(kappa)) return fail(GSE_ERROR_INVALID_VALUE, "kappa > 0");
+  if ((calc_energy && !energy) || (calc_virial && !virial))
+    return fail(GSE_ERROR_INVALID_VALUE, "energy/virial output is NULL");
+  double r3[3];
+  GSECHK(pme_recip3(recip, r3));
+  const bool want = calc_energy || calc_virial;
+  HIPCHK(launch_pme_scalar_sum(h->ct_dbl, h->X, h->Y, h->Z, r3, kappa, h->prefac.as<double>(),
+                               h->spec.p, want, h->partials.as<double>(), h->stream));
+  if (want)
+    HIPCHK(launch_pme_finalize(h->partials.as<double>(), kPmeCcelec, calc_energy ? energy : nullptr,
+                               calc_virial ? virial : nullptr, h->stream));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_pme_recip_c2r_fft(gse_pme_recip_t h) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  return h->fft->fft.exec(false, h->spec.p, h->grid.p, h->stream);
+}
+
+gse_status_t gse_pme_recip_gather_force(gse_pme_recip_t h, const void* xyzq, int ncoord,
+                                        const double* recip, int stride, void* force) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  if (ncoord < 0 || (ncoord > 0 && (!xyzq || !force)))
+    return fail(GSE_ERROR_INVALID_VALUE, "bad particles or force array");
+  if (stride < ncoord) return fail(GSE_ERROR_INVALID_VALUE, "stride must be >= ncoord");
+  double r3[3];
+  GSECHK(pme_recip3(recip, r3));
+  const size_t as = tsize(h->at_dbl);
+  char* f = (char*)force;
+  HIPCHK(launch_pme_gather(h->ct_dbl, h->at_dbl, h->order, xyzq, ncoord, h->X, h->Y, h->Z, r3,
+                           kPmeCcelec, 1.0, h->grid.p, f, f + as * (size_t)stride,
+                           f + 2 * as * (size_t)stride, h->stream));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_pme_recip_calc_self_energy(gse_pme_recip_t h, const void* xyzq, int ncoord,
+                                            double kappa, double* energy) {
+  if (!h || !energy) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  if (ncoord < 0 || (ncoord > 0 && !xyzq)) return fail(GSE_ERROR_INVALID_VALUE, "bad particles");
+  if (!(kappa > 0)) return fail(GSE_ERROR_INVALID_VALUE, "kappa > 0");
+  double* t = h->tmp2.as<double>();
+  HIPCHK(launch_charge_moments(h->ct_dbl, xyzq, ncoord, t, h->stream));
+  // -kappa ccelec/sqrt(pi) sum q^2 (CudaPMERecip.cu:2359-2380) is the
+  // Gaussian self term with sigma = 1/(sqrt(2) kappa)
+  HIPCHK(launch_corrections_finalize(t, kRedBlocks, 1.0 / (std::sqrt(2.0) * kappa), 1.0, 1.0, 1.0,
+                                     kPmeCcelec, t + 2 * kRedBlocks, h->stream));
+  HIPCHK(hipMemcpyAsync(energy, t + 2 * kRedBlocks, sizeof(double), hipMemcpyDeviceToDevice,
+                        h->stream));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_pme_recip_step(gse_pme_recip_t h, const void* xyzq, int ncoord, double boxx,
+                                double boxy, double boxz, double kappa, double* energy,
+                                double* virial, void* fx, void* fy, void* fz) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  GSECHK(check_box(boxx, boxy, boxz));
+  if ((fx || fy || fz) && !(fx && fy && fz))
+    return fail(GSE_ERROR_INVALID_VALUE, "fx/fy/fz must be all set or all NULL");
+  const double recip[9] = {1.0 / boxx, 0, 0, 0, 1.0 / boxy, 0, 0, 0, 1.0 / boxz};
+  GSECHK(gse_pme_recip_spread_charge(h, xyzq, ncoord, recip));
+  GSECHK(gse_pme_recip_r2c_fft(h));
+  GSECHK(gse_pme_recip_scalar_sum(h, recip, kappa, energy != nullptr, virial != nullptr, energy,
+                                  virial));
+  if (fx) {
+    GSECHK(gse_pme_recip_c2r_fft(h));
+    HIPCHK(launch_pme_gather(h->ct_dbl, h->at_dbl, h->order, xyzq, ncoord, h->X, h->Y, h->Z,
+                             h->recip3, kPmeCcelec, -1.0, h->grid.p, fx, fy, fz, h->stream));
+  }
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_pme_recip_get_grid(gse_pme_recip_t h, void** grid) {
+  if (!h || !grid) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  *grid = h->grid.p;
+  return GSE_SUCCESS;
+}
+
+// ------------------------------- Ewald direct (CpuEwaldDirect) --------------
+gse_status_t gse_ewald_direct_create(gse_ewald_direct_t* out, double sigma, double rcut,
+                                     double boxx, double boxy, double boxz, double ccelec) {
+  if (!out) return fail(GSE_ERROR_INVALID_VALUE, "out is NULL");
+  *out = nullptr;
+  GSECHK(check_box(boxx, boxy, boxz));
+  GSECHK(check_sigma_rcut(sigma, rcut));
+  auto* h = new gse_ewald_direct_s();
+  h->sigma = sigma;
+  h->rcut = rcut;
+  h->ccelec = ccelec;
+  h->box[0] = boxx;
+  h->box[1] = boxy;
+  h->box[2] = boxz;
+  *out = h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_ewald_direct_destroy(gse_ewald_direct_t h) {
+  if (!h) return GSE_SUCCESS;
+  for (DevBuf* b : {&h->xofs, &h->xlist, &h->cell_of, &h->counts, &h->starts, &h->fill, &h->tmp,
+                    &h->sidx, &h->sx, &h->partials, &h->self2, &h->mom})
+    b->release();
+  delete h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_ewald_direct_set_stream(gse_ewald_direct_t h, void* s) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  h->stream = (hipStream_t)s;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_ewald_direct_setup_exclusions(gse_ewald_direct_t h, int n_iblo14,
+                                               const int* iblo14, const int* inb14) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  if (n_iblo14 < 0 || (n_iblo14 > 0 && (!iblo14 || !inb14)))
+    return fail(GSE_ERROR_INVALID_VALUE, "bad exclusion arrays");
+  if (n_iblo14 == 0) {
+    h->have_excl = false;
+    h->n_excl_atoms = 0;
+    return GSE_SUCCESS;
+  }
+  // CpuEwaldDirect.cpp:31-40: atom j excludes inb14[iblo14[j-1] .. iblo14[j]) (1-based,
+  // sign ignored) among the atoms after it; both ends of each pair get it here
+  const int ntot = iblo14[n_iblo14 - 1];
+  std::vector<std::vector<int>> lists(n_iblo14);
+  int prev = 0;
+  for (int j = 0; j < n_iblo14; j++) {
+    const int end = iblo14[j];
+    if (end < prev || end > ntot)
+      return fail(GSE_ERROR_INVALID_VALUE, "iblo14 must be non-decreasing running ends");
+    for (int k = prev; k < end; k++) {
+      const int i = std::abs(inb14[k]) - 1;
+      if (i < 0 || i >= n_iblo14) return fail(GSE_ERROR_INVALID_VALUE, "inb14 atom out of range");
+      if (i > j) {  // the reference only tests partners after j
+        lists[j].push_back(i);
+        lists[i].push_back(j);
+      }
+    }
+    prev = end;
+  }
+  std::vector<int> ofs(n_iblo14 + 1, 0), flat;
+  for (int j = 0; j < n_iblo14; j++) {
+    std::sort(lists[j].begin(), lists[j].end());
+    lists[j].erase(std::unique(lists[j].begin(), lists[j].end()), lists[j].end());
+    flat.insert(flat.end(), lists[j].begin(), lists[j].end());
+    ofs[j + 1] = (int)flat.size();
+  }
+  GSECHK(h->xofs.ensure(sizeof(int) * ofs.size()));
+  GSECHK(h->xlist.ensure(sizeof(int) * std::max<size_t>(1, flat.size())));
+  HIPCHK(hipMemcpy(h->xofs.p, ofs.data(), sizeof(int) * ofs.size(), hipMemcpyHostToDevice));
+  if (!flat.empty())
+    HIPCHK(hipMemcpy(h->xlist.p, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice));
+  h->have_excl = true;
+  h->n_excl_atoms = n_iblo14;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_ewald_direct_calc_force_energy(gse_ewald_direct_t h, int n, const void* xyzq,
+                                                double* fx, double* fy, double* fz,
+                                                double* energy) {
+  if (!h || !energy) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  if (n < 0 || (n > 0 && (!xyzq || !fx || !fy || !fz)))
+    return fail(GSE_ERROR_INVALID_VALUE, "bad particle or force arrays");
+  if (h->have_excl && h->n_excl_atoms != n)  // CpuEwaldDirect.cpp:47
+    return fail(GSE_ERROR_INVALID_VALUE, "the exclusion list covers %d atoms, the call %d",
+                h->n_excl_atoms, n);
+  EwCells c;
+  c.ncell = 1;
+  for (int a = 0; a < 3; a++) {
+    c.box[a] = h->box[a];
+    int nc = (int)std::floor(h->box[a] / h->rcut);
+    if (nc > 128) nc = 128;
+    c.nc[a] = nc >= 3 ? nc : 1;
+    c.ncell *= c.nc[a];
+  }
+  EwPair p;
+  p.rcut2 = h->rcut * h->rcut;
+  p.ccelec = h->ccelec;
+  p.inv_sqrt2sigma = 1.0 / (std::sqrt(2.0) * h->sigma);
+  p.inv_2sigmasq = 1.0 / (2.0 * h->sigma * h->sigma);
+  p.two_sqrtpi = 2.0 / std::sqrt(M_PI);
+  const size_t n1 = (size_t)(n > 0 ? n : 1);
+  GSECHK(h->cell_of.ensure(sizeof(int) * n1));
+  GSECHK(h->tmp.ensure(sizeof(int) * n1));
+  GSECHK(h->sidx.ensure(sizeof(int) * n1));
+  GSECHK(h->sx.ensure(4 * sizeof(double) * n1));
+  GSECHK(h->counts.ensure(sizeof(int) * c.ncell));
+  GSECHK(h->starts.ensure(sizeof(int) * c.ncell));
+  GSECHK(h->fill.ensure(sizeof(int) * c.ncell));
+  GSECHK(h->partials.ensure(sizeof(double) * c.ncell));
+  GSECHK(h->mom.ensure(sizeof(double) * kRedBlocks * 2));
+  GSECHK(h->self2.ensure(sizeof(double) * 2));
+  hipStream_t s = h->stream;
+  HIPCHK(launch_ew_cells(xyzq, n, c, h->cell_of.as<int>(), h->counts.as<int>(),
+                         h->starts.as<int>(), h->fill.as<int>(), h->tmp.as<int>(),
+                         h->sidx.as<int>(), h->sx.p, s));
+  HIPCHK(launch_ew_force(c, p, h->starts.as<int>(), h->counts.as<int>(), h->sidx.as<int>(),
+                         h->sx.p, h->have_excl ? h->xofs.as<int>() : nullptr,
+                         h->have_excl ? h->xlist.as<int>() : nullptr, fx, fy, fz,
+                         h->partials.as<double>(), s));
+  // self energy -ccelec sum q^2 / sqrt(2 pi sigma^2) (CpuEwaldDirect.cpp:124-128)
+  HIPCHK(launch_charge_moments(true, xyzq, n, h->mom.as<double>(), s));
+  HIPCHK(launch_corrections_finalize(h->mom.as<double>(), kRedBlocks, h->sigma, 1.0, 1.0, 1.0,
+                                     h->ccelec, h->self2.as<double>(), s));
+  HIPCHK(launch_ew_finalize(h->partials.as<double>(), c.ncell, h->self2.as<double>(), energy, s));
   return GSE_SUCCESS;
 }
 

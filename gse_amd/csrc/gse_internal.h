@@ -350,4 +350,59 @@ hipError_t launch_conv_pass(bool dbl, int pass, int X, int Y, int Z, int R,
 hipError_t launch_charge_moments(bool dbl, const void* xyzq, int n,
                                  double* partials, hipStream_t s);
 
+// ---- SPME reciprocal comparator (ewald_pme.hip) -------------------------
+// recip3: 1/Lx, 1/Ly, 1/Lz.  stats: max |q| and the rejected count of the
+// call (cleared here), read by the spread and the int64 -> grid conversion.
+constexpr int kPmeSumBlocks = 512;
+hipError_t launch_pme_qmax(bool dbl, const void* xyzq, int n, const double* recip3,
+                           BinStats* stats, hipStream_t s);
+hipError_t launch_pme_spread(bool dbl, int order, const void* xyzq, int n, int X, int Y, int Z,
+                             const double* recip3, const BinStats* stats,
+                             unsigned long long* g64, hipStream_t s);
+// prefac: device [X + Y + Z] B-spline moduli; partials [kPmeSumBlocks][7]
+hipError_t launch_pme_scalar_sum(bool dbl, int X, int Y, int Z, const double* recip3,
+                                 double kappa, const double* prefac, void* spec, bool want,
+                                 double* partials, hipStream_t s);
+hipError_t launch_pme_finalize(const double* partials, double ccelec, double* energy,
+                               double* virial, hipStream_t s);
+// out{x,y,z}[i] = sign * dE/dr_i (AT); invalid particles get 0
+hipError_t launch_pme_gather(bool ct_dbl, bool at_dbl, int order, const void* xyzq, int n, int X,
+                             int Y, int Z, const double* recip3, double ccelec, double sign,
+                             const void* Qc, void* ox, void* oy, void* oz, hipStream_t s);
+
+// ---- Ewald direct space (ewald_pme.hip) ----------------------------------
+struct EwCells {
+  int nc[3];      // cells per axis (1 when fewer than 3 cells of edge >= rcut fit)
+  int ncell;
+  double box[3];
+};
+struct EwPair {
+  double rcut2, ccelec, inv_sqrt2sigma, inv_2sigmasq, two_sqrtpi;
+};
+// cell lists: counts/starts/fill [ncell], cell_of/tmp/sidx [n], sx [n] double4
+hipError_t launch_ew_cells(const void* xyzq, int n, const EwCells& c, int* cell_of, int* counts,
+                           int* starts, int* fill, int* tmp, int* sidx, void* sx,
+                           hipStream_t s);
+// xofs/xlist: symmetric exclusion lists (CSR over particles; xofs NULL: none)
+hipError_t launch_ew_force(const EwCells& c, const EwPair& p, const int* starts,
+                           const int* counts, const int* sidx, const void* sx, const int* xofs,
+                           const int* xlist, double* gx, double* gy, double* gz,
+                           double* partials, hipStream_t s);
+hipError_t launch_ew_finalize(const double* partials, int ncell, const double* self2,
+                              double* energy, hipStream_t s);
+
+// ---- Ewald reciprocal sum (ewald_pme.hip) --------------------------------
+struct ErArgs {
+  int kcut, ncube;     // ncube = (2 kcut + 1)^3
+  double ib[3];        // 1 / box
+  double pref;         // 2 pi ccelec / V
+  double half_sigmasq;
+};
+int ewald_recip_blocks(const ErArgs& a);
+// sk: [ncube][3] (Re S, Im S, pek); partials: [ewald_recip_blocks]; forces
+// -dE/dr (fx NULL: energy only), energy: device double
+hipError_t launch_ewald_recip(const ErArgs& a, const void* xyzq, int n, double* sk,
+                              double* partials, double* fx, double* fy, double* fz,
+                              double* energy, hipStream_t s);
+
 }  // namespace gse

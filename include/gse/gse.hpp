@@ -12,14 +12,25 @@
 //   CpuFFTSolver<T>         CpuFFTSolver.h:9-43
 //   CpuGSE<AT,CT>, CpuGSEk<AT,CT>   CpuGSE.h:18-68, CpuGSEk.h:18-37
 //   xyzq_t<T>               TypesGSE.h:20-22
+// and the reference's GPU classes, over device memory:
+//   CudaGrid<T>             CudaGrid.h:14-42
+//   CudaGaussCharge<AT,CT>  CudaGaussCharge.h:11-33
+//   CudaConvolution<T>      CudaConvolution.h:9-37
+// and the comparators the reference's driver runs beside GSE:
+//   CpuEwaldDirect          CpuEwaldDirect.h:11-44 (host arrays)
+//   CudaPMERecip<AT,CT>     cuda/CudaPMERecip.h:138-180 (device arrays)
 //
-// Memory model.  As in the reference, grids and particle/force arrays are
-// HOST memory owned by the caller (CpuGrid allocates with new[]).  Each call
+// Memory model.  As in the reference, the Cpu* classes work on HOST memory
+// owned by the caller (CpuGrid allocates with new[]).  Each call
 // copies its inputs to device buffers that the objects keep between calls,
 // runs the HIP kernels and copies its outputs back; the calls therefore
 // include PCIe transfers and return with the results in host memory.
-// Callers that keep data resident in HBM call include/gse_hip.h directly
-// with device pointers.
+// The Cuda* classes work on DEVICE memory as the reference's do: CudaGrid
+// owns a device array, CudaGaussCharge reads device particles, and results
+// stay on the device until CpuGrid::copy(const CudaGrid&) brings them back.
+// Their calls are ordered on the null stream and return when the device work
+// is done, like the reference's synchronous CUDA calls.  Callers that manage
+// their own streams call include/gse_hip.h directly with device pointers.
 //
 // Differences from the reference, all on error paths or outside the path:
 //   * failures throw gse_amd::Error (status + message) instead of assert()
@@ -124,6 +135,9 @@ class Grid {
   int getSize() const { return size; }
 };
 
+template <typename T>
+class CudaGrid;
+
 // CpuGrid.h:17-106: host grid, x fastest.  The device mirror is a staging
 // buffer used by the classes below.
 template <typename T>
@@ -153,6 +167,8 @@ class CpuGrid : public Grid<T> {
     const T2* g = grid.getDataPointer();
     for (int i = 0; i < this->size; i++) this->data[i] = (T)g[i];
   }
+  // CpuGrid.h:47-50: device -> host, synchronous
+  void copy(const CudaGrid<T>& d_grid);
   void scale(const T val) {
     for (int i = 0; i < this->size; i++) this->data[i] *= val;
   }
@@ -422,6 +438,208 @@ class CpuFFTSolver {
   CpuGreensFunc<T>& greensFunc;
   CpuGrid<T> rhoK;
   gse_fft_solver_t h_ = nullptr;
+};
+
+// CudaGrid.h:14-42: a grid in device memory (x fastest), allocated by the
+// constructor and freed by the destructor.
+template <typename T>
+class CudaGrid : public Grid<T> {
+ public:
+  CudaGrid(const int sizeX, const int sizeY, const int sizeZ) : Grid<T>(sizeX, sizeY, sizeZ) {
+    void* p = nullptr;
+    check(gse_device_malloc(&p, bytes()));
+    this->data = (T*)p;
+  }
+  ~CudaGrid() { (void)gse_device_free(this->data); }
+  CudaGrid(const CudaGrid&) = delete;
+  CudaGrid& operator=(const CudaGrid&) = delete;
+
+  // CudaGrid.h:28-30
+  void clear() { check(gse_device_clear(this->data, bytes())); }
+  // CudaGrid.h:32-35: host -> device, synchronous
+  void copy(const CpuGrid<T>& h_grid) {
+    detail::check_dims(h_grid, this->sizeX, this->sizeY, this->sizeZ);
+    check(gse_memcpy_to_device(this->data, h_grid.getDataPointer(), bytes()));
+  }
+  // CudaGrid.h:37-40: device -> device, synchronous
+  void copy(const CudaGrid<T>& d_grid) {
+    detail::check_dims(d_grid, this->sizeX, this->sizeY, this->sizeZ);
+    check(gse_memcpy_device_to_device(this->data, d_grid.getDataPointer(), bytes()));
+  }
+  size_t bytes() const { return sizeof(T) * (size_t)this->size; }
+};
+
+template <typename T>
+void CpuGrid<T>::copy(const CudaGrid<T>& d_grid) {
+  detail::check_dims(d_grid, this->sizeX, this->sizeY, this->sizeZ);
+  check(gse_memcpy_to_host(this->data, d_grid.getDataPointer(), bytes()));
+}
+
+// CudaGaussCharge.h:11-33: sigma1 spread of DEVICE particles into a device
+// grid (CudaGaussCharge.cu:255-301).  The reference spreads with float
+// atomics; this spread is the fixed-point one of CpuGaussCharge, so the grid
+// is bit-identical run to run and equals the CPU spread to rounding.
+template <typename AT, typename CT>
+class CudaGaussCharge {
+ public:
+  CudaGaussCharge(const int sizeX, const int sizeY, const int sizeZ)
+      : sizeX(sizeX), sizeY(sizeY), sizeZ(sizeZ) {
+    check(gse_gauss_charge_create(&h_, dtype_of<AT>::value, dtype_of<CT>::value, sizeX, sizeY,
+                                  sizeZ));
+  }
+  ~CudaGaussCharge() { (void)gse_gauss_charge_destroy(h_); }
+  CudaGaussCharge(const CudaGaussCharge&) = delete;
+  CudaGaussCharge& operator=(const CudaGaussCharge&) = delete;
+
+  // CudaGaussCharge.h:29-32; xyzq is a device array of numCoord records
+  void spreadChargeToGrid(const double sigma, const double rcut, const int numCoord,
+                          const xyzq_t<CT>* xyzq, const double boxx, const double boxy,
+                          const double boxz, CudaGrid<CT>& rho) {
+    detail::check_dims(rho, sizeX, sizeY, sizeZ);
+    check(gse_gauss_charge_spread_charge_to_grid(h_, sigma, rcut, numCoord, xyzq, boxx, boxy,
+                                                 boxz, rho.getDataPointer(), 0));
+    check(gse_device_synchronize());
+  }
+
+  gse_gauss_charge_t handle() const { return h_; }
+
+ private:
+  const int sizeX, sizeY, sizeZ;
+  gse_gauss_charge_t h_ = nullptr;
+};
+
+// CudaConvolution.h:9-37: separable convolution of device grids.  run() is
+// the X pass followed by the copy into dataXYZ, as CudaConvolution.cu:115-150
+// does; runXYZ() (not in the reference class) runs all three passes.
+template <typename T>
+class CudaConvolution {
+ public:
+  CudaConvolution(const int sizeX, const int sizeY, const int sizeZ)
+      : sizeX(sizeX), sizeY(sizeY), sizeZ(sizeZ) {
+    check(gse_convolution_create(&h_, dtype_of<T>::value, sizeX, sizeY, sizeZ));
+  }
+  ~CudaConvolution() { (void)gse_convolution_destroy(h_); }
+  CudaConvolution(const CudaConvolution&) = delete;
+  CudaConvolution& operator=(const CudaConvolution&) = delete;
+
+  // CudaConvolution.h:33-34: the taps are HOST arrays of 2R+1 values
+  void setupKernel(const int kernelRadiusX, const int kernelRadiusY, const int kernelRadiusZ,
+                   const T* h_kernelX, const T* h_kernelY, const T* h_kernelZ) {
+    check(gse_convolution_setup_kernel(h_, kernelRadiusX, kernelRadiusY, kernelRadiusZ,
+                                       h_kernelX, h_kernelY, h_kernelZ));
+  }
+  // CudaConvolution.h:35
+  void run(const CudaGrid<T>& data, CudaGrid<T>& dataXYZ) {
+    detail::check_dims(data, sizeX, sizeY, sizeZ);
+    detail::check_dims(dataXYZ, sizeX, sizeY, sizeZ);
+    check(gse_convolution_run(h_, data.getDataPointer(), dataXYZ.getDataPointer()));
+    check(gse_device_synchronize());
+  }
+  void runXYZ(const CudaGrid<T>& data, CudaGrid<T>& dataXYZ) {
+    detail::check_dims(data, sizeX, sizeY, sizeZ);
+    detail::check_dims(dataXYZ, sizeX, sizeY, sizeZ);
+    check(gse_convolution_run_xyz(h_, data.getDataPointer(), dataXYZ.getDataPointer()));
+    check(gse_device_synchronize());
+  }
+
+ private:
+  const int sizeX, sizeY, sizeZ;
+  gse_convolution_t h_ = nullptr;
+};
+
+// CpuEwaldDirect.h:11-44: direct-space erfc sum within rcut plus the self
+// energy, FP64.  Host particle and force arrays, as the reference; the
+// forces it returns are dE/dr (CpuEwaldDirect.cpp:107-117).
+class CpuEwaldDirect {
+ public:
+  CpuEwaldDirect(const double sigma, const double rcut, const double boxx, const double boxy,
+                 const double boxz, const double ccelec = 332.0716) {
+    check(gse_ewald_direct_create(&h_, sigma, rcut, boxx, boxy, boxz, ccelec));
+  }
+  ~CpuEwaldDirect() { (void)gse_ewald_direct_destroy(h_); }
+  CpuEwaldDirect(const CpuEwaldDirect&) = delete;
+  CpuEwaldDirect& operator=(const CpuEwaldDirect&) = delete;
+
+  // CpuEwaldDirect.h:39
+  void setupExclusions(const int n_iblo14_in, const int* iblo14_in, const int* inb14_in) {
+    check(gse_ewald_direct_setup_exclusions(h_, n_iblo14_in, iblo14_in, inb14_in));
+  }
+  // CpuEwaldDirect.h:41-42
+  double calcForceEnergy(const int n, const xyzq_t<double>* xyzq, double* fx, double* fy,
+                         double* fz) {
+    const void* dx = detail::upload_xyzq(xyzq_, n, xyzq);
+    const size_t fb = sizeof(double) * (size_t)(n > 0 ? n : 0);
+    char* f = (char*)force_.get(3 * fb + sizeof(double));
+    double* e = (double*)(f + 3 * fb);
+    check(gse_ewald_direct_calc_force_energy(h_, n, dx, (double*)f, (double*)(f + fb),
+                                             (double*)(f + 2 * fb), e));
+    detail::download(fx, f, n);
+    detail::download(fy, f + fb, n);
+    detail::download(fz, f + 2 * fb, n);
+    double eh = 0.0;
+    check(gse_memcpy_to_host(&eh, e, sizeof(double)));
+    return eh;
+  }
+
+ private:
+  gse_ewald_direct_t h_ = nullptr;
+  DeviceBuffer xyzq_, force_;
+};
+
+// cuda/CudaPMERecip.h:138-180: SPME reciprocal space on DEVICE particles
+// (xyzq_t<CT>, the reference's float4) and device force arrays, one GPU,
+// BOX FFT.  The energies and the virial that the reference adds into its
+// CudaEnergyVirial registry are kept here and read with getEnergy(),
+// getSelfEnergy() and getVirial().
+template <typename AT, typename CT>
+class CudaPMERecip {
+ public:
+  CudaPMERecip(const int nfftx, const int nffty, const int nfftz, const int order) {
+    check(gse_pme_recip_create(&h_, dtype_of<AT>::value, dtype_of<CT>::value, nfftx, nffty,
+                               nfftz, order));
+    ev_.get(sizeof(double) * 11);
+  }
+  ~CudaPMERecip() { (void)gse_pme_recip_destroy(h_); }
+  CudaPMERecip(const CudaPMERecip&) = delete;
+  CudaPMERecip& operator=(const CudaPMERecip&) = delete;
+
+  // CudaPMERecip.h:155
+  void spread_charge(const xyzq_t<CT>* xyzq, const int ncoord, const double* recip) {
+    check(gse_pme_recip_spread_charge(h_, xyzq, ncoord, recip));
+  }
+  // CudaPMERecip.h:173-174
+  void r2c_fft() { check(gse_pme_recip_r2c_fft(h_)); }
+  void c2r_fft() { check(gse_pme_recip_c2r_fft(h_)); }
+  // CudaPMERecip.h:157-158
+  void scalar_sum(const double* recip, const double kappa, const bool calc_energy,
+                  const bool calc_virial) {
+    double* ev = (double*)ev_.ptr();
+    check(gse_pme_recip_scalar_sum(h_, recip, kappa, calc_energy, calc_virial, ev, ev + 1));
+  }
+  // CudaPMERecip.h:160
+  void calc_self_energy(const xyzq_t<CT>* xyzq, const int ncoord, const double kappa) {
+    check(gse_pme_recip_calc_self_energy(h_, xyzq, ncoord, kappa, (double*)ev_.ptr() + 10));
+  }
+  // CudaPMERecip.h:165-167: force[i], force[i+stride], force[i+2*stride] = dE/dr_i
+  void gather_force(const xyzq_t<CT>* xyzq, const int ncoord, const double* recip,
+                    const int stride, AT* force) {
+    check(gse_pme_recip_gather_force(h_, xyzq, ncoord, recip, stride, force));
+  }
+  double getEnergy() const { return read(0); }
+  double getSelfEnergy() const { return read(10); }
+  void getVirial(double* virial) const {
+    check(gse_memcpy_to_host(virial, (const double*)ev_.ptr() + 1, 9 * sizeof(double)));
+  }
+  gse_pme_recip_t handle() const { return h_; }
+
+ private:
+  double read(int k) const {
+    double v = 0.0;
+    check(gse_memcpy_to_host(&v, (const double*)ev_.ptr() + k, sizeof(double)));
+    return v;
+  }
+  gse_pme_recip_t h_ = nullptr;
+  mutable DeviceBuffer ev_;
 };
 
 // CpuGSE.h:18-68.  The device handle holds rhoS and phiM.

@@ -86,6 +86,13 @@ GSE_API gse_status_t gse_device_malloc(void** ptr, size_t bytes);
 GSE_API gse_status_t gse_device_free(void* ptr);
 GSE_API gse_status_t gse_memcpy_to_device(void* dst, const void* src, size_t bytes);
 GSE_API gse_status_t gse_memcpy_to_host(void* dst, const void* src, size_t bytes);
+/* The device-side helpers of CudaGrid (CudaGrid.h:28-40): zero `bytes` at a
+ * device pointer, device -> device copy (both synchronous, null stream), and
+ * a wait for all device work, which the reference's CUDA classes do after
+ * their kernels (CudaConvolution.cu:149). */
+GSE_API gse_status_t gse_memcpy_device_to_device(void* dst, const void* src, size_t bytes);
+GSE_API gse_status_t gse_device_clear(void* ptr, size_t bytes);
+GSE_API gse_status_t gse_device_synchronize(void);
 
 /* ------------------------------------------------------------------------
  * CpuGaussCharge<AT,CT>  (CpuGaussCharge.h:11-49)
@@ -392,6 +399,87 @@ GSE_API gse_status_t gse_batch_get_config(gse_batch_t h, int* config);
  * be NULL).  Synchronises. */
 GSE_API gse_status_t gse_batch_autotune(gse_batch_t h, const void* xyzq, int reps,
                                         double* best_ms);
+
+/* ------------------------------------------------------------------------
+ * SPME reciprocal space, the comparator cpu_gse/gpu_gse run beside GSE
+ * (CudaPMERecip<AT,CT,CT2>, cuda/CudaPMERecip.h:138-180; BOX FFT on one GPU).
+ * The charge grid accumulates in int64 fixed point (bit-identical for any
+ * particle order) and is converted to CT; FFTs are rocFFT; energy and virial
+ * are FP64 sums.  ccelec is the reference's 332.0716 (CudaPMERecip.h:25).
+ * recip: HOST array of 9 doubles, the reciprocal box (orthorhombic only:
+ * off-diagonal entries must be zero, as CudaPMERecip.cu:3028-3031 requires).
+ * ---------------------------------------------------------------------- */
+typedef struct gse_pme_recip_s* gse_pme_recip_t;
+
+/* CudaPMERecip.h:143-145  CudaPMERecip(nfftx, nffty, nfftz, order, BOX, 1, 0, ...);
+ * order 4, 6 or 8; every dimension at least the order. */
+GSE_API gse_status_t gse_pme_recip_create(gse_pme_recip_t* out, gse_dtype_t at, gse_dtype_t ct,
+                                          int nfftx, int nffty, int nfftz, int order);
+GSE_API gse_status_t gse_pme_recip_destroy(gse_pme_recip_t h);
+/* CudaPMERecip.h:152 */
+GSE_API gse_status_t gse_pme_recip_set_stream(gse_pme_recip_t h, void* hip_stream);
+/* CudaPMERecip.h:155  spread_charge(xyzq, ncoord, recip): B-spline spread
+ * of device particles {x, y, z, q} (CT) into the handle's grid */
+GSE_API gse_status_t gse_pme_recip_spread_charge(gse_pme_recip_t h, const void* xyzq, int ncoord,
+                                                 const double* recip);
+/* CudaPMERecip.h:173-174  r2c_fft(), c2r_fft() on the handle's grid */
+GSE_API gse_status_t gse_pme_recip_r2c_fft(gse_pme_recip_t h);
+GSE_API gse_status_t gse_pme_recip_c2r_fft(gse_pme_recip_t h);
+/* CudaPMERecip.h:157-158  scalar_sum(recip, kappa, calc_energy, calc_virial):
+ * multiplies the spectrum by the SPME kernel and, when asked, writes the
+ * reciprocal energy (device, 1 double) and virial (device, 9 doubles,
+ * row-major) -- overwritten, where the reference adds into its registry. */
+GSE_API gse_status_t gse_pme_recip_scalar_sum(gse_pme_recip_t h, const double* recip,
+                                              double kappa, int calc_energy, int calc_virial,
+                                              double* energy, double* virial);
+/* CudaPMERecip.h:165-167  gather_force(xyzq, ncoord, recip, stride, force):
+ * force[i], force[i+stride], force[i+2*stride] (AT) receive dE/dr_i, the
+ * reference's sign (its driver negates them, gpu_gse.cu:145-149). */
+GSE_API gse_status_t gse_pme_recip_gather_force(gse_pme_recip_t h, const void* xyzq, int ncoord,
+                                                const double* recip, int stride, void* force);
+/* CudaPMERecip.h:160  calc_self_energy: -kappa ccelec/sqrt(pi) sum q^2
+ * written to energy (device, 1 double) */
+GSE_API gse_status_t gse_pme_recip_calc_self_energy(gse_pme_recip_t h, const void* xyzq,
+                                                    int ncoord, double kappa, double* energy);
+/* Build-defined fused step as gpu_gse.cu:61-73 sequences it for an
+ * orthorhombic box: spread -> r2c -> scalar_sum -> c2r -> gather, with the
+ * forces -dE/dr in fx, fy, fz (AT; all NULL: no gather); energy and virial
+ * (device) may be NULL. */
+GSE_API gse_status_t gse_pme_recip_step(gse_pme_recip_t h, const void* xyzq, int ncoord,
+                                        double boxx, double boxy, double boxz, double kappa,
+                                        double* energy, double* virial, void* fx, void* fy,
+                                        void* fz);
+/* Device pointer of the handle's CT grid: the charge grid after a spread,
+ * the convolved grid after c2r_fft. */
+GSE_API gse_status_t gse_pme_recip_get_grid(gse_pme_recip_t h, void** grid);
+
+/* ------------------------------------------------------------------------
+ * Ewald direct space + self energy (CpuEwaldDirect, CpuEwaldDirect.h:11-44),
+ * FP64 as the reference.  Pairs within rcut (minimum image) are found
+ * through cells of edge >= rcut; each thread sums its own particle's terms
+ * in a fixed order, so results are deterministic.
+ * ---------------------------------------------------------------------- */
+typedef struct gse_ewald_direct_s* gse_ewald_direct_t;
+
+/* CpuEwaldDirect.h:34-36  CpuEwaldDirect(sigma, rcut, boxx, boxy, boxz, ccelec) */
+GSE_API gse_status_t gse_ewald_direct_create(gse_ewald_direct_t* out, double sigma, double rcut,
+                                             double boxx, double boxy, double boxz,
+                                             double ccelec);
+/* CpuEwaldDirect.h:37 */
+GSE_API gse_status_t gse_ewald_direct_destroy(gse_ewald_direct_t h);
+GSE_API gse_status_t gse_ewald_direct_set_stream(gse_ewald_direct_t h, void* hip_stream);
+/* CpuEwaldDirect.h:39  setupExclusions(n_iblo14, iblo14, inb14): HOST
+ * arrays in the reference's format (iblo14[j] running end of atom j's list
+ * in inb14, 1-based atom numbers, sign ignored); n_iblo14 = 0 clears them. */
+GSE_API gse_status_t gse_ewald_direct_setup_exclusions(gse_ewald_direct_t h, int n_iblo14,
+                                                       const int* iblo14, const int* inb14);
+/* CpuEwaldDirect.h:41-42  calcForceEnergy(n, xyzq, fx, fy, fz): device
+ * xyzq (double), fx/fy/fz (double) receive dE/dr_i (the reference's sign:
+ * CpuEwaldDirect.cpp:107-117), energy (device, 1 double) receives
+ * E_direct + E_self. */
+GSE_API gse_status_t gse_ewald_direct_calc_force_energy(gse_ewald_direct_t h, int n,
+                                                        const void* xyzq, double* fx, double* fy,
+                                                        double* fz, double* energy);
 
 #ifdef __cplusplus
 }

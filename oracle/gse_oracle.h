@@ -141,6 +141,37 @@ void orc_corrections(double sigma, double bx, double by, double bz,
                      double ccelec, int n, const double *xyzq,
                      double *e_self, double *e_net);
 
+/* ---- Ewald direct space + self energy (CpuEwaldDirect.cpp:31-131), ewald_oracle.c ----
+ * Returns E_direct + E_self; fx, fy, fz receive dE/dr_i (the reference's
+ * sign); exclusions in the reference's iblo14/inb14 form (n_iblo14 = 0: none). */
+double orc_ewald_direct(double sigma, double rcut, double boxx, double boxy, double boxz,
+                        double ccelec, int n, const double *xyzq, int n_iblo14,
+                        const int *iblo14, const int *inb14, double *fx, double *fy,
+                        double *fz);
+
+/* ---- SPME reciprocal comparator (cuda/CudaPMERecip.cu), spme_oracle.c ---- */
+/* B-spline weights theta[0..order-1] and their derivatives at fraction w
+ * (calc_theta_dtheta, CudaPMERecip.cu:266-327) */
+void orc_pme_theta(int order, double w, double *theta, double *dtheta);
+/* 1/|b(k)|^2 B-spline moduli of one axis (calc_prefac/dftmod, :3385-3478) */
+void orc_pme_prefac(int order, int nfft, double *prefac);
+/* Q = sum_i q_i theta theta theta (spread_charge_ortho_*, :436-934) */
+void orc_pme_spread(int X, int Y, int Z, int order, int n, const double *xyzq, double bx,
+                    double by, double bz, double *Q);
+/* scalar_sum (:946-1256) in place on the r2c half spectrum; energy and the
+ * 3x3 virial (row-major) may be NULL */
+void orc_pme_scalar_sum(int X, int Y, int Z, int order, double bx, double by, double bz,
+                        double kappa, double ccelec, double *Qk, double *energy,
+                        double *vir9);
+/* energy gradient dE/dr from the convolved grid (gather_force, :1612-1890) */
+void orc_pme_gather(int X, int Y, int Z, int order, int n, const double *xyzq, double bx,
+                    double by, double bz, double ccelec, const double *Qc, double *gx,
+                    double *gy, double *gz);
+/* full step (gpu_gse.cu:61-73): energy, virial, forces -dE/dr, the charge grid */
+void orc_pme_step(int X, int Y, int Z, int order, int n, const double *xyzq, double bx,
+                  double by, double bz, double kappa, double ccelec, double *energy,
+                  double *vir9, double *fx, double *fy, double *fz, double *Q_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -351,3 +351,74 @@ def erms(ref, x):
     ref = np.asarray(ref, dtype=np.float64)
     x = np.asarray(x, dtype=np.float64)
     return float(np.sqrt(np.sum((x - ref) ** 2) / np.sum(ref ** 2)))
+
+
+# ---------------------------------------------------------------- Ewald direct
+def _excl(iblo14, inb14):
+    if iblo14 is None:
+        return 0, None, None
+    ib = np.ascontiguousarray(iblo14, dtype=np.int32)
+    nb = np.ascontiguousarray(inb14 if len(inb14) else [0], dtype=np.int32)
+    return len(ib), ib, nb
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def ewald_direct(sigma, rcut, box, xyzq, iblo14=None, inb14=None, ccelec=332.0716,
+                 reference=False):
+    """CpuEwaldDirect::calcForceEnergy (CpuEwaldDirect.cpp:45-131): returns
+    (E_direct + E_self, dE/dr as an (n, 3) array).  reference=True runs the
+    reference's own compiled class (oracle/_ref) instead of the restatement."""
+    a, n = _xyzq(xyzq)
+    ne, ib, nb = _excl(iblo14, inb14)
+    g = np.zeros((3, max(n, 1)))
+    if reference:
+        L = ref_lib()
+        fn = L.ref_ewald_direct
+    else:
+        L = lib()
+        fn = L.orc_ewald_direct
+    fn.restype = _d
+    e = fn(_d(sigma), _d(rcut), *map(_d, box), _d(ccelec), n, _p(a), ne, _ip(ib), _ip(nb),
+           _p(g[0]), _p(g[1]), _p(g[2]))
+    return e, g[:, :n].T.copy()
+
+
+# ------------------------------------------------------------------- SPME
+def pme_theta(order, w):
+    th = np.zeros(16)
+    dth = np.zeros(16)
+    lib().orc_pme_theta(int(order), _d(w), _p(th), _p(dth))
+    return th[:order].copy(), dth[:order].copy()
+
+
+def pme_prefac(order, nfft):
+    p = np.zeros(nfft)
+    lib().orc_pme_prefac(int(order), int(nfft), _p(p))
+    return p
+
+
+def pme_spread(dims, order, xyzq, box):
+    a, n = _xyzq(xyzq)
+    X, Y, Z = dims
+    Q = np.zeros((Z, Y, X))
+    lib().orc_pme_spread(X, Y, Z, int(order), n, _p(a), *map(_d, box), _p(Q))
+    return Q
+
+
+def pme_step(dims, order, xyzq, box, kappa, ccelec=332.0716, want_grid=False):
+    """SPME reciprocal step as gpu_gse.cu:61-73 drives CudaPMERecip: returns
+    (energy, virial 3x3, forces (n, 3) = -dE/dr[, the charge grid])."""
+    a, n = _xyzq(xyzq)
+    X, Y, Z = dims
+    e = _d()
+    vir = np.zeros(9)
+    f = np.zeros((3, max(n, 1)))
+    Q = np.zeros((Z, Y, X)) if want_grid else None
+    lib().orc_pme_step(X, Y, Z, int(order), n, _p(a), *map(_d, box), _d(kappa), _d(ccelec),
+                       ctypes.byref(e), _p(vir), _p(f[0]), _p(f[1]), _p(f[2]),
+                       _p(Q) if want_grid else None)
+    out = (e.value, vir.reshape(3, 3), f[:, :n].T.copy())
+    return out + (Q,) if want_grid else out

@@ -11,11 +11,13 @@
 //   CpuGreensFuncGSE<T>    CpuGreensFuncGSE.h:16-19
 //   CpuGreensFuncG2<T>     CpuGreensFuncG2.h:16-19
 //   CpuEwaldRecip          CpuEwaldRecip.h:29-31
+//   CpuEwaldDirect         CpuEwaldDirect.h:35-43
 // CpuFFTSolver / CpuGSEk need FFTW, which this image does not have; they are
 // not built.
 #include <cstring>
 
 #include "CpuConvolution.h"
+#include "CpuEwaldDirect.h"
 #include "CpuEwaldRecip.h"
 #include "CpuGaussCharge.h"
 #include "CpuGreensFuncG2.h"
@@ -167,3 +169,17 @@ double ref_ewald_recip(double sigma, int kcut, double bx, double by, double bz,
 }
 
 }  // extern "C"
+
+// CpuEwaldDirect::setupExclusions + calcForceEnergy (CpuEwaldDirect.cpp:31-131)
+extern "C" double ref_ewald_direct(double sigma, double rcut, double bx, double by, double bz,
+                                   double ccelec, int n, const double* xyzq, int n_iblo14,
+                                   const int* iblo14, const int* inb14, double* fx, double* fy,
+                                   double* fz) {
+  CpuEwaldDirect d(sigma, rcut, bx, by, bz, ccelec);
+  if (n_iblo14 > 0) d.setupExclusions(n_iblo14, iblo14, inb14);
+  xyzq_t<double>* p = new xyzq_t<double>[n > 0 ? n : 1];
+  to_xyzq(n, xyzq, p);
+  const double e = d.calcForceEnergy(n, p, fx, fy, fz);
+  delete[] p;
+  return e;
+}

@@ -2,7 +2,9 @@
 // cpu_gse.cpp drives the reference: the testPair GSE-k sequence
 // (cpu_gse.cpp:638-736), the TEST_CHARGE_SPREAD identity through
 // CpuFFTSolver<CpuGreensFuncG2> (cpu_gse.cpp:752-794), and every other
-// CpuGaussCharge / CpuConvolution / CpuFFTSolver method.  Each result is
+// CpuGaussCharge / CpuConvolution / CpuFFTSolver method, and the reference's
+// GPU harness sequences gpu_conv.cu:27-76 and gpu_gauss.cu:52-108 through
+// CudaGrid / CudaConvolution / CudaGaussCharge.  Each result is
 // checked against the FP64 oracle (oracle/liboracle_gse.so, test
 // infrastructure).  Prints one line per check and exits non-zero on failure.
 #include <cmath>
@@ -225,6 +227,107 @@ int main() {
       gm.interpolateForce(N, pf.data(), dx.data(), dy.data(), dz.data());
       report("CpuGSEk<double,float> 3k Erms", erms(dx, dy, dz), 1e-5);
     }
+  }
+
+  // ------------------- gpu_conv.cu:27-76 through CudaGrid / CudaConvolution
+  {
+    const int n = 128, R = 5;
+    const double sigma = 2.12;
+    const double inv_2sigmasq = 1.0 / (2.0 * sigma * sigma);
+    const double pref = std::pow(2 * pi_dbl * sigma * sigma, -3.0 / 2.0);
+    std::vector<double> kx(2 * R + 1), ky(2 * R + 1), kz(2 * R + 1);
+    for (int ii = -R; ii <= R; ii++) {
+      kx[ii + R] = std::exp(-(ii * ii) * inv_2sigmasq);
+      ky[ii + R] = std::exp(-(ii * ii) * inv_2sigmasq);
+      kz[ii + R] = pref * std::exp(-(ii * ii) * inv_2sigmasq);
+    }
+    CpuGrid<double> cpuRho(n, n, n), cpuRhoXYZ(n, n, n);
+    CudaGrid<double> cudaRho(n, n, n), cudaRhoXYZ(n, n, n);
+    double* d = cpuRho.getDataPointer();
+    for (int i = 0; i < n * n * n; i++) d[i] = urand();
+    cudaRho.copy(cpuRho);
+    CudaConvolution<double> cudaConv(n, n, n);
+    cudaConv.setupKernel(R, R, R, kx.data(), ky.data(), kz.data());
+    cudaRhoXYZ.clear();
+    cudaConv.run(cudaRho, cudaRhoXYZ);
+    cpuRhoXYZ.copy(cudaRhoXYZ);  // CpuGrid::copy(const CudaGrid&), device -> host
+    const size_t np = (size_t)n * n * n;
+    std::vector<double> o(np);
+    orc_conv_run_x(n, n, n, R, kx.data(), d, o.data());
+    report("CudaConvolution::run n=128 R=5 (X pass)", maxrel(cpuRhoXYZ.getDataPointer(), o.data(), np), 1e-10);
+    // CudaGrid::copy(const CudaGrid&) and clear()
+    CudaGrid<double> dup(n, n, n);
+    dup.copy(cudaRhoXYZ);
+    CpuGrid<double> back(n, n, n);
+    back.copy(dup);
+    report("CudaGrid::copy(CudaGrid) device -> device", maxrel(back.getDataPointer(), o.data(), np), 0.0);
+    dup.clear();
+    back.copy(dup);
+    report("CudaGrid::clear", back.maxAbsValue(), 0.0);
+    // the full separable form on device grids, FP32 with compensated sums
+    CpuGrid<float> rf(n, n, n), of(n, n, n);
+    for (size_t i = 0; i < np; i++) rf.getDataPointer()[i] = (float)d[i];
+    CudaGrid<float> cf(n, n, n), cfo(n, n, n);
+    cf.copy(rf);
+    std::vector<float> kxf(kx.begin(), kx.end()), kyf(ky.begin(), ky.end()), kzf(kz.begin(), kz.end());
+    CudaConvolution<float> convf(n, n, n);
+    convf.setupKernel(R, R, R, kxf.data(), kyf.data(), kzf.data());
+    convf.runXYZ(cf, cfo);
+    of.copy(cfo);
+    std::vector<double> in32(np), k32x(kxf.begin(), kxf.end()), k32y(kyf.begin(), kyf.end()),
+        k32z(kzf.begin(), kzf.end());
+    for (size_t i = 0; i < np; i++) in32[i] = rf.getDataPointer()[i];
+    orc_conv_run_xyz(n, n, n, R, R, R, k32x.data(), k32y.data(), k32z.data(), in32.data(), o.data());
+    report("CudaConvolution<float>::runXYZ n=128", maxrel(as_double(of.getDataPointer(), np).data(), o.data(), np), 1e-6);
+  }
+
+  // ----------------- gpu_gauss.cu:52-108 through CudaGaussCharge (device xyzq)
+  {
+    // the reference reads data/xyzq.txt (23,558 charges, box 62.23); the
+    // same count and box with charges of alternating sign stand in for it
+    const double sigma = 2.12132034355964 / std::sqrt(2.0), lambdaSigma = 3.0;
+    const int n = 64, numCoord = 23558;
+    const double box = 62.23;
+    std::vector<xyzq_t<double>> h(numCoord);
+    for (int i = 0; i < numCoord; i++)
+      h[i] = {urand() * box, urand() * box, urand() * box, (i & 1) ? 0.5 : -0.5};
+    std::vector<xyzq_t<float>> hf(numCoord);
+    for (int i = 0; i < numCoord; i++)
+      hf[i] = {(float)h[i].x, (float)h[i].y, (float)h[i].z, (float)h[i].q};
+    void* d_xyzq = nullptr;
+    check(gse_device_malloc(&d_xyzq, sizeof(xyzq_t<double>) * numCoord));
+    check(gse_memcpy_to_device(d_xyzq, h.data(), sizeof(xyzq_t<double>) * numCoord));
+    void* d_xyzqf = nullptr;
+    check(gse_device_malloc(&d_xyzqf, sizeof(xyzq_t<float>) * numCoord));
+    check(gse_memcpy_to_device(d_xyzqf, hf.data(), sizeof(xyzq_t<float>) * numCoord));
+    const double rcut = lambdaSigma * std::sqrt(2.0) * sigma;
+    const size_t np = (size_t)n * n * n;
+    std::vector<double> o(np);
+    orc_spread(n, n, n, sigma, rcut, numCoord, &h[0].x, box, box, box, o.data(), 0);
+    {
+      CudaGaussCharge<double, double> cudaGaussCharge(n, n, n);
+      CudaGrid<double> cudaRho(n, n, n);
+      cudaGaussCharge.spreadChargeToGrid(sigma, rcut, numCoord, (const xyzq_t<double>*)d_xyzq,
+                                         box, box, box, cudaRho);
+      CpuGrid<double> cpuRho(n, n, n);
+      cpuRho.copy(cudaRho);
+      report("CudaGaussCharge<double,double> 23558 charges", maxrel(cpuRho.getDataPointer(), o.data(), np), 1e-10);
+    }
+    {
+      std::vector<xyzq_t<double>> hr(numCoord);
+      for (int i = 0; i < numCoord; i++) hr[i] = {hf[i].x, hf[i].y, hf[i].z, hf[i].q};
+      orc_spread(n, n, n, sigma, rcut, numCoord, &hr[0].x, box, box, box, o.data(), 0);
+      CudaGaussCharge<float, float> cudaGaussCharge(n, n, n);
+      CudaGrid<float> cudaRho(n, n, n);
+      cudaGaussCharge.spreadChargeToGrid(sigma, rcut, numCoord, (const xyzq_t<float>*)d_xyzqf,
+                                         box, box, box, cudaRho);
+      CpuGrid<float> cpuRho(n, n, n);
+      cpuRho.copy(cudaRho);
+      report("CudaGaussCharge<float,float> 23558 charges",
+             maxrel(as_double(cpuRho.getDataPointer(), np).data(), o.data(), np), 2e-6);
+    }
+    check(gse_device_free(d_xyzq));
+    check(gse_device_free(d_xyzqf));
   }
 
   // ------------------------------------------------------------ error path

@@ -1,7 +1,8 @@
 """The C++ drop-in classes (include/gse/gse.hpp), as a C++ caller uses them.
 
 CPU: the header compiles stand-alone with the host compiler and names every
-reference class and method it replaces.  GPU: tests/cpp/test_dropin (built
+reference class and method it replaces, the GPU ones (CudaGrid,
+CudaGaussCharge, CudaConvolution) included.  GPU: tests/cpp/test_dropin (built
 by __graft_entry__.build()) runs the cpu_gse.cpp testPair sequence and every
 class method against libgse_hip.so and checks each result against the FP64
 oracle (1e-10 relative in FP64, Erms <= 1e-5 in FP32).
@@ -27,6 +28,12 @@ def test_header_compiles_standalone(tmp_path):
                    "template class CpuFFTSolver<double>;\n"
                    "template class CpuGSEk<double,double>;\n"
                    "template class CpuGSEk<float,float>;\n"
+                   "template class CudaGrid<double>;\n"
+                   "template class CudaGaussCharge<double,double>;\n"
+                   "template class CudaGaussCharge<double,float>;\n"
+                   "template class CudaGaussCharge<float,float>;\n"
+                   "template class CudaConvolution<float>;\n"
+                   "template class CudaConvolution<double>;\n"
                    "int main() { return 0; }\n")
     subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only",
                     "-I", os.path.join(ROOT, "include"), str(src)], check=True)
@@ -43,7 +50,10 @@ def test_header_keeps_reference_signatures():
                  "void mul(", "void spreadCharge1(", "void spreadCharge2(",
                  "void solvePoisson()", "AT calculateEnergy()", "CpuGrid<CT>& getRhoS()",
                  "CpuGrid<CT>& getPhi()", "void setBoxSize(", "double getSigma1()",
-                 "double getSigma2()", "double getLambdaSigma1()"]:
+                 "double getSigma2()", "double getLambdaSigma1()", "class CudaGrid",
+                 "class CudaGaussCharge", "class CudaConvolution",
+                 "void copy(const CpuGrid<T>& h_grid)", "void copy(const CudaGrid<T>& d_grid)",
+                 "void clear()"]:
         assert name in txt, name
 
 
