@@ -14,6 +14,7 @@ GSE_FLOAT32 = 0
 GSE_FLOAT64 = 1
 GREENS_GSE = 0
 GREENS_G2 = 1
+GREENS_HERMITIAN = 2
 
 GATHER_SPHERE_FORCE = 1
 GATHER_CUBE_FORCE = 2
@@ -129,6 +130,29 @@ _SIGS = {
     "gse_ewald_direct_set_stream": [V, V],
     "gse_ewald_direct_setup_exclusions": [V, I, PI, PI],
     "gse_ewald_direct_calc_force_energy": [V, I, V, V, V, V, V],
+    "gse_gser_create": [PV, I, I, I, I, I, D, D, D, D, D, D, D, D],
+    "gse_gser_destroy": [V],
+    "gse_gser_set_stream": [V, V],
+    "gse_gser_spread_charge1": [V, I, V],
+    "gse_gser_spread_charge2": [V],
+    "gse_gser_solve_poisson": [V],
+    "gse_gser_calculate_energy": [V, PD],
+    "gse_gser_interpolate_force": [V, I, V, V, V, V],
+    "gse_gser_copy_grid": [V, I, V],
+    "gse_gser_step": [V, I, V, ctypes.POINTER(StepDesc)],
+    "gse_gseu_create": [PV, I, I, I, D, I, I, I, D, D, D, D, D, D, D, D],
+    "gse_gseu_destroy": [V],
+    "gse_gseu_set_stream": [V, V],
+    "gse_gseu_spread_charge1": [V, I, V],
+    "gse_gseu_solve_poisson": [V],
+    "gse_gseu_calculate_energy": [V, PD],
+    "gse_gseu_interpolate_force": [V, I, V, V, V, V],
+    "gse_gseu_copy_grid": [V, I, V],
+    "gse_gseu_step": [V, I, V, ctypes.POINTER(StepDesc)],
+    "gse_ewald_recip_create": [PV, D, I, D, D, D, D],
+    "gse_ewald_recip_destroy": [V],
+    "gse_ewald_recip_set_stream": [V, V],
+    "gse_ewald_recip_calc_force_energy": [V, I, V, V, V, V, V],
 }
 _RET = {"gse_status_string": ctypes.c_char_p, "gse_last_error": ctypes.c_char_p,
         "gse_version": I}

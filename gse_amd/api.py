@@ -15,7 +15,7 @@ import ctypes
 import torch
 
 from . import _lib
-from ._lib import GSE_FLOAT32, GSE_FLOAT64, GREENS_G2, GREENS_GSE, check  # noqa: F401
+from ._lib import GSE_FLOAT32, GSE_FLOAT64, GREENS_G2, GREENS_GSE, GREENS_HERMITIAN, check  # noqa: F401
 
 _DT = {torch.float32: GSE_FLOAT32, torch.float64: GSE_FLOAT64}
 _TD = {GSE_FLOAT32: torch.float32, GSE_FLOAT64: torch.float64}
@@ -250,6 +250,13 @@ class CpuGreensFuncGSE:
 class CpuGreensFuncG2(CpuGreensFuncGSE):
     """CpuGreensFuncG2<T> (CpuGreensFuncG2.h)."""
     kind = GREENS_G2
+
+
+class CpuGreensFuncHermitian(CpuGreensFuncGSE):
+    """The HERMITIAN (Mehrstellen) finite-difference Poisson operator of
+    CpuMultiGridSolver.cpp:237-251 as a Green's function: phi = A^-1 B
+    (-4 pi rho), zero mean; sigma is not used."""
+    kind = GREENS_HERMITIAN
 
 
 class CpuFFTSolver:
@@ -574,7 +581,7 @@ def _recip9(recip):
 
 
 class CudaPMERecip:
-    """CudaPMERecip<AT,CT,CT2> (cuda/CudaPMERecip.h:138-180) on the GPU: the
+    """CudaPMERecip<AT,CT,CT2> (cuda/CudaPMERecip.h:36-182) on the GPU: the
     SPME reciprocal comparator, order 4, 6 or 8, BOX FFT on one GPU.  recip is
     the 9-entry reciprocal box (orthorhombic) or the three box lengths."""
 
@@ -673,7 +680,7 @@ class CudaPMERecip:
 
 
 class CpuEwaldDirect:
-    """CpuEwaldDirect (CpuEwaldDirect.h:11-44) on the GPU, FP64: direct-space
+    """CpuEwaldDirect (CpuEwaldDirect.h:10-44) on the GPU, FP64: direct-space
     erfc pair sum within rcut (minimum image) plus the self energy."""
 
     def __init__(self, sigma, rcut, boxx, boxy, boxz, ccelec=332.0716):
@@ -714,3 +721,169 @@ class CpuEwaldDirect:
         check(self._L.gse_ewald_direct_calc_force_energy(self._h, n, _ptr(xyzq), _ptr(forceX),
                                                          _ptr(forceY), _ptr(forceZ), _ptr(e)))
         return float(e.cpu()[0])
+
+
+class CpuEwaldRecip:
+    """CpuEwaldRecip (CpuEwaldRecip.h:12-46) on the GPU, FP64: the exact Ewald
+    k-space sum over 0 < |m| <= kcut."""
+
+    def __init__(self, sigma, kcut, boxx, boxy, boxz, ccelec=332.0716):
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        check(L.gse_ewald_recip_create(ctypes.byref(h), float(sigma), int(kcut), float(boxx),
+                                       float(boxy), float(boxz), float(ccelec)))
+        self._h = h
+        self._L = L
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.gse_ewald_recip_destroy(self._h)
+            self._h = None
+
+    def calcForceEnergy(self, n, xyzq, forceX=None, forceY=None, forceZ=None):
+        """Writes the forces -dE/dr into the float64 device arrays (all None:
+        energy only) and returns the energy (synchronises)."""
+        _check_xyzq(xyzq, torch.float64)
+        _check_count(n, xyzq)
+        fs = (forceX, forceY, forceZ)
+        if any(f is not None for f in fs):
+            for f in fs:
+                if f is None or f.dtype != torch.float64 or f.numel() < n:
+                    raise ValueError("force arrays must be float64 with n entries")
+        e = torch.empty(1, dtype=torch.float64, device=xyzq.device)
+        check(self._L.gse_ewald_recip_set_stream(self._h, _stream()))
+        check(self._L.gse_ewald_recip_calc_force_energy(self._h, n, _ptr(xyzq), *(
+            _ptr(f) if f is not None else None for f in fs), _ptr(e)))
+        return float(e.cpu()[0])
+
+
+class _GSEMethod:
+    """Shared parts of the GSE-r and GSE-u mirrors (prefix: C-ABI name stem)."""
+
+    _prefix = ""
+
+    def _fn(self, name):
+        return getattr(self._L, f"gse_{self._prefix}_{name}")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._fn("destroy")(self._h)
+            self._h = None
+
+    def _s(self):
+        check(self._fn("set_stream")(self._h, _stream()))
+
+    def spreadCharge1(self, numCoord, xyzq):
+        _check_xyzq(xyzq, self.ct)
+        _check_count(numCoord, xyzq)
+        self._s()
+        check(self._fn("spread_charge1")(self._h, numCoord, _ptr(xyzq)))
+
+    def solvePoisson(self):
+        self._s()
+        check(self._fn("solve_poisson")(self._h))
+
+    def calculateEnergy(self):
+        e = ctypes.c_double()
+        self._s()
+        check(self._fn("calculate_energy")(self._h, ctypes.byref(e)))
+        return e.value
+
+    def interpolateForce(self, numCoord, xyzq, forceX, forceY, forceZ):
+        _check_xyzq(xyzq, self.ct)
+        _check_count(numCoord, xyzq)
+        for f in (forceX, forceY, forceZ):
+            if f.dtype != self.at or f.numel() < numCoord:
+                raise ValueError("force arrays must be AT with numCoord entries")
+        self._s()
+        check(self._fn("interpolate_force")(self._h, numCoord, _ptr(xyzq), _ptr(forceX),
+                                            _ptr(forceY), _ptr(forceZ)))
+
+    def _copy_grid(self, which):
+        X, Y, Z = self.dims
+        out = torch.empty((Z, Y, X), dtype=self.ct, device="cuda")
+        self._s()
+        check(self._fn("copy_grid")(self._h, which, _ptr(out)))
+        return out
+
+    def step(self, xyzq, force=True, potential=False, cube=False, out=None):
+        """Fused step: energy (device float64) and the requested outputs."""
+        _check_xyzq(xyzq, self.ct)
+        n = xyzq.shape[0]
+        dev = xyzq.device
+        o = {} if out is None else out
+        d = _lib.StepDesc()
+        if force:
+            o.setdefault("force", torch.empty((3, n), dtype=self.at, device=dev))
+            d.forceX, d.forceY, d.forceZ = (_ptr(o["force"][k]) for k in range(3))
+        if potential:
+            o.setdefault("pot", torch.empty(n, dtype=self.at, device=dev))
+            d.potential = _ptr(o["pot"])
+        if cube:
+            o.setdefault("force_cube", torch.empty((3, n), dtype=self.at, device=dev))
+            d.cubeX, d.cubeY, d.cubeZ = (_ptr(o["force_cube"][k]) for k in range(3))
+        o.setdefault("energy", torch.empty(1, dtype=torch.float64, device=dev))
+        d.energy = _ptr(o["energy"])
+        self._s()
+        check(self._fn("step")(self._h, n, _ptr(xyzq), ctypes.byref(d)))
+        return o
+
+
+class CpuGSEr(_GSEMethod):
+    """CpuGSEr<AT,CT> (CpuGSEr.h:18-37) on the GPU: sigma1 spread, on-grid
+    sigma2 convolution, the HERMITIAN-stencil Poisson equation solved exactly
+    (zero-mean potential), energy from the sigma1 grid."""
+
+    _prefix = "gser"
+
+    def __init__(self, at, ct, sizeX, sizeY, sizeZ, sigma, kappa, lambdaSigma, lambdaSigma1,
+                 boxx, boxy, boxz, ccelec=332.0716):
+        L = _lib.load()
+        self._L = L
+        self.at = _TD[_code(at)]
+        self.ct = _TD[_code(ct)]
+        self.dims = (sizeX, sizeY, sizeZ)
+        h = ctypes.c_void_p()
+        check(L.gse_gser_create(ctypes.byref(h), _code(at), _code(ct), sizeX, sizeY, sizeZ,
+                                sigma, kappa, lambdaSigma, lambdaSigma1, boxx, boxy, boxz,
+                                ccelec))
+        self._h = h
+
+    def spreadCharge2(self):
+        self._s()
+        check(self._L.gse_gser_spread_charge2(self._h))
+
+    def getRho(self):
+        return self._copy_grid(0)
+
+    def getRhoS(self):
+        return self._copy_grid(1)
+
+    def getPhi(self):
+        return self._copy_grid(2)
+
+
+class CpuGSEu(_GSEMethod):
+    """CpuGSEu<AT,CT> (CpuGSEu.h:18-42) on the GPU: the u-series of N on-grid
+    convolutions with widths sigma_j = sqrt(sigma^2 b^(2j) - 2 sigma1^2)."""
+
+    _prefix = "gseu"
+
+    def __init__(self, at, ct, N, b, sizeX, sizeY, sizeZ, sigma, kappa, lambdaSigma,
+                 lambdaSigma1, boxx, boxy, boxz, ccelec=332.0716):
+        L = _lib.load()
+        self._L = L
+        self.at = _TD[_code(at)]
+        self.ct = _TD[_code(ct)]
+        self.dims = (sizeX, sizeY, sizeZ)
+        h = ctypes.c_void_p()
+        check(L.gse_gseu_create(ctypes.byref(h), _code(at), _code(ct), int(N), float(b), sizeX,
+                                sizeY, sizeZ, sigma, kappa, lambdaSigma, lambdaSigma1, boxx, boxy,
+                                boxz, ccelec))
+        self._h = h
+
+    def getRhoS(self):
+        return self._copy_grid(0)
+
+    def getPhi(self):
+        return self._copy_grid(1)

@@ -618,7 +618,7 @@ struct gse_batch_s {
   }
 };
 
-// SPME reciprocal comparator (cuda/CudaPMERecip.h:138-180, BOX FFT, one GPU)
+// SPME reciprocal comparator (cuda/CudaPMERecip.h:36-182, BOX FFT, one GPU)
 struct gse_pme_recip_s {
   bool at_dbl = true, ct_dbl = true;
   int X = 0, Y = 0, Z = 0, order = 4;
@@ -629,7 +629,36 @@ struct gse_pme_recip_s {
   hipStream_t stream = nullptr;
 };
 
-// Ewald direct space + self energy (CpuEwaldDirect.h:11-44)
+// Ewald reciprocal sum (CpuEwaldRecip.h:6-40)
+struct gse_ewald_recip_s {
+  double sigma = 0, ccelec = 332.0716, box[3] = {0, 0, 0};
+  int kcut = 0;
+  DevBuf sk, partials;
+  hipStream_t stream = nullptr;
+};
+
+// GSE-r (CpuGSEr.h): a GSE-k handle's engine, solver and grids (core->rhoS
+// holds the sigma2 grid, core->phiM the potential) plus the sigma1 grid rho
+struct gse_gser_s {
+  gse_gsek_s* core = nullptr;
+  DevBuf rho;
+};
+
+// GSE-u (CpuGSEu.h): the core's rhoS is the sigma1 grid; N convolved terms
+struct gse_gseu_s {
+  gse_gsek_s* core = nullptr;
+  int N = 0;
+  double b = 0.0;
+  std::vector<double> sig, fac;        // sigma_j and (2 pi sigma_j^2)^(3/2)/sigma_j
+  struct Taps3 {
+    DevBuf t[3];
+    int R[3];
+  };
+  std::vector<Taps3*> taps;            // the three tap arrays and radii of each term
+  std::vector<DevBuf*> terms;          // rhoG_j
+};
+
+// Ewald direct space + self energy (CpuEwaldDirect.h:10-44)
 struct gse_ewald_direct_s {
   double sigma = 0, rcut = 0, ccelec = 332.0716, box[3] = {0, 0, 0};
   DevBuf xofs, xlist, cell_of, counts, starts, fill, tmp, sidx, sx, partials, self2, mom;
@@ -1130,7 +1159,7 @@ gse_status_t gse_fft_solver_create(gse_fft_solver_t* out, gse_dtype_t t, gse_gre
   if (!out) return fail(GSE_ERROR_INVALID_VALUE, "out is NULL");
   *out = nullptr;
   if (t != GSE_FLOAT32 && t != GSE_FLOAT64) return fail(GSE_ERROR_INVALID_VALUE, "bad dtype");
-  if (kind != GSE_GREENS_GSE && kind != GSE_GREENS_G2)
+  if (kind != GSE_GREENS_GSE && kind != GSE_GREENS_G2 && kind != GSE_GREENS_HERMITIAN)
     return fail(GSE_ERROR_INVALID_VALUE, "bad Green's function kind");
   GSECHK(check_dims(X, Y, Z));
   auto* h = new gse_fft_solver_s();
@@ -1182,7 +1211,7 @@ gse_status_t gse_greens_func_mul(gse_dtype_t t, gse_greens_kind_t kind, int X, i
                                  void* stream) {
   if (!rhoK) return fail(GSE_ERROR_INVALID_VALUE, "rhoK is NULL");
   if (t != GSE_FLOAT32 && t != GSE_FLOAT64) return fail(GSE_ERROR_INVALID_VALUE, "bad dtype");
-  if (kind != GSE_GREENS_GSE && kind != GSE_GREENS_G2)
+  if (kind != GSE_GREENS_GSE && kind != GSE_GREENS_G2 && kind != GSE_GREENS_HERMITIAN)
     return fail(GSE_ERROR_INVALID_VALUE, "bad Green's function kind");
   GSECHK(check_dims(X, Y, Z));
   GSECHK(check_box(bx, by, bz));
@@ -1859,6 +1888,315 @@ gse_status_t gse_ewald_direct_calc_force_energy(gse_ewald_direct_t h, int n, con
   HIPCHK(launch_corrections_finalize(h->mom.as<double>(), kRedBlocks, h->sigma, 1.0, 1.0, 1.0,
                                      h->ccelec, h->self2.as<double>(), s));
   HIPCHK(launch_ew_finalize(h->partials.as<double>(), c.ncell, h->self2.as<double>(), energy, s));
+  return GSE_SUCCESS;
+}
+
+// ------------------------------- Ewald reciprocal (CpuEwaldRecip) --------
+gse_status_t gse_ewald_recip_create(gse_ewald_recip_t* out, double sigma, int kcut, double boxx,
+                                    double boxy, double boxz, double ccelec) {
+  if (!out) return fail(GSE_ERROR_INVALID_VALUE, "out is NULL");
+  *out = nullptr;
+  GSECHK(check_box(boxx, boxy, boxz));
+  if (!(sigma >= 0) || !std::isfinite(sigma)) return fail(GSE_ERROR_INVALID_VALUE, "sigma >= 0");
+  if (kcut < 0 || kcut > 400) return fail(GSE_ERROR_INVALID_VALUE, "kcut must be in [0, 400]");
+  auto* h = new gse_ewald_recip_s();
+  h->sigma = sigma;
+  h->kcut = kcut;
+  h->ccelec = ccelec;
+  h->box[0] = boxx;
+  h->box[1] = boxy;
+  h->box[2] = boxz;
+  *out = h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_ewald_recip_destroy(gse_ewald_recip_t h) {
+  if (!h) return GSE_SUCCESS;
+  h->sk.release();
+  h->partials.release();
+  delete h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_ewald_recip_set_stream(gse_ewald_recip_t h, void* s) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  h->stream = (hipStream_t)s;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_ewald_recip_calc_force_energy(gse_ewald_recip_t h, int n, const void* xyzq,
+                                               double* fx, double* fy, double* fz,
+                                               double* energy) {
+  if (!h || !energy) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  if (n < 0 || (n > 0 && !xyzq)) return fail(GSE_ERROR_INVALID_VALUE, "bad particles");
+  if ((fx || fy || fz) && !(fx && fy && fz))
+    return fail(GSE_ERROR_INVALID_VALUE, "fx/fy/fz must be all set or all NULL");
+  ErArgs a;
+  a.kcut = h->kcut;
+  const long long n1 = 2LL * h->kcut + 1;
+  a.ncube = (int)(n1 * n1 * n1);
+  for (int d = 0; d < 3; d++) a.ib[d] = 1.0 / h->box[d];
+  a.pref = 2.0 * M_PI * a.ib[0] * a.ib[1] * a.ib[2] * h->ccelec;  // CpuEwaldRecip.cpp:95
+  a.half_sigmasq = 0.5 * h->sigma * h->sigma;
+  GSECHK(h->sk.ensure(3 * sizeof(double) * (size_t)a.ncube));
+  GSECHK(h->partials.ensure(sizeof(double) * (size_t)ewald_recip_blocks(a)));
+  HIPCHK(launch_ewald_recip(a, xyzq, n, h->sk.as<double>(), h->partials.as<double>(), fx, fy, fz,
+                            energy, h->stream));
+  return GSE_SUCCESS;
+}
+
+// ------------------------------- GSE-r (CpuGSEr) ---------------------------
+gse_status_t gse_gser_create(gse_gser_t* out, gse_dtype_t at, gse_dtype_t ct, int X, int Y,
+                             int Z, double sigma, double kappa, double ls, double ls1, double bx,
+                             double by, double bz, double ccelec) {
+  if (!out) return fail(GSE_ERROR_INVALID_VALUE, "out is NULL");
+  *out = nullptr;
+  gse_gsek_t core = nullptr;
+  GSECHK(gse_gsek_create(&core, at, ct, X, Y, Z, sigma, kappa, ls, ls1, bx, by, bz, ccelec));
+  auto* h = new gse_gser_s();
+  h->core = core;
+  const gse_status_t st = h->rho.ensure(tsize(core->ct_dbl) * (size_t)X * Y * Z);
+  if (st != GSE_SUCCESS) {
+    gse_gser_destroy(h);
+    return st;
+  }
+  *out = h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gser_destroy(gse_gser_t h) {
+  if (!h) return GSE_SUCCESS;
+  gse_gsek_destroy(h->core);
+  h->rho.release();
+  delete h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gser_set_stream(gse_gser_t h, void* s) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  return gse_gsek_set_stream(h->core, s);
+}
+
+gse_status_t gse_gser_spread_charge1(gse_gser_t h, int n, const void* xyzq) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  gse_gsek_s* c = h->core;
+  GSECHK(c->eng.bin_particles(n, xyzq, c->bx, c->by, c->bz, c->sigma1, c->rcut1(), false,
+                              c->stream));
+  GSECHK(c->eng.spread_binned(c->sigma1, false, c->stream));
+  return c->eng.convert(h->rho.p, c->stream);
+}
+
+gse_status_t gse_gser_spread_charge2(gse_gser_t h) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  gse_gsek_s* c = h->core;
+  GSECHK(gsek_taps(c));
+  return conv3(c->ct_dbl, c->X, c->Y, c->Z, c->tapsR, c->taps, h->rho.p, c->rhoS.p, c->tmp,
+               c->tmp2, c->stream);
+}
+
+gse_status_t gse_gser_solve_poisson(gse_gser_t h) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  gse_gsek_s* c = h->core;
+  return c->ks.solve(GSE_GREENS_HERMITIAN, 0.0, c->bx, c->by, c->bz, c->rhoS.p, c->phiM.p,
+                     nullptr, nullptr, c->ccelec, c->stream);
+}
+
+// 1/2 ccelec h^3 sum a*b into dst (device)
+static gse_status_t grid_energy(gse_gsek_s* c, const void* a, const void* b, double scale,
+                                double* dst) {
+  const double vcell = (c->bx / c->X) * (c->by / c->Y) * (c->bz / c->Z);
+  HIPCHK(launch_dot(c->ct_dbl, a, b, (size_t)c->X * c->Y * c->Z, c->part.as<double>(), c->stream));
+  HIPCHK(launch_reduce_partials(c->part.as<double>(), kRedBlocks, 1, 1,
+                                scale * 0.5 * c->ccelec * vcell, dst, c->stream));
+  return GSE_SUCCESS;
+}
+
+static gse_status_t host_energy(gse_gsek_s* c, double* energy) {
+  HIPCHK(hipMemcpyAsync(energy, c->dout.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gser_calculate_energy(gse_gser_t h, double* energy) {
+  if (!h || !energy) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  GSECHK(grid_energy(h->core, h->rho.p, h->core->phiM.p, 1.0, h->core->dout.as<double>()));
+  return host_energy(h->core, energy);
+}
+
+gse_status_t gse_gser_interpolate_force(gse_gser_t h, int n, const void* xyzq, void* fx,
+                                        void* fy, void* fz) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  return gse_gsek_interpolate_force(h->core, n, xyzq, fx, fy, fz);
+}
+
+gse_status_t gse_gser_copy_grid(gse_gser_t h, int which, void* dst) {
+  if (!h || !dst || which < 0 || which > 2) return fail(GSE_ERROR_INVALID_VALUE, "bad argument");
+  gse_gsek_s* c = h->core;
+  const void* src = which == 0 ? h->rho.p : (which == 1 ? c->rhoS.p : c->phiM.p);
+  HIPCHK(hipMemcpyAsync(dst, src, tsize(c->ct_dbl) * (size_t)c->X * c->Y * c->Z,
+                        hipMemcpyDeviceToDevice, c->stream));
+  return GSE_SUCCESS;
+}
+
+static gse_status_t step_outputs(const gse_step_desc_t* d, double fscale, GatherOut& o,
+                                 int* flags) {
+  if (d->virial) return fail(GSE_ERROR_NOT_SUPPORTED, "the virial is defined for GSE-k only");
+  *flags = 0;
+  if (d->forceX || d->forceY || d->forceZ) *flags |= GF_SPHERE;
+  if (d->cubeX || d->cubeY || d->cubeZ) *flags |= GF_CUBE;
+  if (d->potential) *flags |= GF_POT;
+  if ((*flags & GF_SPHERE) && !(d->forceX && d->forceY && d->forceZ))
+    return fail(GSE_ERROR_INVALID_VALUE, "forceX/Y/Z must be all set or all NULL");
+  if ((*flags & GF_CUBE) && !(d->cubeX && d->cubeY && d->cubeZ))
+    return fail(GSE_ERROR_INVALID_VALUE, "cubeX/Y/Z must be all set or all NULL");
+  o.fx = d->forceX;
+  o.fy = d->forceY;
+  o.fz = d->forceZ;
+  o.cx = d->cubeX;
+  o.cy = d->cubeY;
+  o.cz = d->cubeZ;
+  o.pot = d->potential;
+  o.fscale = o.pscale = o.vscale = fscale;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gser_step(gse_gser_t h, int n, const void* xyzq, const gse_step_desc_t* d) {
+  if (!h || !d) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  gse_gsek_s* c = h->core;
+  GatherOut o;
+  int flags = 0;
+  GSECHK(step_outputs(d, c->ccelec, o, &flags));
+  GSECHK(gse_gser_spread_charge1(h, n, xyzq));
+  GSECHK(gse_gser_spread_charge2(h));
+  GSECHK(gse_gser_solve_poisson(h));
+  GSECHK(grid_energy(c, h->rho.p, c->phiM.p, 1.0, d->energy ? d->energy : c->dout.as<double>()));
+  if (flags) GSECHK(c->eng.gather_binned(c->sigma1, c->rcut1(), flags, c->phiM.p, o, nullptr,
+                                         c->stream));
+  return GSE_SUCCESS;
+}
+
+// ------------------------------- GSE-u (CpuGSEu) ---------------------------
+gse_status_t gse_gseu_create(gse_gseu_t* out, gse_dtype_t at, gse_dtype_t ct, int N, double b,
+                             int X, int Y, int Z, double sigma, double kappa, double ls,
+                             double ls1, double bx, double by, double bz, double ccelec) {
+  if (!out) return fail(GSE_ERROR_INVALID_VALUE, "out is NULL");
+  *out = nullptr;
+  if (N <= 0 || N > 64) return fail(GSE_ERROR_INVALID_VALUE, "N must be in [1, 64] (CpuGSEu.cpp:17)");
+  if (!(b > 1.0) || !std::isfinite(b)) return fail(GSE_ERROR_INVALID_VALUE, "b must be > 1 (CpuGSEu.cpp:18)");
+  gse_gsek_t core = nullptr;
+  GSECHK(gse_gsek_create(&core, at, ct, X, Y, Z, sigma, kappa, ls, ls1, bx, by, bz, ccelec));
+  auto* h = new gse_gseu_s();
+  h->core = core;
+  h->N = N;
+  h->b = b;
+  gse_status_t st = GSE_SUCCESS;
+  const size_t gb = tsize(core->ct_dbl) * (size_t)X * Y * Z;
+  for (int j = 0; j < N && st == GSE_SUCCESS; j++) {
+    // CpuGSEu.cpp:52-56
+    const double sj = std::sqrt(sigma * sigma * std::pow(b, 2 * j) -
+                                2 * core->sigma1 * core->sigma1);
+    if (!(sj > 0) || !std::isfinite(sj)) {
+      st = fail(GSE_ERROR_INVALID_VALUE, "sigma_%d is not positive", j);
+      break;
+    }
+    h->sig.push_back(sj);
+    h->fac.push_back(std::pow(2 * M_PI * sj * sj, 3.0 / 2.0) / std::pow(sj * sj, 1.0 / 2.0));
+    auto* t3 = new gse_gseu_s::Taps3();
+    h->taps.push_back(t3);
+    st = make_ongrid_taps(core->ct_dbl, X, Y, Z, sj, ls1 * std::sqrt(2.0) * sj, bx, by, bz, t3->t,
+                          t3->R);
+    DevBuf* g = new DevBuf();
+    h->terms.push_back(g);
+    if (st == GSE_SUCCESS) st = g->ensure(gb);
+  }
+  if (st != GSE_SUCCESS) {
+    gse_gseu_destroy(h);
+    return st;
+  }
+  *out = h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gseu_destroy(gse_gseu_t h) {
+  if (!h) return GSE_SUCCESS;
+  gse_gsek_destroy(h->core);
+  for (auto* t3 : h->taps) {
+    for (int a = 0; a < 3; a++) t3->t[a].release();
+    delete t3;
+  }
+  for (DevBuf* g : h->terms) {
+    g->release();
+    delete g;
+  }
+  delete h;
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gseu_set_stream(gse_gseu_t h, void* s) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  return gse_gsek_set_stream(h->core, s);
+}
+
+gse_status_t gse_gseu_spread_charge1(gse_gseu_t h, int n, const void* xyzq) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  return gse_gsek_spread_charge1(h->core, n, xyzq);
+}
+
+gse_status_t gse_gseu_solve_poisson(gse_gseu_t h) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  gse_gsek_s* c = h->core;
+  const size_t np = (size_t)c->X * c->Y * c->Z;
+  for (int j = 0; j < h->N; j++) {
+    GSECHK(conv3(c->ct_dbl, c->X, c->Y, c->Z, h->taps[j]->R, h->taps[j]->t, c->rhoS.p,
+                 h->terms[j]->p, c->tmp, c->tmp2, c->stream));
+    HIPCHK(launch_grid_scale(c->ct_dbl, h->terms[j]->p, h->terms[j]->p, np, h->fac[j], c->stream));
+  }
+  // phiM = rhoG_{N-1} + ... + rhoG_0, added in that order (CpuGSEu.cpp:60-66)
+  HIPCHK(hipMemcpyAsync(c->phiM.p, h->terms[h->N - 1]->p, tsize(c->ct_dbl) * np,
+                        hipMemcpyDeviceToDevice, c->stream));
+  for (int j = h->N - 2; j >= 0; j--)
+    HIPCHK(launch_grid_add(c->ct_dbl, c->phiM.p, h->terms[j]->p, np, c->stream));
+  return GSE_SUCCESS;
+}
+
+gse_status_t gse_gseu_calculate_energy(gse_gseu_t h, double* energy) {
+  if (!h || !energy) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  gse_gsek_s* c = h->core;
+  GSECHK(grid_energy(c, c->rhoS.p, c->phiM.p, 2.0 * std::log(h->b), c->dout.as<double>()));
+  return host_energy(c, energy);
+}
+
+gse_status_t gse_gseu_interpolate_force(gse_gseu_t h, int n, const void* xyzq, void* fx,
+                                        void* fy, void* fz) {
+  if (!h || (n > 0 && (!fx || !fy || !fz))) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  gse_gsek_s* c = h->core;
+  GSECHK(c->eng.bin_particles(n, xyzq, c->bx, c->by, c->bz, c->sigma1, c->rcut1(), false,
+                              c->stream));
+  GatherOut o;
+  o.fx = fx;
+  o.fy = fy;
+  o.fz = fz;
+  o.fscale = 2.0 * std::log(h->b) * c->ccelec;  // CpuGSEu.cpp:78
+  return c->eng.gather_binned(c->sigma1, c->rcut1(), GF_SPHERE, c->phiM.p, o, nullptr, c->stream);
+}
+
+gse_status_t gse_gseu_copy_grid(gse_gseu_t h, int which, void* dst) {
+  if (!h) return fail(GSE_ERROR_INVALID_VALUE, "handle is NULL");
+  return gse_gsek_copy_grid(h->core, which, dst);
+}
+
+gse_status_t gse_gseu_step(gse_gseu_t h, int n, const void* xyzq, const gse_step_desc_t* d) {
+  if (!h || !d) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  gse_gsek_s* c = h->core;
+  GatherOut o;
+  int flags = 0;
+  GSECHK(step_outputs(d, 2.0 * std::log(h->b) * c->ccelec, o, &flags));
+  GSECHK(gse_gseu_spread_charge1(h, n, xyzq));
+  GSECHK(gse_gseu_solve_poisson(h));
+  GSECHK(grid_energy(c, c->rhoS.p, c->phiM.p, 2.0 * std::log(h->b),
+                     d->energy ? d->energy : c->dout.as<double>()));
+  if (flags) GSECHK(c->eng.gather_binned(c->sigma1, c->rcut1(), flags, c->phiM.p, o, nullptr,
+                                         c->stream));
   return GSE_SUCCESS;
 }
 

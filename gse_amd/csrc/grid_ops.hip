@@ -57,6 +57,7 @@ struct KArgs {
   int halfX, halfY, halfZ;
   double twopi_bx, twopi_by, twopi_bz;
   double fourpi_vol, inv_vol, sigmafac;
+  double ih2[3];  // (M_d / L_d)^2, the HERMITIAN stencil's 1/h^2
 };
 
 static KArgs make_kargs(int X, int Y, int Z, double bx, double by, double bz, double sigma) {
@@ -74,7 +75,36 @@ static KArgs make_kargs(int X, int Y, int Z, double bx, double by, double bz, do
   a.fourpi_vol = 4.0 * M_PI / (double)(X * Y * Z);
   a.inv_vol = 1.0 / (double)(X * Y * Z);
   a.sigmafac = sigma * sigma / 2.0;
+  const double ih[3] = {(double)X / bx, (double)Y / by, (double)Z / bz};
+  for (int d = 0; d < 3; d++) a.ih2[d] = ih[d] * ih[d];
   return a;
+}
+
+// The HERMITIAN (Mehrstellen) discretisation of CpuMultiGridSolver.cpp:
+// 237-251, A phi = B rho' with rho' = -4 pi rhoS, solved exactly per wave
+// vector: -4 pi B^/(A^ XYZ), zero at k = 0.  With s_d = 2 sin^2(pi i_d/M_d)
+// the symbols lose no digits near k = 0:
+//   A^ = -2 sum_d b_d s_d - 4 sum_{d<e} c_de (s_d + s_e - s_d s_e)
+//   B^ = -1 + (s_x + s_y + s_z)/6
+__device__ __forceinline__ double mehrstellen_g(const KArgs& a, size_t idx) {
+  if (idx == 0) return 0.0;
+  const int ix = (int)(idx % a.Xh);
+  const size_t t = idx / a.Xh;
+  const int iy = (int)(t % a.Y), iz = (int)(t / a.Y);
+  double sx = sin(M_PI * ix / a.X), sy = sin(M_PI * iy / a.Y), sz = sin(M_PI * iz / a.Z);
+  sx = 2.0 * sx * sx;
+  sy = 2.0 * sy * sy;
+  sz = 2.0 * sz * sz;
+  const double S = a.ih2[0] + a.ih2[1] + a.ih2[2];
+  const double bx = -5.0 / 6.0 * a.ih2[0] + S / 6.0, by = -5.0 / 6.0 * a.ih2[1] + S / 6.0,
+               bz = -5.0 / 6.0 * a.ih2[2] + S / 6.0;
+  const double cxy = -(a.ih2[0] + a.ih2[1]) / 12.0, cxz = -(a.ih2[0] + a.ih2[2]) / 12.0,
+               cyz = -(a.ih2[1] + a.ih2[2]) / 12.0;
+  const double A = -2.0 * (bx * sx + by * sy + bz * sz) -
+                   4.0 * (cxy * (sx + sy - sx * sy) + cxz * (sx + sz - sx * sz) +
+                          cyz * (sy + sz - sy * sz));
+  const double B = -1.0 + (sx + sy + sz) / 6.0;
+  return -4.0 * M_PI * B / (A * (double)a.X * a.Y * a.Z);
 }
 
 __device__ __forceinline__ void kvec(const KArgs& a, size_t idx, int& ix, double& kx, double& ky,
@@ -103,8 +133,10 @@ __global__ void k_greens_table(KArgs a, int kind, T* __restrict__ table) {
     double fac;
     if (kind == 0) {
       fac = idx == 0 ? 0.0 : a.fourpi_vol * exp(-a.sigmafac * kd2) / kd2;
-    } else {
+    } else if (kind == 1) {
       fac = a.inv_vol * exp(-a.sigmafac * kd2);
+    } else {
+      fac = mehrstellen_g(a, idx);
     }
     table[idx] = (T)fac;
   }
@@ -619,6 +651,39 @@ hipError_t launch_corrections_finalize(const double* partials, int nblocks, doub
                                        double bx, double by, double bz, double ccelec,
                                        double* out, hipStream_t s) {
   k_corrections<<<1, 256, 0, s>>>(partials, nblocks, sigma, bx * by * bz, ccelec, out);
+  return hipGetLastError();
+}
+
+template <typename T>
+__global__ void k_scale_into(T* __restrict__ out, const T* __restrict__ in, size_t n, double f) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    out[i] = in[i] * (T)f;
+}
+
+template <typename T>
+__global__ void k_add_into(T* __restrict__ out, const T* __restrict__ in, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    out[i] = out[i] + in[i];
+}
+
+hipError_t launch_grid_scale(bool dbl, void* out, const void* in, size_t n, double f,
+                             hipStream_t s) {
+  const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+  if (dbl)
+    k_scale_into<double><<<g > 0 ? g : 1, 256, 0, s>>>((double*)out, (const double*)in, n, f);
+  else
+    k_scale_into<float><<<g > 0 ? g : 1, 256, 0, s>>>((float*)out, (const float*)in, n, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_grid_add(bool dbl, void* out, const void* in, size_t n, hipStream_t s) {
+  const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+  if (dbl)
+    k_add_into<double><<<g > 0 ? g : 1, 256, 0, s>>>((double*)out, (const double*)in, n);
+  else
+    k_add_into<float><<<g > 0 ? g : 1, 256, 0, s>>>((float*)out, (const float*)in, n);
   return hipGetLastError();
 }
 

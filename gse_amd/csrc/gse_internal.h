@@ -329,6 +329,10 @@ hipError_t launch_add_sym6(const double* partials, int nblocks, double* vir9, hi
 hipError_t launch_corrections_finalize(const double* partials, int nblocks, double sigma,
                                        double bx, double by, double bz, double ccelec,
                                        double* out, hipStream_t s);
+// out[i] = f * in[i] (out may be in); out[i] += in[i]
+hipError_t launch_grid_scale(bool dbl, void* out, const void* in, size_t n, double f,
+                             hipStream_t s);
+hipError_t launch_grid_add(bool dbl, void* out, const void* in, size_t n, hipStream_t s);
 // sums n per-particle values of `ncomp` components laid out [ncomp][n]
 hipError_t launch_sum_columns(const double* v, int n, int ncomp, double* partials,
                               hipStream_t s);

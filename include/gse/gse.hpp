@@ -17,8 +17,9 @@
 //   CudaGaussCharge<AT,CT>  CudaGaussCharge.h:11-33
 //   CudaConvolution<T>      CudaConvolution.h:9-37
 // and the comparators the reference's driver runs beside GSE:
-//   CpuEwaldDirect          CpuEwaldDirect.h:11-44 (host arrays)
-//   CudaPMERecip<AT,CT>     cuda/CudaPMERecip.h:138-180 (device arrays)
+//   CpuEwaldDirect          CpuEwaldDirect.h:10-44 (host arrays)
+//   CpuEwaldRecip           CpuEwaldRecip.h:12-46 (host arrays)
+//   CudaPMERecip<AT,CT>     cuda/CudaPMERecip.h:36-182 (device arrays)
 //
 // Memory model.  As in the reference, the Cpu* classes work on HOST memory
 // owned by the caller (CpuGrid allocates with new[]).  Each call
@@ -547,7 +548,7 @@ class CudaConvolution {
   gse_convolution_t h_ = nullptr;
 };
 
-// CpuEwaldDirect.h:11-44: direct-space erfc sum within rcut plus the self
+// CpuEwaldDirect.h:10-44: direct-space erfc sum within rcut plus the self
 // energy, FP64.  Host particle and force arrays, as the reference; the
 // forces it returns are dE/dr (CpuEwaldDirect.cpp:107-117).
 class CpuEwaldDirect {
@@ -586,7 +587,41 @@ class CpuEwaldDirect {
   DeviceBuffer xyzq_, force_;
 };
 
-// cuda/CudaPMERecip.h:138-180: SPME reciprocal space on DEVICE particles
+// CpuEwaldRecip.h:12-46: the exact Ewald k-space sum, FP64, host arrays;
+// the forces are -dE/dr (CpuEwaldRecip.cpp:125-139).
+class CpuEwaldRecip {
+ public:
+  CpuEwaldRecip(const double sigma, const int kcut, const double boxx_in, const double boxy_in,
+                const double boxz_in, const double ccelec = 332.0716) {
+    check(gse_ewald_recip_create(&h_, sigma, kcut, boxx_in, boxy_in, boxz_in, ccelec));
+  }
+  ~CpuEwaldRecip() { (void)gse_ewald_recip_destroy(h_); }
+  CpuEwaldRecip(const CpuEwaldRecip&) = delete;
+  CpuEwaldRecip& operator=(const CpuEwaldRecip&) = delete;
+
+  // CpuEwaldRecip.h:35-36
+  double calcForceEnergy(const int n, const xyzq_t<double>* xyzq, double* fx, double* fy,
+                         double* fz) {
+    const void* dx = detail::upload_xyzq(xyzq_, n, xyzq);
+    const size_t fb = sizeof(double) * (size_t)(n > 0 ? n : 0);
+    char* f = (char*)force_.get(3 * fb + sizeof(double));
+    double* e = (double*)(f + 3 * fb);
+    check(gse_ewald_recip_calc_force_energy(h_, n, dx, (double*)f, (double*)(f + fb),
+                                            (double*)(f + 2 * fb), e));
+    detail::download(fx, f, n);
+    detail::download(fy, f + fb, n);
+    detail::download(fz, f + 2 * fb, n);
+    double eh = 0.0;
+    check(gse_memcpy_to_host(&eh, e, sizeof(double)));
+    return eh;
+  }
+
+ private:
+  gse_ewald_recip_t h_ = nullptr;
+  DeviceBuffer xyzq_, force_;
+};
+
+// cuda/CudaPMERecip.h:36-182: SPME reciprocal space on DEVICE particles
 // (xyzq_t<CT>, the reference's float4) and device force arrays, one GPU,
 // BOX FFT.  The energies and the virial that the reference adds into its
 // CudaEnergyVirial registry are kept here and read with getEnergy(),

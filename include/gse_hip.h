@@ -46,8 +46,17 @@ typedef enum gse_dtype { GSE_FLOAT32 = 0, GSE_FLOAT64 = 1 } gse_dtype_t;
 /* Green's function plug-ins (CpuGreensFunc.h:13-19):
  *   GSE: 4 pi/(XYZ) exp(-sigma^2 k^2/2)/k^2, k=0 -> 0 (CpuGreensFuncGSE.cpp:9-56);
  *        sigma = 0 gives the bare Coulomb kernel.
- *   G2 : exp(-sigma^2 k^2/2)/(XYZ) (CpuGreensFuncG2.cpp:9-60). */
-typedef enum gse_greens_kind { GSE_GREENS_GSE = 0, GSE_GREENS_G2 = 1 } gse_greens_kind_t;
+ *   G2 : exp(-sigma^2 k^2/2)/(XYZ) (CpuGreensFuncG2.cpp:9-60).
+ *   HERMITIAN: see below. */
+typedef enum gse_greens_kind {
+  GSE_GREENS_GSE = 0,
+  GSE_GREENS_G2 = 1,
+  /* the HERMITIAN (Mehrstellen) finite-difference Poisson operator of
+   * CpuMultiGridSolver.cpp:237-251 solved exactly: phi = A^-1 B (-4 pi rho)
+   * with zero mean (sigma unused) -- the discrete solution the reference's
+   * red-black Gauss-Seidel sweeps converge to */
+  GSE_GREENS_HERMITIAN = 2
+} gse_greens_kind_t;
 
 GSE_API const char* gse_status_string(gse_status_t status);
 GSE_API const char* gse_last_error(void);
@@ -353,6 +362,77 @@ GSE_API gse_status_t gse_gsek_corrections(gse_gsek_t h, int numCoord, const void
                                           double* out);
 
 /* ------------------------------------------------------------------------
+ * CpuGSEr<AT,CT>  (CpuGSEr.h:18-37): real-space GSE.  sigma1 spread into
+ * rho, on-grid sigma2 convolution into rhoS, the Poisson equation of the
+ * HERMITIAN stencil for -4 pi rhoS solved exactly in k-space (the reference
+ * relaxes 200 red-black Gauss-Seidel sweeps towards the same solution; the
+ * exact solve fixes the free constant of phi to zero mean), energy
+ * 1/2 ccelec h^3 sum rho phi, forces ccelec * interpolateForce(sigma1, phi).
+ * ---------------------------------------------------------------------- */
+typedef struct gse_gser_s* gse_gser_t;
+/* CpuGSEr.h:28-31  CpuGSEr(sizeX, sizeY, sizeZ, sigma, kappa, lambdaSigma,
+ * lambdaSigma1, boxx, boxy, boxz, ccelec) */
+GSE_API gse_status_t gse_gser_create(gse_gser_t* out, gse_dtype_t at, gse_dtype_t ct, int sizeX,
+                                     int sizeY, int sizeZ, double sigma, double kappa,
+                                     double lambdaSigma, double lambdaSigma1, double boxx,
+                                     double boxy, double boxz, double ccelec);
+/* CpuGSEr.h:32 */
+GSE_API gse_status_t gse_gser_destroy(gse_gser_t h);
+GSE_API gse_status_t gse_gser_set_stream(gse_gser_t h, void* hip_stream);
+/* CpuGSEr.h:33  spreadCharge1(numCoord, xyzq): sigma1 spread into rho */
+GSE_API gse_status_t gse_gser_spread_charge1(gse_gser_t h, int numCoord, const void* xyzq);
+/* CpuGSEr.h:34  spreadCharge2(): sigma2 on-grid convolution rho -> rhoS */
+GSE_API gse_status_t gse_gser_spread_charge2(gse_gser_t h);
+/* CpuGSEr.h:35  solvePoisson(): phiM from -4 pi rhoS (HERMITIAN stencil) */
+GSE_API gse_status_t gse_gser_solve_poisson(gse_gser_t h);
+/* CpuGSEr.h:36  calculateEnergy(): 1/2 ccelec h^3 sum rho phiM
+ * (CpuGSEr.cpp:157-160).  Synchronises; host result. */
+GSE_API gse_status_t gse_gser_calculate_energy(gse_gser_t h, double* energy);
+/* CpuGSE.h:61  interpolateForce: sphere force x ccelec on phiM */
+GSE_API gse_status_t gse_gser_interpolate_force(gse_gser_t h, int numCoord, const void* xyzq,
+                                                void* forceX, void* forceY, void* forceZ);
+/* device copy of rho (which = 0), rhoS (1) or phiM (2) into dst */
+GSE_API gse_status_t gse_gser_copy_grid(gse_gser_t h, int which, void* dst);
+/* Build-defined fused step: spreadCharge1, spreadCharge2, solvePoisson,
+ * energy (device) and the requested per-particle outputs (no virial). */
+GSE_API gse_status_t gse_gser_step(gse_gser_t h, int numCoord, const void* xyzq,
+                                   const gse_step_desc_t* desc);
+
+/* ------------------------------------------------------------------------
+ * CpuGSEu<AT,CT>  (CpuGSEu.h:18-42): the u-series GSE.  sigma1 spread into
+ * rhoS; phiM = sum_j (2 pi s_j^2)^(3/2)/s_j * (rhoS convolved with the
+ * on-grid Gaussian of s_j = sqrt(sigma^2 b^(2j) - 2 sigma1^2)), j < N, added
+ * from j = N-1 down; energy 2 ln(b) 1/2 ccelec h^3 sum rhoS phiM; forces
+ * 2 ln(b) ccelec interpolateForce(sigma1, phiM).  The reference writes
+ * rhoG%d.txt at every solve; this library writes no files.
+ * ---------------------------------------------------------------------- */
+typedef struct gse_gseu_s* gse_gseu_t;
+/* CpuGSEu.h:31-35  CpuGSEu(N, b, sizeX, sizeY, sizeZ, sigma, kappa,
+ * lambdaSigma, lambdaSigma1, boxx, boxy, boxz, ccelec); N in [1, 64], b > 1 */
+GSE_API gse_status_t gse_gseu_create(gse_gseu_t* out, gse_dtype_t at, gse_dtype_t ct, int N,
+                                     double b, int sizeX, int sizeY, int sizeZ, double sigma,
+                                     double kappa, double lambdaSigma, double lambdaSigma1,
+                                     double boxx, double boxy, double boxz, double ccelec);
+/* CpuGSEu.h:36 */
+GSE_API gse_status_t gse_gseu_destroy(gse_gseu_t h);
+GSE_API gse_status_t gse_gseu_set_stream(gse_gseu_t h, void* hip_stream);
+/* CpuGSEu.h:39  spreadCharge1(numCoord, xyzq) */
+GSE_API gse_status_t gse_gseu_spread_charge1(gse_gseu_t h, int numCoord, const void* xyzq);
+/* CpuGSEu.h:40  solvePoisson() */
+GSE_API gse_status_t gse_gseu_solve_poisson(gse_gseu_t h);
+/* CpuGSEu.h:41  calculateEnergy().  Synchronises; host result. */
+GSE_API gse_status_t gse_gseu_calculate_energy(gse_gseu_t h, double* energy);
+/* CpuGSEu.h:38  interpolateForce(numCoord, xyzq, forceX, forceY, forceZ) */
+GSE_API gse_status_t gse_gseu_interpolate_force(gse_gseu_t h, int numCoord, const void* xyzq,
+                                                void* forceX, void* forceY, void* forceZ);
+/* device copy of rhoS (which = 0) or phiM (1) into dst */
+GSE_API gse_status_t gse_gseu_copy_grid(gse_gseu_t h, int which, void* dst);
+/* Build-defined fused step: spreadCharge1, solvePoisson, energy (device)
+ * and the requested per-particle outputs, all scaled by 2 ln(b) (no virial). */
+GSE_API gse_status_t gse_gseu_step(gse_gseu_t h, int numCoord, const void* xyzq,
+                                   const gse_step_desc_t* desc);
+
+/* ------------------------------------------------------------------------
  * Batched independent systems (build-defined; no reference counterpart:
  * the reference runs one system per CpuGSEk object)
  *
@@ -402,7 +482,7 @@ GSE_API gse_status_t gse_batch_autotune(gse_batch_t h, const void* xyzq, int rep
 
 /* ------------------------------------------------------------------------
  * SPME reciprocal space, the comparator cpu_gse/gpu_gse run beside GSE
- * (CudaPMERecip<AT,CT,CT2>, cuda/CudaPMERecip.h:138-180; BOX FFT on one GPU).
+ * (CudaPMERecip<AT,CT,CT2>, cuda/CudaPMERecip.h:36-182; BOX FFT on one GPU).
  * The charge grid accumulates in int64 fixed point (bit-identical for any
  * particle order) and is converted to CT; FFTs are rocFFT; energy and virial
  * are FP64 sums.  ccelec is the reference's 332.0716 (CudaPMERecip.h:25).
@@ -454,7 +534,7 @@ GSE_API gse_status_t gse_pme_recip_step(gse_pme_recip_t h, const void* xyzq, int
 GSE_API gse_status_t gse_pme_recip_get_grid(gse_pme_recip_t h, void** grid);
 
 /* ------------------------------------------------------------------------
- * Ewald direct space + self energy (CpuEwaldDirect, CpuEwaldDirect.h:11-44),
+ * Ewald direct space + self energy (CpuEwaldDirect, CpuEwaldDirect.h:10-44),
  * FP64 as the reference.  Pairs within rcut (minimum image) are found
  * through cells of edge >= rcut; each thread sums its own particle's terms
  * in a fixed order, so results are deterministic.
@@ -480,6 +560,29 @@ GSE_API gse_status_t gse_ewald_direct_setup_exclusions(gse_ewald_direct_t h, int
 GSE_API gse_status_t gse_ewald_direct_calc_force_energy(gse_ewald_direct_t h, int n,
                                                         const void* xyzq, double* fx, double* fy,
                                                         double* fz, double* energy);
+
+/* ------------------------------------------------------------------------
+ * Ewald reciprocal sum (CpuEwaldRecip, CpuEwaldRecip.h:12-46), FP64: the
+ * exact k-space sum over 0 < |m| <= kcut the reference's driver compares
+ * every grid method with.  One thread per k-vector forms S(k) (particles in
+ * order), one thread per particle sums its force over the k-vectors in the
+ * reference's loop order.
+ * ---------------------------------------------------------------------- */
+typedef struct gse_ewald_recip_s* gse_ewald_recip_t;
+
+/* CpuEwaldRecip.h:29-31  CpuEwaldRecip(sigma, kcut, boxx, boxy, boxz, ccelec) */
+GSE_API gse_status_t gse_ewald_recip_create(gse_ewald_recip_t* out, double sigma, int kcut,
+                                            double boxx, double boxy, double boxz,
+                                            double ccelec);
+/* CpuEwaldRecip.h:32 */
+GSE_API gse_status_t gse_ewald_recip_destroy(gse_ewald_recip_t h);
+GSE_API gse_status_t gse_ewald_recip_set_stream(gse_ewald_recip_t h, void* hip_stream);
+/* CpuEwaldRecip.h:35-36  calcForceEnergy(n, xyzq, fx, fy, fz): device xyzq
+ * (double); fx/fy/fz (device double, all NULL: energy only) receive the
+ * forces -dE/dr_i (CpuEwaldRecip.cpp:125-139); energy (device, 1 double). */
+GSE_API gse_status_t gse_ewald_recip_calc_force_energy(gse_ewald_recip_t h, int n,
+                                                       const void* xyzq, double* fx, double* fy,
+                                                       double* fz, double* energy);
 
 #ifdef __cplusplus
 }

@@ -141,6 +141,27 @@ void orc_corrections(double sigma, double bx, double by, double bz,
                      double ccelec, int n, const double *xyzq,
                      double *e_self, double *e_net);
 
+/* ---- GSE-u and GSE-r (CpuGSEu.cpp, CpuGSEr.cpp, CpuMultiGridSolver.cpp),
+ * gse_ru_oracle.c ---- */
+/* nsweep red-black Gauss-Seidel sweeps of the HERMITIAN stencil on phi (in
+ * place), CpuMultiGridSolver.cpp:204-290; rho is the source (-4 pi rhoS) */
+void orc_rbgs_hermitian(int X, int Y, int Z, double bx, double by, double bz,
+                        const double *rho, double *phi, int nsweep);
+/* -4 pi B^/(A^ XYZ) of half-spectrum index (ix, iy, iz), 0 at k = 0 */
+double orc_mehrstellen_g(int X, int Y, int Z, double bx, double by, double bz, int ix, int iy,
+                         int iz);
+/* zero-mean exact solution of the HERMITIAN system for source -4 pi rhoS */
+void orc_mehrstellen_solve(int X, int Y, int Z, double bx, double by, double bz,
+                           const double *rhoS, double *phi);
+/* GSE-r step; nsweep > 0: the reference's relaxation with that many sweeps
+ * from phi = 0, nsweep = 0: the exact (zero-mean) solution */
+void orc_gser_step(const orc_gse_params *p, int nsweep, int n, const double *xyzq,
+                   double *energy, double *fx, double *fy, double *fz, double *rho_out,
+                   double *rhoS_out, double *phi_out);
+/* GSE-u step with N terms and base b */
+void orc_gseu_step(int N, double b, const orc_gse_params *p, int n, const double *xyzq,
+                   double *energy, double *fx, double *fy, double *fz, double *phi_out);
+
 /* ---- Ewald direct space + self energy (CpuEwaldDirect.cpp:31-131), ewald_oracle.c ----
  * Returns E_direct + E_self; fx, fy, fz receive dE/dr_i (the reference's
  * sign); exclusions in the reference's iblo14/inb14 form (n_iblo14 = 0: none). */

@@ -422,3 +422,78 @@ def pme_step(dims, order, xyzq, box, kappa, ccelec=332.0716, want_grid=False):
                        _p(Q) if want_grid else None)
     out = (e.value, vir.reshape(3, 3), f[:, :n].T.copy())
     return out + (Q,) if want_grid else out
+
+
+# ------------------------------------------------------------ GSE-u, GSE-r
+def gser_step(params, xyzq, nsweep=0):
+    """GSE-r (CpuGSEr.cpp): nsweep > 0 relaxes that many red-black sweeps
+    from phi = 0 as the reference does, nsweep = 0 solves exactly.  Returns
+    (energy, forces (n, 3) x ccelec, rho, rhoS, phi)."""
+    a, n = _xyzq(xyzq)
+    X, Y, Z = params.dims
+    e = _d()
+    f = np.zeros((3, max(n, 1)))
+    g = [np.zeros((Z, Y, X)) for _ in range(3)]
+    c = params._c()
+    lib().orc_gser_step(ctypes.byref(c), int(nsweep), n, _p(a), ctypes.byref(e), _p(f[0]),
+                        _p(f[1]), _p(f[2]), _p(g[0]), _p(g[1]), _p(g[2]))
+    return e.value, f[:, :n].T.copy(), g[0], g[1], g[2]
+
+
+def gseu_step(params, N, b, xyzq):
+    """GSE-u (CpuGSEu.cpp): returns (energy, forces (n, 3), phi)."""
+    a, n = _xyzq(xyzq)
+    X, Y, Z = params.dims
+    e = _d()
+    f = np.zeros((3, max(n, 1)))
+    phi = np.zeros((Z, Y, X))
+    c = params._c()
+    lib().orc_gseu_step(int(N), _d(b), ctypes.byref(c), n, _p(a), ctypes.byref(e), _p(f[0]),
+                        _p(f[1]), _p(f[2]), _p(phi))
+    return e.value, f[:, :n].T.copy(), phi
+
+
+def mehrstellen_solve(dims, box, rhoS):
+    X, Y, Z = dims
+    phi = np.zeros((Z, Y, X))
+    lib().orc_mehrstellen_solve(X, Y, Z, *map(_d, box), _p(_grid(rhoS)), _p(phi))
+    return phi
+
+
+def rbgs_hermitian(dims, box, rho, phi0, nsweep):
+    X, Y, Z = dims
+    phi = np.ascontiguousarray(phi0, dtype=np.float64).copy()
+    lib().orc_rbgs_hermitian(X, Y, Z, *map(_d, box), _p(_grid(rho)), _p(phi), int(nsweep))
+    return phi
+
+
+def ref_gseu_step(params, N, b, xyzq):
+    """The reference's own CpuGSEu (oracle/_ref); writes rhoG%d.txt into the
+    working directory as the reference does."""
+    a, n = _xyzq(xyzq)
+    X, Y, Z = params.dims
+    f = np.zeros((3, max(n, 1)))
+    phi = np.zeros((Z, Y, X))
+    L = ref_lib()
+    L.ref_gseu_step.restype = _d
+    e = L.ref_gseu_step(int(N), _d(b), X, Y, Z, _d(params.sigma), _d(params.kappa),
+                        _d(params.lambda_sigma), _d(params.lambda_sigma1), *map(_d, params.box),
+                        n, _p(a), _p(f[0]), _p(f[1]), _p(f[2]), _p(phi))
+    return e, f[:, :n].T.copy(), phi
+
+
+def ref_gser_step(params, xyzq):
+    """The reference's own CpuGSEr with its 200-sweep solver (oracle/_ref);
+    writes phiCoarsest.txt, rhoCoarsest.txt and resCoarsest.txt into the
+    working directory.  Returns (energy, forces, rhoS, phi)."""
+    a, n = _xyzq(xyzq)
+    X, Y, Z = params.dims
+    f = np.zeros((3, max(n, 1)))
+    rhoS = np.zeros((Z, Y, X))
+    phi = np.zeros((Z, Y, X))
+    L = ref_lib()
+    L.ref_gser_step.restype = _d
+    e = L.ref_gser_step(X, Y, Z, _d(params.sigma), _d(params.kappa), _d(params.lambda_sigma),
+                        _d(params.lambda_sigma1), *map(_d, params.box), n, _p(a), _p(f[0]),
+                        _p(f[1]), _p(f[2]), _p(rhoS), _p(phi))
+    return e, f[:, :n].T.copy(), rhoS, phi

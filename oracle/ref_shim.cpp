@@ -12,6 +12,10 @@
 //   CpuGreensFuncG2<T>     CpuGreensFuncG2.h:16-19
 //   CpuEwaldRecip          CpuEwaldRecip.h:29-31
 //   CpuEwaldDirect         CpuEwaldDirect.h:35-43
+//   CpuGSEu<AT,CT>         CpuGSEu.h:19-43 (writes rhoG%d.txt into the
+//                          working directory at every solve)
+//   CpuGSEr<AT,CT>         CpuGSEr.h, with CpuMultiGridSolver (writes
+//                          phiCoarsest.txt, rhoCoarsest.txt, resCoarsest.txt)
 // CpuFFTSolver / CpuGSEk need FFTW, which this image does not have; they are
 // not built.
 #include <cstring>
@@ -22,6 +26,8 @@
 #include "CpuGaussCharge.h"
 #include "CpuGreensFuncG2.h"
 #include "CpuGreensFuncGSE.h"
+#include "CpuGSEr.h"
+#include "CpuGSEu.h"
 
 template <typename T>
 static void to_xyzq(int n, const double* in, xyzq_t<T>* out) {
@@ -180,6 +186,44 @@ extern "C" double ref_ewald_direct(double sigma, double rcut, double bx, double 
   xyzq_t<double>* p = new xyzq_t<double>[n > 0 ? n : 1];
   to_xyzq(n, xyzq, p);
   const double e = d.calcForceEnergy(n, p, fx, fy, fz);
+  delete[] p;
+  return e;
+}
+
+// CpuGSEu driven as cpu_gse.cpp:752-760 does: spreadCharge1, solvePoisson,
+// calculateEnergy, interpolateForce; phiM copied out
+extern "C" double ref_gseu_step(int N, double b, int X, int Y, int Z, double sigma, double kappa,
+                                double ls, double ls1, double bx, double by, double bz, int n,
+                                const double* xyzq, double* fx, double* fy, double* fz,
+                                double* phi) {
+  CpuGSEu<double, double> g(N, b, X, Y, Z, sigma, kappa, ls, ls1, bx, by, bz);
+  xyzq_t<double>* p = new xyzq_t<double>[n > 0 ? n : 1];
+  to_xyzq(n, xyzq, p);
+  g.spreadCharge1(n, p);
+  g.solvePoisson();
+  const double e = g.calculateEnergy();
+  g.interpolateForce(n, p, fx, fy, fz);
+  grid_out(g.getPhi(), phi);
+  delete[] p;
+  return e;
+}
+
+// CpuGSEr driven as cpu_gse.cpp:696-707 does: spreadCharge1, spreadCharge2,
+// solvePoisson (200 red-black sweeps), calculateEnergy, interpolateForce
+extern "C" double ref_gser_step(int X, int Y, int Z, double sigma, double kappa, double ls,
+                                double ls1, double bx, double by, double bz, int n,
+                                const double* xyzq, double* fx, double* fy, double* fz,
+                                double* rhoS, double* phi) {
+  CpuGSEr<double, double> g(X, Y, Z, sigma, kappa, ls, ls1, bx, by, bz);
+  xyzq_t<double>* p = new xyzq_t<double>[n > 0 ? n : 1];
+  to_xyzq(n, xyzq, p);
+  g.spreadCharge1(n, p);
+  g.spreadCharge2();
+  g.solvePoisson();
+  const double e = g.calculateEnergy();
+  g.interpolateForce(n, p, fx, fy, fz);
+  grid_out(g.getRhoS(), rhoS);
+  grid_out(g.getPhi(), phi);
   delete[] p;
   return e;
 }
