@@ -21,10 +21,13 @@ GATHER_CUBE_FORCE = 2
 GATHER_POTENTIAL = 4
 GATHER_VIRIAL = 8
 
-LAYOUT_FIELDS = 10
+LAYOUT_FIELDS = 11
 # conversion kernel of the last spread (gse_*_get_layout field 8)
 REDUCE_PATHS = {-1: "none", 0: "atomic", 1: "generic", 2: "tile4", 3: "tile4_nt", 4: "tile2",
                 5: "tile"}
+
+# k-space form of an FFT solve (gse_*_get_layout field 10, gse_fft_solver_get_form)
+KSPACE_FORMS = {-1: "none", 0: "3d", 1: "fused"}
 
 _lib = None
 
@@ -85,6 +88,7 @@ _SIGS = {
     "gse_fft_solver_set_stream": [V, V],
     "gse_fft_solver_run": [V, V, V, D, D, D, D],
     "gse_fft_solver_r2c": [V, V, V],
+    "gse_fft_solver_get_form": [V, PI],
     "gse_fft_solver_c2r": [V, V, V],
     "gse_greens_func_mul": [I, I, I, I, I, D, D, D, D, V, V],
     "gse_gsek_create": [PV, I, I, I, I, I, D, D, D, D, D, D, D, D],

@@ -72,7 +72,8 @@ def _layout(fn, h):
     check(fn(h, info))
     v = list(info)
     return {"bin": tuple(v[0:3]), "nbins": v[3], "brick": tuple(v[4:7]), "direct": v[7],
-            "reduce": _lib.REDUCE_PATHS[v[8]], "cap_bin": v[9]}
+            "reduce": _lib.REDUCE_PATHS[v[8]], "cap_bin": v[9],
+            "kspace": _lib.KSPACE_FORMS[v[10]]}
 
 
 def _check_grid(g, dims, dt):
@@ -290,6 +291,13 @@ class CpuFFTSolver:
         check(self._L.gse_fft_solver_set_stream(self._h, _stream()))
         check(self._L.gse_fft_solver_c2r(self._h, _ptr(rhoK), _ptr(phi)))
 
+    def form(self):
+        """"fused": run() takes 2D transforms around the fused z pass;
+        "3d": 3D transforms and a separate k-pass."""
+        v = ctypes.c_int()
+        check(self._L.gse_fft_solver_get_form(self._h, ctypes.byref(v)))
+        return _lib.KSPACE_FORMS[v.value]
+
 
 class CpuGSEk:
     """CpuGSEk<AT,CT> (CpuGSEk.h:18-37, base CpuGSE.h:18-68) on the GPU, plus
@@ -468,7 +476,7 @@ def set_tiling(bin_edge=0, lds_budget=0):
 
 TUNING_KEYS = ("bin_edge", "brick_budget", "spread_block", "gather_block", "spread_stage",
                "gather_stage", "spread_lds32", "spread_cols", "spread_flush",
-               "reduce_tile", "reduce_nt", "reduce_tile2", "bin_agg")
+               "reduce_tile", "reduce_nt", "reduce_tile2", "bin_agg", "fused_kz")
 
 
 def set_tuning(**kw):

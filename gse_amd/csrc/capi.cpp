@@ -238,9 +238,10 @@ struct Engine {
   }
 
   // [Bx, By, Bz, nbins, Lx, Ly, Lz, direct, reduce path, slots per bin]
-  void layout(int* info) const {
+  void layout(int* info, int kform = -1) const {
     const int v[GSE_LAYOUT_FIELDS] = {shape.Bx, shape.By, shape.Bz, shape.nbins, shape.Lx,
-                                      shape.Ly, shape.Lz, shape.direct, reduce_path, bin.cap_bin};
+                                      shape.Ly, shape.Lz, shape.direct, reduce_path, bin.cap_bin,
+                                      kform};
     std::memcpy(info, v, sizeof(v));
   }
 
@@ -292,19 +293,41 @@ struct PlanPair {
 };
 struct PlanKey {
   int dbl, X, Y, Z, batch;
+  int dim = 3;  // 3: 3D transforms; 2: 2D x-y transforms of the Z planes
   bool operator<(const PlanKey& o) const {
-    return std::tie(dbl, X, Y, Z, batch) < std::tie(o.dbl, o.X, o.Y, o.Z, o.batch);
+    return std::tie(dbl, X, Y, Z, batch, dim) < std::tie(o.dbl, o.X, o.Y, o.Z, o.batch, o.dim);
   }
 };
 std::mutex g_plan_mu;
 std::map<PlanKey, PlanPair> g_plans;
 
 gse_status_t create_plan_pair(const PlanKey& k, PlanPair& pp) {
-  const int X = k.X, Y = k.Y, Z = k.Z, batch = k.batch;
+  const int X = k.X, Y = k.Y, Z = k.Z;
+  const int batch = k.dim == 2 ? Z : k.batch;
   const size_t lengths[3] = {(size_t)X, (size_t)Y, (size_t)Z};
   const rocfft_precision prec = k.dbl ? rocfft_precision_double : rocfft_precision_single;
   rocfft_plan_description df = nullptr, db = nullptr;
-  if (batch > 1) {
+  if (k.dim == 2) {
+    // the Z x-y planes of a [z][y][x] grid into the [z][y][x <= X/2] half
+    // spectrum: the layout the 3D r2c leaves, before its z transform
+    const size_t Xh = (size_t)X / 2 + 1;
+    const size_t rs[2] = {1, (size_t)X};
+    const size_t cs[2] = {1, Xh};
+    const size_t rd = (size_t)X * Y, cd = Xh * Y;
+    if (rocfft_plan_description_create(&df) != rocfft_status_success ||
+        rocfft_plan_description_create(&db) != rocfft_status_success ||
+        rocfft_plan_description_set_data_layout(df, rocfft_array_type_real,
+                                                rocfft_array_type_hermitian_interleaved,
+                                                nullptr, nullptr, 2, rs, rd, 2, cs, cd) !=
+            rocfft_status_success ||
+        rocfft_plan_description_set_data_layout(db, rocfft_array_type_hermitian_interleaved,
+                                                rocfft_array_type_real, nullptr, nullptr, 2, cs,
+                                                cd, 2, rs, rd) != rocfft_status_success) {
+      if (df) rocfft_plan_description_destroy(df);
+      if (db) rocfft_plan_description_destroy(db);
+      return fail(GSE_ERROR_FFT, "rocfft 2D plan description (%dx%d x %d) failed", X, Y, Z);
+    }
+  } else if (batch > 1) {
     const size_t Xh = (size_t)X / 2 + 1;
     const size_t rs[3] = {1, (size_t)X, (size_t)X * Y};
     const size_t cs[3] = {1, Xh, Xh * Y};
@@ -324,12 +347,13 @@ gse_status_t create_plan_pair(const PlanKey& k, PlanPair& pp) {
                   batch);
     }
   }
+  const size_t ndim = k.dim == 2 ? 2 : 3;
   const rocfft_status sf =
       rocfft_plan_create(&pp.fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
-                         prec, 3, lengths, (size_t)batch, df);
+                         prec, ndim, lengths, (size_t)batch, df);
   const rocfft_status sb =
       rocfft_plan_create(&pp.bwd, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
-                         prec, 3, lengths, (size_t)batch, db);
+                         prec, ndim, lengths, (size_t)batch, db);
   if (df) rocfft_plan_description_destroy(df);
   if (db) rocfft_plan_description_destroy(db);
   if (sf != rocfft_status_success || sb != rocfft_status_success) {
@@ -379,13 +403,14 @@ struct FFT3 {
   size_t work_f = 0, work_b = 0;
 
   // batch > 1: that many transforms of packed, back-to-back grids (distance
-  // X*Y*Z reals) and half spectra (distance (X/2+1)*Y*Z complex)
-  gse_status_t init(bool d, int x, int y, int z, int batch = 1) {
+  // X*Y*Z reals) and half spectra (distance (X/2+1)*Y*Z complex).
+  // dim = 2: the 2D x-y transforms of the Z planes only.
+  gse_status_t init(bool d, int x, int y, int z, int batch = 1, int dim = 3) {
     dbl = d;
     X = x;
     Y = y;
     Z = z;
-    key = PlanKey{d ? 1 : 0, x, y, z, batch};
+    key = PlanKey{d ? 1 : 0, x, y, z, batch, dim};
     PlanPair pp;
     GSECHK(acquire_plans(key, &pp));
     fwd = pp.fwd;
@@ -436,6 +461,12 @@ struct KSolver {
   bool dbl = true;
   int X = 0, Y = 0, Z = 0;
   FFT3 fft;
+  // fused form: 2D x-y transforms around one z pass (ZPlan), when Z allows it
+  bool fused = false;
+  FFT3 fft2;
+  ZPlan zp{};
+  DevBuf tw;
+  int zblocks = 0;
   DevBuf spec, table, part;
   int tab_kind = -1;
   double tab_sigma = NAN, tab_bx = NAN, tab_by = NAN, tab_bz = NAN;
@@ -450,11 +481,33 @@ struct KSolver {
     GSECHK(fft.init(d, x, y, z));
     GSECHK(spec.ensure(2 * tsize(d) * nk()));
     GSECHK(table.ensure(tsize(d) * nk()));
-    GSECHK(part.ensure(sizeof(double) * kKpassBlocks * 7));
+    size_t nparts = kKpassBlocks;
+    fused = g_tuning.fused_kz && zplan_make(z, d, &zp);
+    if (fused) {
+      GSECHK(fft2.init(d, x, y, z, 1, 2));
+      zblocks = zpass_blocks(x, y, zp);
+      nparts = std::max<size_t>(nparts, (size_t)zblocks);
+      // twiddles exp(-2 pi i m/Z), m < Z, in FP64 rounded to the compute type
+      std::vector<double> w(2 * (size_t)z);
+      for (int m = 0; m < z; m++) {
+        w[2 * m] = std::cos(2.0 * M_PI * m / z);
+        w[2 * m + 1] = -std::sin(2.0 * M_PI * m / z);
+      }
+      GSECHK(tw.ensure(2 * tsize(d) * (size_t)z));
+      if (d) {
+        HIPCHK(hipMemcpy(tw.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+      } else {
+        std::vector<float> wf(w.begin(), w.end());
+        HIPCHK(hipMemcpy(tw.p, wf.data(), sizeof(float) * wf.size(), hipMemcpyHostToDevice));
+      }
+    }
+    GSECHK(part.ensure(sizeof(double) * nparts * 7));
     return GSE_SUCCESS;
   }
   void release() {
     fft.release();
+    fft2.release();
+    tw.release();
     spec.release();
     table.release();
     part.release();
@@ -479,26 +532,35 @@ struct KSolver {
                      void* phi, double* energy, double* virial, double ccelec, hipStream_t s,
                      gse_gsek_s* marker = nullptr, GatherOut* defer = nullptr) {
     GSECHK(ensure_table(kind, sigma, bx, by, bz, s));
-    GSECHK(fft.exec(true, rho, spec.p, s));
+    FFT3& f = fused ? fft2 : fft;
+    GSECHK(f.exec(true, rho, spec.p, s));
     if (marker) GSECHK(stage_mark(marker));
-    HIPCHK(launch_kpass(dbl, kind, X, Y, Z, bx, by, bz, sigma, spec.p, table.p,
-                        virial != nullptr, part.as<double>(), s));
+    int nblocks = kKpassBlocks;
+    if (fused) {
+      HIPCHK(launch_zpass(dbl, kind, X, Y, Z, bx, by, bz, sigma, zp, spec.p, table.p, tw.p,
+                          virial != nullptr, part.as<double>(), s));
+      nblocks = zblocks;
+    } else {
+      HIPCHK(launch_kpass(dbl, kind, X, Y, Z, bx, by, bz, sigma, spec.p, table.p,
+                          virial != nullptr, part.as<double>(), s));
+    }
     if (energy || virial) {
       const double vcell = (bx / X) * (by / Y) * (bz / Z);
       const int ncomp = virial ? 7 : 1;
       if (defer) {
         defer->fin_partials = part.as<double>();
+        defer->fin_nblocks = nblocks;
         defer->fin_ncomp = ncomp;
         defer->fin_pref = 0.5 * ccelec * vcell;
         defer->fin_energy = energy;
         defer->fin_virial = virial;
       } else {
-        HIPCHK(launch_kpass_finalize(part.as<double>(), ncomp, 0.5 * ccelec * vcell, energy,
-                                     virial, s));
+        HIPCHK(launch_kpass_finalize(part.as<double>(), nblocks, ncomp, 0.5 * ccelec * vcell,
+                                     energy, virial, s));
       }
     }
     if (marker) GSECHK(stage_mark(marker));
-    GSECHK(fft.exec(false, spec.p, phi, s));
+    GSECHK(f.exec(false, spec.p, phi, s));
     if (marker) GSECHK(stage_mark(marker));
     return GSE_SUCCESS;
   }
@@ -768,6 +830,7 @@ gse_status_t gse_set_tuning_param(const char* name, int value) {
   else if (n == "reduce_tile2" && (value == 0 || value == 1)) t.reduce_tile2 = value;
   else if (n == "bin_agg" && (value == 0 || value == 1 || value == 2 || value == 4 || value == 8))
     t.bin_agg = value;
+  else if (n == "fused_kz" && (value == 0 || value == 1)) t.fused_kz = value;
   else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter or value out of range: %s=%d",
                    name, value);
   return GSE_SUCCESS;
@@ -790,6 +853,7 @@ gse_status_t gse_get_tuning_param(const char* name, int* value) {
   else if (n == "reduce_nt") *value = t.reduce_nt;
   else if (n == "reduce_tile2") *value = t.reduce_tile2;
   else if (n == "bin_agg") *value = t.bin_agg;
+  else if (n == "fused_kz") *value = t.fused_kz;
   else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter: %s", name);
   return GSE_SUCCESS;
 }
@@ -1206,6 +1270,12 @@ gse_status_t gse_fft_solver_c2r(gse_fft_solver_t h, void* rhoK, void* phi) {
   return h->ks.fft.exec(false, rhoK, phi, h->stream);
 }
 
+gse_status_t gse_fft_solver_get_form(gse_fft_solver_t h, int* form) {
+  if (!h || !form) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
+  *form = h->ks.fused ? 1 : 0;
+  return GSE_SUCCESS;
+}
+
 gse_status_t gse_greens_func_mul(gse_dtype_t t, gse_greens_kind_t kind, int X, int Y, int Z,
                                  double bx, double by, double bz, double sigma, void* rhoK,
                                  void* stream) {
@@ -1363,7 +1433,7 @@ gse_status_t gse_gsek_interpolate_force(gse_gsek_t h, int n, const void* xyzq, v
 
 gse_status_t gse_gsek_get_layout(gse_gsek_t h, int* info) {
   if (!h || !info) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
-  h->eng.layout(info);
+  h->eng.layout(info, h->ks.fused ? 1 : 0);
   return GSE_SUCCESS;
 }
 

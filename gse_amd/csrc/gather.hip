@@ -582,7 +582,7 @@ __global__ void __launch_bounds__(
   extern __shared__ __align__(32) double smem_raw[];
   const double ds = o.dscale ? *o.dscale : 1.0;
   if ((int)blockIdx.x == a.nwg_bins + kOverflowWG) {  // spare workgroup: k-pass finalize
-    kpass_finalize_block(o.fin_partials, kKpassBlocks, o.fin_ncomp, o.fin_pref, o.fin_energy,
+    kpass_finalize_block(o.fin_partials, o.fin_nblocks, o.fin_ncomp, o.fin_pref, o.fin_energy,
                          o.fin_virial);
     return;
   }
@@ -619,7 +619,7 @@ __global__ void __launch_bounds__(256)
                     BinStats* __restrict__ stats_other, const CT* __restrict__ phi, GatherOut o) {
   if (blockIdx.x == gridDim.x - 1) {  // spare workgroup: k-pass finalize
     if (o.fin_partials)
-      kpass_finalize_block(o.fin_partials, kKpassBlocks, o.fin_ncomp, o.fin_pref, o.fin_energy,
+      kpass_finalize_block(o.fin_partials, o.fin_nblocks, o.fin_ncomp, o.fin_pref, o.fin_energy,
                            o.fin_virial);
     return;
   }

@@ -187,6 +187,236 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused z pass: between a 2D r2c and a 2D c2r over the x-y planes, each
+// one-wave workgroup takes C consecutive columns (ky, kx) of the half
+// spectrum [z][y][x <= X/2] and, in LDS, runs the z transform (mixed-radix
+// Stockham, radices 2..8, twiddles exp(-2 pi i m/Z) from a table), the G(k)
+// multiply with the FP64 energy / virial partial sums of k_kpass, and the
+// inverse z transform, then stores the columns in place.  Stockham step of
+// radix p after a sub-length T (k = i mod T, j = (i - k) p + k):
+//   y_s = sum_q (x[i + q Z/p] w^(q k Z/(p T))) w^(q s Z/p),  x'[j + s T] = y_s
+// The p roots w^(r Z/p) of the inner DFT are read once per stage into
+// registers; q k Z/(p T) < Z, so the stage twiddle needs no reduction.
+// ---------------------------------------------------------------------------
+template <typename T, int P, int C>
+__device__ __forceinline__ void zfft_butterfly(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst,
+                                               const C2<T>* __restrict__ W, int Z, int Tsub,
+                                               bool inv) {
+  const int m = Z / P;
+  const int wstep = Z / (P * Tsub);
+  C2<T> root[P];
+#pragma unroll
+  for (int r = 0; r < P; r++) {
+    root[r] = W[r * m];
+    if (inv) root[r].y = -root[r].y;
+  }
+  for (int w = threadIdx.x; w < m * C; w += blockDim.x) {
+    const int i = w / C, cc = w % C;  // C is a power of two
+    const int k = i % Tsub;
+    C2<T> x[P];
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      C2<T> v = src[(i + q * m) * C + cc];
+      if (q > 0 && k > 0) {
+        C2<T> t = W[q * k * wstep];
+        if (inv) t.y = -t.y;
+        v = C2<T>{v.x * t.x - v.y * t.y, v.x * t.y + v.y * t.x};
+      }
+      x[q] = v;
+    }
+    const int j = (i - k) * P + k;
+#pragma unroll
+    for (int s = 0; s < P; s++) {
+      T yr = x[0].x, yi = x[0].y;
+#pragma unroll
+      for (int q = 1; q < P; q++) {
+        const C2<T> t = root[(q * s) % P];
+        yr += x[q].x * t.x - x[q].y * t.y;
+        yi += x[q].x * t.y + x[q].y * t.x;
+      }
+      dst[(j + s * Tsub) * C + cc] = C2<T>{yr, yi};
+    }
+  }
+}
+
+// the whole z transform; returns the buffer holding the result
+template <typename T, int C>
+__device__ C2<T>* zfft(C2<T>* a, C2<T>* b, const C2<T>* W, const ZPlan& pl, bool inv) {
+  int Tsub = 1;
+  for (int st = 0; st < pl.nst; st++) {
+    const int p = pl.p[st];
+    switch (p) {
+      case 2: zfft_butterfly<T, 2, C>(a, b, W, pl.Z, Tsub, inv); break;
+      case 3: zfft_butterfly<T, 3, C>(a, b, W, pl.Z, Tsub, inv); break;
+      case 4: zfft_butterfly<T, 4, C>(a, b, W, pl.Z, Tsub, inv); break;
+      case 5: zfft_butterfly<T, 5, C>(a, b, W, pl.Z, Tsub, inv); break;
+      case 6: zfft_butterfly<T, 6, C>(a, b, W, pl.Z, Tsub, inv); break;
+      case 7: zfft_butterfly<T, 7, C>(a, b, W, pl.Z, Tsub, inv); break;
+      default: zfft_butterfly<T, 8, C>(a, b, W, pl.Z, Tsub, inv); break;
+    }
+    __syncthreads();
+    C2<T>* t = a;
+    a = b;
+    b = t;
+    Tsub *= p;
+  }
+  return a;
+}
+
+constexpr int kZpassThreads = 256;
+constexpr int kZpassPerThread = 8;  // Z * C <= 2048 complex values per workgroup
+
+template <typename T, int C, bool VIR>
+__global__ void __launch_bounds__(kZpassThreads)
+    k_zpass(KArgs a, ZPlan pl, int kind, C2<T>* __restrict__ spec, const T* __restrict__ table,
+            const C2<T>* __restrict__ tw, double* __restrict__ partials) {
+  extern __shared__ __align__(16) char zsm[];
+  const int Z = pl.Z, total = Z * C;
+  C2<T>* A = (C2<T>*)zsm;
+  C2<T>* B = A + (size_t)Z * C;
+  C2<T>* W = B + (size_t)Z * C;
+  const int ncol = a.Xh * a.Y;
+  const size_t zs = (size_t)ncol;
+  const int col0 = blockIdx.x * C;
+  // every load of the workgroup's columns and of their G(k) values is issued
+  // before the first LDS store, so a thread has all of them in flight; the
+  // G values stay in registers for the multiply (same item per thread)
+  C2<T> v[kZpassPerThread];
+  T gk[kZpassPerThread];
+#pragma unroll
+  for (int u = 0; u < kZpassPerThread; u++) {
+    const int idx = threadIdx.x + u * kZpassThreads;
+    const int z = idx / C, c = col0 + idx % C;
+    const bool ok = idx < total && c < ncol;
+    const size_t kidx = ok ? z * zs + c : 0;
+    v[u] = ok ? spec[kidx] : C2<T>{(T)0, (T)0};
+    gk[u] = ok ? table[kidx] : (T)0;
+  }
+  for (int i = threadIdx.x; i < Z; i += kZpassThreads) W[i] = tw[i];
+#pragma unroll
+  for (int u = 0; u < kZpassPerThread; u++) {
+    const int idx = threadIdx.x + u * kZpassThreads;
+    if (idx < total) A[idx] = v[u];
+  }
+  __syncthreads();
+  C2<T>* F = zfft<T, C>(A, B, W, pl, false);
+  C2<T>* G = F == A ? B : A;
+  double e = 0.0, vxx = 0.0, vxy = 0.0, vxz = 0.0, vyy = 0.0, vyz = 0.0, vzz = 0.0;
+#pragma unroll
+  for (int u = 0; u < kZpassPerThread; u++) {
+    const int idx = threadIdx.x + u * kZpassThreads;
+    const int z = idx / C, c = col0 + idx % C;
+    if (idx >= total || c >= ncol) continue;
+    const C2<T> f = F[idx];
+    const T g = gk[u];
+    const double re = (double)f.x, im = (double)f.y;
+    F[idx] = C2<T>{f.x * g, f.y * g};
+    const int ix = c % a.Xh;
+    const double mult = (ix == 0 || 2 * ix == a.X) ? 1.0 : 2.0;
+    const double ek = mult * (double)g * (re * re + im * im);
+    e += ek;
+    if constexpr (VIR) {
+      int ixx;
+      double kx, ky, kz;
+      kvec(a, z * zs + c, ixx, kx, ky, kz);
+      const double k2 = kx * kx + ky * ky + kz * kz;
+      if (k2 > 0.0) {
+        const double cf = ek * 2.0 * (a.sigmafac + (kind == 0 ? 1.0 / k2 : 0.0));
+        vxx += cf * kx * kx;
+        vxy += cf * kx * ky;
+        vxz += cf * kx * kz;
+        vyy += cf * ky * ky;
+        vyz += cf * ky * kz;
+        vzz += cf * kz * kz;
+      }
+    }
+  }
+  __syncthreads();
+  C2<T>* R = zfft<T, C>(F, G, W, pl, true);
+#pragma unroll
+  for (int u = 0; u < kZpassPerThread; u++) {
+    const int idx = threadIdx.x + u * kZpassThreads;
+    const int z = idx / C, c = col0 + idx % C;
+    if (idx < total && c < ncol) spec[z * zs + c] = R[idx];
+  }
+  if constexpr (VIR) {
+    double acc[7] = {e, vxx, vxy, vxz, vyy, vyz, vzz};
+    block_sum_store<7>(acc, partials);
+  } else {
+    double acc[1] = {e};
+    block_sum_store<1>(acc, partials);
+  }
+}
+
+bool zplan_make(int Z, bool dbl, ZPlan* pl) {
+  // greedy, largest radix first (8 x 3 x 3 x 3 at Z = 216 measured faster
+  // than 6 x 6 x 6)
+  pl->Z = Z;
+  pl->nst = 0;
+  int m = Z;
+  while (m > 1) {
+    int p = 0;
+    for (int r : {8, 7, 6, 5, 4, 3, 2})
+      if (m % r == 0) {
+        p = r;
+        break;
+      }
+    if (!p || pl->nst == kZMaxStages) return false;
+    pl->p[pl->nst++] = p;
+    m /= p;
+  }
+  // columns per workgroup: 8 (64-byte FP32 / 128-byte FP64 pieces of a z
+  // row), 4 for longer columns; a workgroup holds at most
+  // kZpassThreads * kZpassPerThread values and two buffers fit 64 KiB of LDS
+  const size_t cplx = dbl ? 16 : 8;
+  const int maxv = kZpassThreads * kZpassPerThread;
+  int C = 8;
+  if (Z * C > maxv || (2 * (size_t)Z * C + Z) * cplx > 64 * 1024) C = 4;
+  if (Z * C > maxv || (2 * (size_t)Z * C + Z) * cplx > 64 * 1024) return false;
+  pl->C = C;
+  return true;
+}
+
+size_t zpass_lds_bytes(const ZPlan& pl, bool dbl) {
+  const size_t cplx = dbl ? 16 : 8;
+  return (2 * (size_t)pl.Z * pl.C + pl.Z) * cplx;
+}
+
+int zpass_blocks(int X, int Y, const ZPlan& pl) {
+  const int ncol = (X / 2 + 1) * Y;
+  return (ncol + pl.C - 1) / pl.C;
+}
+
+template <typename T, int C>
+static void zpass_t(const KArgs& a, const ZPlan& pl, int kind, void* spec, const void* table,
+                    const void* tw, bool vir, double* partials, unsigned g, size_t lds,
+                    hipStream_t s) {
+  if (vir)
+    k_zpass<T, C, true><<<g, kZpassThreads, lds, s>>>(a, pl, kind, (C2<T>*)spec, (const T*)table,
+                                                      (const C2<T>*)tw, partials);
+  else
+    k_zpass<T, C, false><<<g, kZpassThreads, lds, s>>>(a, pl, kind, (C2<T>*)spec, (const T*)table,
+                                                       (const C2<T>*)tw, partials);
+}
+
+hipError_t launch_zpass(bool dbl, int kind, int X, int Y, int Z, double bx, double by, double bz,
+                        double sigma, const ZPlan& pl, void* spec, const void* table,
+                        const void* tw, bool vir, double* partials, hipStream_t s) {
+  const KArgs a = make_kargs(X, Y, Z, bx, by, bz, sigma);
+  const unsigned g = (unsigned)zpass_blocks(X, Y, pl);
+  const size_t lds = zpass_lds_bytes(pl, dbl);
+  if (pl.C != 8 && pl.C != 4) return hipErrorInvalidValue;
+  if (dbl) {
+    if (pl.C == 8) zpass_t<double, 8>(a, pl, kind, spec, table, tw, vir, partials, g, lds, s);
+    else zpass_t<double, 4>(a, pl, kind, spec, table, tw, vir, partials, g, lds, s);
+  } else {
+    if (pl.C == 8) zpass_t<float, 8>(a, pl, kind, spec, table, tw, vir, partials, g, lds, s);
+    else zpass_t<float, 4>(a, pl, kind, spec, table, tw, vir, partials, g, lds, s);
+  }
+  return hipGetLastError();
+}
+
 __global__ void __launch_bounds__(256)
     k_kpass_finalize(const double* __restrict__ partials, int nblocks, int ncomp, double pref,
                      double* __restrict__ energy, double* __restrict__ virial) {
@@ -595,9 +825,9 @@ hipError_t launch_kpass(bool dbl, int kind, int X, int Y, int Z, double bx, doub
   return hipGetLastError();
 }
 
-hipError_t launch_kpass_finalize(const double* partials, int ncomp, double pref, double* energy,
-                                 double* virial, hipStream_t s) {
-  k_kpass_finalize<<<1, 256, 0, s>>>(partials, kKpassBlocks, ncomp, pref, energy, virial);
+hipError_t launch_kpass_finalize(const double* partials, int nblocks, int ncomp, double pref,
+                                 double* energy, double* virial, hipStream_t s) {
+  k_kpass_finalize<<<1, 256, 0, s>>>(partials, nblocks, ncomp, pref, energy, virial);
   return hipGetLastError();
 }
 

@@ -7,6 +7,9 @@
 
 namespace gse {
 
+// Workgroups of the k-pass (one [ncomp] partial sum each).
+constexpr int kKpassBlocksDefault = 512;
+
 // Geometry of one periodic grid and the Gaussian support on it.  The spacing
 // and its inverse are derived exactly as CpuGaussCharge.cpp:244-252 does:
 // h = box/M, inv_h = 1.0/h, n = (int)ceil(rcut/h).
@@ -49,6 +52,7 @@ struct Tuning {
   int reduce_nt = 0;           // FP32 tiled reduce: 1 non-temporal brick loads
   int reduce_tile2 = 1;        // 64-bit bricks: tiled reduce with 16-byte pieces
   int bin_agg = 4;             // particles per workgroup / 256 of the aggregated binning (0: off)
+  int fused_kz = 1;            // FFT solve as 2D x-y transforms around one fused z pass
 };
 Tuning& tuning();
 // LDS bytes for the gather's staged particles (the gather_stage knob, or its
@@ -195,6 +199,7 @@ struct GatherOut {
   const double* dscale = nullptr;  // optional device scalar on forces and potential
   // optional k-pass finalize run by one spare workgroup of the gather
   const double* fin_partials = nullptr;
+  int fin_nblocks = kKpassBlocksDefault;  // partial sums to reduce
   int fin_ncomp = 0;
   double fin_pref = 0.0;
   double* fin_energy = nullptr;
@@ -306,16 +311,35 @@ hipError_t launch_greens_table(bool dbl, int kind, int X, int Y, int Z,
 // In-place multiply of the half spectrum by the table; FP64 partial sums of
 // the energy (and, when vir is set, the six k-space virial moments) per
 // block into partials[kKpassBlocks][vir ? 7 : 1].
-constexpr int kKpassBlocks = 512;
+constexpr int kKpassBlocks = kKpassBlocksDefault;
 hipError_t launch_kpass(bool dbl, int kind, int X, int Y, int Z, double bx,
                         double by, double bz, double sigma, void* spec,
                         const void* table, bool vir, double* partials,
                         hipStream_t s);
 // energy[0] = pref*E (may be null); virial[0..8] = W^k (ncomp 7 only) with
-// pref = 0.5*ccelec*vcell.  The same reduction can run in a spare workgroup
-// of the following gather (GatherOut::fin), saving a launch.
-hipError_t launch_kpass_finalize(const double* partials, int ncomp, double pref,
+// pref = 0.5*ccelec*vcell, from nblocks partial sums.  The same reduction can
+// run in a spare workgroup of the following gather (GatherOut::fin), saving
+// a launch.
+hipError_t launch_kpass_finalize(const double* partials, int nblocks, int ncomp, double pref,
                                  double* energy, double* virial, hipStream_t s);
+
+// Fused z pass between 2D (x-y) transforms: z FFT, G(k) multiply with the
+// k-pass's energy / virial partials, inverse z FFT, per column block.
+constexpr int kZMaxStages = 8;
+struct ZPlan {
+  int Z, nst;
+  int p[kZMaxStages];  // radices 2..8, product Z
+  int C;               // spectrum columns per workgroup
+};
+// false when Z has a prime factor above 8 or the columns do not fit LDS
+bool zplan_make(int Z, bool dbl, ZPlan* pl);
+size_t zpass_lds_bytes(const ZPlan& pl, bool dbl);
+int zpass_blocks(int X, int Y, const ZPlan& pl);
+// tw: device [Z] complex exp(-2 pi i m/Z) of the compute type; partials
+// [zpass_blocks][vir ? 7 : 1]
+hipError_t launch_zpass(bool dbl, int kind, int X, int Y, int Z, double bx, double by, double bz,
+                        double sigma, const ZPlan& pl, void* spec, const void* table,
+                        const void* tw, bool vir, double* partials, hipStream_t s);
 
 // real-space reductions
 constexpr int kRedBlocks = 512;

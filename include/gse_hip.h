@@ -83,7 +83,12 @@ GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
  *   reduce_tile2  that sum over 64-bit bricks with two x points per thread (0/1, 1)
  *   bin_agg       binning claims bin slots once per workgroup and bin, with
  *                 256 x this many particles per workgroup (grids of at most
- *                 16384 bins); 0 = one claim per particle (0,1,2,4,8; 4) */
+ *                 16384 bins); 0 = one claim per particle (0,1,2,4,8; 4)
+ *   fused_kz      FFT solves of handles created afterwards run 2D x-y
+ *                 transforms around one kernel that does the z transform,
+ *                 the G(k) multiply with its energy/virial sums and the
+ *                 inverse z transform (grid sizes whose Z has no prime
+ *                 factor above 8); 0 = 3D transforms and a separate k-pass (0/1, 1) */
 GSE_API gse_status_t gse_set_tuning_param(const char* name, int value);
 GSE_API gse_status_t gse_get_tuning_param(const char* name, int* value);
 
@@ -214,8 +219,10 @@ GSE_API gse_status_t gse_gauss_charge_invalid_count(gse_gauss_charge_t h, int* c
  *       1 generic brick reduce, 2 tiled 32-bit, 3 tiled 32-bit with
  *       non-temporal loads, 4 tiled 64-bit two points per thread, 5 tiled
  *       one point per thread
- *   [9] particle slots per bin (the rest goes to the overflow list) */
-#define GSE_LAYOUT_FIELDS 10
+ *   [9] particle slots per bin (the rest goes to the overflow list)
+ *   [10] k-space form of the handle's FFT solve (gse_fft_solver_get_form):
+ *        -1 no solver (gauss-charge handles) */
+#define GSE_LAYOUT_FIELDS 11
 GSE_API gse_status_t gse_gauss_charge_get_layout(gse_gauss_charge_t h, int* info);
 
 /* ------------------------------------------------------------------------
@@ -264,6 +271,12 @@ GSE_API gse_status_t gse_fft_solver_run(gse_fft_solver_t h, void* phi, const voi
  * overwrite rhoK. */
 GSE_API gse_status_t gse_fft_solver_r2c(gse_fft_solver_t h, const void* rho, void* rhoK);
 GSE_API gse_status_t gse_fft_solver_c2r(gse_fft_solver_t h, void* rhoK, void* phi);
+/* Build-defined diagnostic (no reference counterpart): *form = 1 when run()
+ * takes 2D x-y transforms around the fused z pass (z transform, G(k)
+ * multiply and energy/virial sums, inverse z transform in one kernel), 0
+ * when it takes 3D transforms and a separate k-pass.  The form is fixed at
+ * create time by the fused_kz tuning knob and the factors of sizeZ. */
+GSE_API gse_status_t gse_fft_solver_get_form(gse_fft_solver_t h, int* form);
 
 /* CpuGreensFunc.h:16-18  mul(nfftX, nfftY, nfftZ, box, sigma, rhoK) */
 GSE_API gse_status_t gse_greens_func_mul(gse_dtype_t t, gse_greens_kind_t kind, int nfftX,

@@ -1,0 +1,149 @@
+"""The fused z pass of the FFT solve (2D x-y rocFFT transforms around one
+kernel that runs the z transform, the G(k) multiply with the FP64
+energy/virial sums and the inverse z transform) against the FP64 oracle and
+against the 3D-transform form of the same solve.
+
+Tolerances: FP64 grids/forces 1e-10 relative to max|oracle| (the Stockham z
+transform differs from rocFFT's only by rounding, ~1e-15), energies 1e-10
+relative; FP32: fused vs 3D form, force RMS (cpu_gse.cpp:131-146 Erms) 1e-5.
+"""
+import contextlib
+
+import numpy as np
+import pytest
+import torch
+
+from gse_amd import api, systems
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+F64 = torch.float64
+F32 = torch.float32
+DEV = "cuda"
+TOL = 1e-10
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def to_dev(a, dt=F64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device=DEV, dtype=dt).contiguous()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().double().cpu().numpy()
+
+
+@contextlib.contextmanager
+def fused(on):
+    old = api.get_tuning()["fused_kz"]
+    api.set_tuning(fused_kz=int(on))
+    try:
+        yield
+    finally:
+        api.set_tuning(fused_kz=old)
+
+
+# Z factorisations: radix 8 only, 8*8*2, 6*4, 7*5, 3*3*3*... (27), 2*2 with
+# odd X, and primes above 8 (no fused plan: the 3D form is kept)
+DIMS = [
+    ((16, 16, 64), (16.0, 16.5, 64.0), "fused"),
+    ((12, 10, 128), (12.0, 10.5, 128.0), "fused"),
+    ((18, 16, 24), (18.0, 16.2, 21.6), "fused"),
+    ((9, 14, 35), (9.5, 14.0, 35.0), "fused"),
+    ((20, 6, 27), (20.0, 6.5, 27.0), "fused"),
+    ((15, 9, 4), (15.0, 9.5, 4.5), "fused"),
+    ((10, 12, 11), (10.0, 12.0, 11.0), "3d"),
+    ((8, 8, 13), (8.0, 8.0, 13.0), "3d"),
+]
+
+
+@pytest.mark.parametrize("kind,sigma", [(0, 0.9487), (0, 0.0), (1, 0.9487), (2, 0.0)])
+@pytest.mark.parametrize("dims,box,form", DIMS, ids=["x".join(map(str, d[0])) for d in DIMS])
+def test_fused_solver_fp64_vs_oracle(kind, sigma, dims, box, form):
+    X, Y, Z = dims
+    rho = np.random.default_rng(X * 1000 + Z).normal(size=(Z, Y, X))
+    gf = [api.CpuGreensFuncGSE, api.CpuGreensFuncG2, api.CpuGreensFuncHermitian][kind]()
+    if kind == 2:
+        rho -= rho.mean()
+        ref = orc.mehrstellen_solve(dims, box, rho)
+    else:
+        ref = orc.fft_solve(kind, dims, sigma, box, rho)
+    out = {}
+    for on in (True, False):
+        with fused(on):
+            s = api.CpuFFTSolver(gf, F64, *dims)
+        assert s.form() == (form if on else "3d")
+        phi = torch.zeros((Z, Y, X), dtype=F64, device=DEV)
+        s.run(phi, to_dev(rho), sigma, *box)
+        out[on] = host(phi)
+        if kind == 2:
+            out[on] -= out[on].mean()
+        assert rel(out[on], ref) <= TOL, (on, rel(out[on], ref))
+    assert rel(out[True], out[False]) <= 1e-12
+
+
+STEP_CASES = [
+    ("water_64", lambda: systems.water_box(700, 27.0, seed=21), (28, 28, 64), (27.0, 27.0, 60.0)),
+    ("water_48", lambda: systems.water_box(1000, 31.0, seed=3), (32, 32, 48), (31.0,) * 3),
+    ("aniso_35", lambda: systems.water_box(150, 18.0, seed=7) * np.array([1.0, 0.9, 1.2, 1.0]),
+     (18, 16, 35), (18.0, 16.2, 21.6)),
+]
+
+
+@pytest.mark.parametrize("name,make,dims,box", STEP_CASES, ids=[c[0] for c in STEP_CASES])
+def test_fused_gsek_step_fp64_vs_oracle(name, make, dims, box):
+    xyzq = make()
+    p = orc.Params(dims, box, 2.12)
+    r = orc.gse_step(p, xyzq, mode=0)
+    res = {}
+    for on in (True, False):
+        with fused(on):
+            gs = api.CpuGSEk(F64, F64, *dims, p.sigma, 0.4, 3.0, 3.0, *box)
+        assert gs.layout()["kspace"] == ("fused" if on else "3d")
+        o = gs.step(to_dev(xyzq), mode=0, force=True, potential=True, virial=True)
+        E = float(host(o["energy"])[0])
+        assert abs(E - r.energy) <= TOL * abs(r.energy), (on, E, r.energy)
+        assert rel(host(o["force"]).T, r.force) <= TOL
+        assert rel(host(o["pot"]), r.pot) <= TOL
+        assert rel(host(o["virial"]).reshape(3, 3), r.virial) <= TOL
+        res[on] = (E, host(o["force"]), host(o["virial"]))
+        # the step repeated on the same handle gives the same result
+        o2 = gs.step(to_dev(xyzq), mode=0, force=True, potential=True, virial=True)
+        assert float(host(o2["energy"])[0]) == E
+    assert abs(res[True][0] - res[False][0]) <= 1e-12 * abs(res[False][0])
+
+
+def test_fused_gsek_fp32_matches_3d_form():
+    xyzq = systems.water_box(4000, 49.0, seed=5)
+    dims, box = (48, 48, 48), (49.0,) * 3
+    out = {}
+    for on in (True, False):
+        with fused(on):
+            gs = api.CpuGSEk(F32, F32, *dims, 2.12, 0.4, 3.0, 3.0, *box)
+        o = gs.step(to_dev(xyzq, F32), mode=0, force=True, potential=False, virial=True)
+        out[on] = (float(host(o["energy"])[0]), host(o["force"]), host(o["virial"]))
+    fa, fb = out[True][1], out[False][1]
+    erms = np.sqrt(np.sum((fa - fb) ** 2) / np.sum(fb ** 2))
+    assert erms <= 1e-5, erms
+    assert abs(out[True][0] - out[False][0]) <= 1e-5 * abs(out[False][0])
+    assert rel(out[True][2], out[False][2]) <= 1e-5
+
+
+def test_fused_gsek_mode_r_fp64():
+    xyzq = systems.water_box(300, 24.0, seed=14)
+    p = orc.Params((24, 24, 40), (24.0, 24.0, 40.0), 2.12)
+    r = orc.gse_step(p, xyzq, mode=1)
+    with fused(True):
+        gs = api.CpuGSEk(F64, F64, *p.dims, p.sigma, 0.4, 3.0, 3.0, *p.box)
+    assert gs.layout()["kspace"] == "fused"
+    o = gs.step(to_dev(xyzq), mode=1, force=True, potential=True)
+    E = float(host(o["energy"])[0])
+    assert abs(E - r.energy) <= TOL * abs(r.energy)
+    assert rel(host(o["force"]).T, r.force) <= TOL
+    assert rel(host(o["pot"]), r.pot) <= TOL
