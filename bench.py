@@ -169,6 +169,41 @@ def bench_batched(api, systems, torch, dev, nsys=256):
             "autotuned": cfg}
 
 
+def bench_rows(api, systems, torch, dev):
+    """The comparators and pipelines beside the GSE-k step, on the same 48k
+    water box (M = 80, sigma = 2.12): SPME order 4 (CudaPMERecip, kappa =
+    1/(sqrt(2) sigma)), the Ewald direct-space sum (rcut 9 A, FP64 cell
+    list), GSE-r (exact HERMITIAN solve) and GSE-u (N = 4, b = 1.168), plus
+    the exact Ewald reciprocal sum on the 3k box (kcut 10).  Steps per second
+    of each whole call, inputs resident in HBM."""
+    xyzq_np, L, M, sigma = systems.make("water_48k_M80")
+    n = len(xyzq_np)
+    out = {"workload": "water_48k_M80"}
+    kappa = 1.0 / (math.sqrt(2.0) * sigma)
+    for name, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
+        x = torch.from_numpy(xyzq_np).to(device=dev, dtype=dt).contiguous()
+        pm = api.CudaPMERecip(dt, dt, M, M, M, 4)
+        o = {}
+        s = time_steps(lambda: pm.step(x, (L, L, L), kappa, out=o), 50, 5, torch)
+        out[f"spme_order4_{name}_steps_per_s"] = round(1.0 / s, 2)
+        for cls, key, extra in ((api.CpuGSEr, "gse_r", ()), (api.CpuGSEu, "gse_u_N4", (4, 1.168))):
+            g = cls(dt, dt, *extra, M, M, M, sigma, 0.4, 3.0, 3.0, L, L, L)
+            og = {}
+            s = time_steps(lambda: g.step(x, force=True, out=og), 30, 3, torch)
+            out[f"{key}_{name}_steps_per_s"] = round(1.0 / s, 2)
+    x64 = torch.from_numpy(xyzq_np).to(device=dev, dtype=torch.float64).contiguous()
+    d = api.CpuEwaldDirect(sigma, 9.0, L, L, L)
+    f = torch.zeros((3, n), dtype=torch.float64, device=dev)
+    s = time_steps(lambda: d.calcForceEnergy(n, x64, f[0], f[1], f[2]), 20, 3, torch)
+    out["ewald_direct_rcut9_fp64_ms"] = round(s * 1e3, 4)
+    x3, L3, _, s3 = systems.make("water_3k_M32")
+    x3d = torch.from_numpy(x3).to(device=dev, dtype=torch.float64).contiguous()
+    er = api.CpuEwaldRecip(s3, 10, L3, L3, L3)
+    s = time_steps(lambda: er.calcForceEnergy(len(x3), x3d), 10, 2, torch)
+    out["ewald_recip_3k_kcut10_fp64_ms"] = round(s * 1e3, 4)
+    return out
+
+
 def host_info():
     """Core count and CPU model of this host (for the baseline record)."""
     model = "unknown"
@@ -320,6 +355,7 @@ def main():
         extras["mode_R_steps_per_s"] = round(1.0 / sm, 3)
         extras["batched"] = bench_batched(api, systems, torch, dev)
         extras["large"] = bench_large(api, systems, torch, dev)
+        extras["comparators"] = bench_rows(api, systems, torch, dev)
         line["extras"] = extras
 
     print(json.dumps(line), flush=True)

@@ -21,7 +21,7 @@ GATHER_CUBE_FORCE = 2
 GATHER_POTENTIAL = 4
 GATHER_VIRIAL = 8
 
-LAYOUT_FIELDS = 11
+LAYOUT_FIELDS = 12
 # conversion kernel of the last spread (gse_*_get_layout field 8)
 REDUCE_PATHS = {-1: "none", 0: "atomic", 1: "generic", 2: "tile4", 3: "tile4_nt", 4: "tile2",
                 5: "tile"}

@@ -73,7 +73,8 @@ def _layout(fn, h):
     v = list(info)
     return {"bin": tuple(v[0:3]), "nbins": v[3], "brick": tuple(v[4:7]), "direct": v[7],
             "reduce": _lib.REDUCE_PATHS[v[8]], "cap_bin": v[9],
-            "kspace": _lib.KSPACE_FORMS[v[10]]}
+            "kspace": _lib.KSPACE_FORMS[v[10]],
+            "spread": {-1: "none", 0: "lds_atomic", 1: "mma"}[v[11]]}
 
 
 def _check_grid(g, dims, dt):
@@ -476,7 +477,7 @@ def set_tiling(bin_edge=0, lds_budget=0):
 
 TUNING_KEYS = ("bin_edge", "brick_budget", "spread_block", "gather_block", "spread_stage",
                "gather_stage", "spread_lds32", "spread_cols", "spread_flush",
-               "reduce_tile", "reduce_nt", "reduce_tile2", "bin_agg", "fused_kz")
+               "reduce_tile", "reduce_nt", "reduce_tile2", "bin_agg", "fused_kz", "spread_mma")
 
 
 def set_tuning(**kw):

@@ -123,6 +123,7 @@ struct Engine {
   DevBuf counts[2], stats[2], ovf, perm, loc, sorted, grid64, vir6, part, bricks, bshift;
   bool brick_flush = false;     // the last spread stored bricks (k_brick_reduce converts)
   int reduce_path = -1;         // ReducePath of the last conversion (-1: none yet)
+  int spread_path = -1;         // last spread: 0 LDS integer atomics, 1 matrix cores
   int cur = 1;                  // buffer of the latest binning
   long long clean[2] = {0, 0};  // leading counts (and the stats) known zero
   bool grid64_clean = false;
@@ -218,6 +219,10 @@ struct Engine {
       bin.bricks = bricks.p;
       bin.bshift = bshift.as<int>();
     }
+    spread_path = !g_tuning.spread_cols && g_tuning.spread_mma &&
+                          spread_mma_ok(ct_dbl, geom, shape, normalize, acc32(), brick_flush)
+                      ? 1
+                      : 0;
     HIPCHK(launch_spread(ct_dbl, geom, shape, sigma, normalize, bin,
                          grid64.as<unsigned long long>(), s));
     consumed();
@@ -237,11 +242,12 @@ struct Engine {
     return GSE_SUCCESS;
   }
 
-  // [Bx, By, Bz, nbins, Lx, Ly, Lz, direct, reduce path, slots per bin]
+  // [Bx, By, Bz, nbins, Lx, Ly, Lz, direct, reduce path, slots per bin,
+  //  k-space form, spread kernel]
   void layout(int* info, int kform = -1) const {
     const int v[GSE_LAYOUT_FIELDS] = {shape.Bx, shape.By, shape.Bz, shape.nbins, shape.Lx,
                                       shape.Ly, shape.Lz, shape.direct, reduce_path, bin.cap_bin,
-                                      kform};
+                                      kform, spread_path};
     std::memcpy(info, v, sizeof(v));
   }
 
@@ -831,6 +837,7 @@ gse_status_t gse_set_tuning_param(const char* name, int value) {
   else if (n == "bin_agg" && (value == 0 || value == 1 || value == 2 || value == 4 || value == 8))
     t.bin_agg = value;
   else if (n == "fused_kz" && (value == 0 || value == 1)) t.fused_kz = value;
+  else if (n == "spread_mma" && (value == 0 || value == 1)) t.spread_mma = value;
   else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter or value out of range: %s=%d",
                    name, value);
   return GSE_SUCCESS;
@@ -854,6 +861,7 @@ gse_status_t gse_get_tuning_param(const char* name, int* value) {
   else if (n == "reduce_tile2") *value = t.reduce_tile2;
   else if (n == "bin_agg") *value = t.bin_agg;
   else if (n == "fused_kz") *value = t.fused_kz;
+  else if (n == "spread_mma") *value = t.spread_mma;
   else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter: %s", name);
   return GSE_SUCCESS;
 }

@@ -199,6 +199,76 @@ __global__ void __launch_bounds__(256)
 // The p roots w^(r Z/p) of the inner DFT are read once per stage into
 // registers; q k Z/(p T) < Z, so the stage twiddle needs no reduction.
 // ---------------------------------------------------------------------------
+// Small DFTs y_s = sum_q x_q w^(q s), w = exp(-2 pi i / P) (exp(+2 pi i / P)
+// for the inverse), in place, as radix-2 FFT steps for P = 2, 4, 8 and the
+// three-point split for P = 3; other radices use the direct sum.
+template <typename T>
+__device__ __forceinline__ C2<T> zc_add(C2<T> a, C2<T> b) {
+  return C2<T>{a.x + b.x, a.y + b.y};
+}
+template <typename T>
+__device__ __forceinline__ C2<T> zc_sub(C2<T> a, C2<T> b) {
+  return C2<T>{a.x - b.x, a.y - b.y};
+}
+// z * (-i) forward, z * (+i) inverse
+template <typename T>
+__device__ __forceinline__ C2<T> zc_rot(C2<T> z, bool inv) {
+  return inv ? C2<T>{-z.y, z.x} : C2<T>{z.y, -z.x};
+}
+template <typename T>
+__device__ __forceinline__ void zdft4(C2<T>& x0, C2<T>& x1, C2<T>& x2, C2<T>& x3, bool inv) {
+  const C2<T> a0 = zc_add(x0, x2), a1 = zc_sub(x0, x2);
+  const C2<T> a2 = zc_add(x1, x3), a3 = zc_rot(zc_sub(x1, x3), inv);
+  x0 = zc_add(a0, a2);
+  x2 = zc_sub(a0, a2);
+  x1 = zc_add(a1, a3);
+  x3 = zc_sub(a1, a3);
+}
+template <typename T, int P>
+__device__ __forceinline__ bool zdft_small(C2<T> (&x)[P], bool inv) {
+  if constexpr (P == 2) {
+    const C2<T> a = x[0];
+    x[0] = zc_add(a, x[1]);
+    x[1] = zc_sub(a, x[1]);
+    return true;
+  } else if constexpr (P == 3) {
+    const T c = (T)0.86602540378443864676;  // sqrt(3) / 2
+    const C2<T> s12 = zc_add(x[1], x[2]), d = zc_sub(x[1], x[2]);
+    const C2<T> t{x[0].x - (T)0.5 * s12.x, x[0].y - (T)0.5 * s12.y};
+    const C2<T> u = zc_rot(C2<T>{c * d.x, c * d.y}, inv);
+    x[0] = zc_add(x[0], s12);
+    x[1] = zc_add(t, u);
+    x[2] = zc_sub(t, u);
+    return true;
+  } else if constexpr (P == 4) {
+    zdft4(x[0], x[1], x[2], x[3], inv);
+    return true;
+  } else if constexpr (P == 8) {
+    const T r = (T)0.70710678118654752440;  // sqrt(1/2)
+    C2<T> e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
+    C2<T> o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
+    zdft4(e0, e1, e2, e3, inv);
+    zdft4(o0, o1, o2, o3, inv);
+    // o_k w8^k, w8 = (1 -+ i) / sqrt(2)
+    const C2<T> t1 = inv ? C2<T>{r * (o1.x - o1.y), r * (o1.x + o1.y)}
+                         : C2<T>{r * (o1.x + o1.y), r * (o1.y - o1.x)};
+    const C2<T> t2 = zc_rot(o2, inv);
+    const C2<T> t3 = inv ? C2<T>{-r * (o3.x + o3.y), r * (o3.x - o3.y)}
+                         : C2<T>{r * (o3.y - o3.x), -r * (o3.x + o3.y)};
+    x[0] = zc_add(e0, o0);
+    x[4] = zc_sub(e0, o0);
+    x[1] = zc_add(e1, t1);
+    x[5] = zc_sub(e1, t1);
+    x[2] = zc_add(e2, t2);
+    x[6] = zc_sub(e2, t2);
+    x[3] = zc_add(e3, t3);
+    x[7] = zc_sub(e3, t3);
+    return true;
+  } else {
+    return false;
+  }
+}
+
 template <typename T, int P, int C>
 __device__ __forceinline__ void zfft_butterfly(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst,
                                                const C2<T>* __restrict__ W, int Z, int Tsub,
@@ -226,16 +296,21 @@ __device__ __forceinline__ void zfft_butterfly(const C2<T>* __restrict__ src, C2
       x[q] = v;
     }
     const int j = (i - k) * P + k;
+    if (zdft_small<T, P>(x, inv)) {
 #pragma unroll
-    for (int s = 0; s < P; s++) {
-      T yr = x[0].x, yi = x[0].y;
+      for (int s = 0; s < P; s++) dst[(j + s * Tsub) * C + cc] = x[s];
+    } else {
 #pragma unroll
-      for (int q = 1; q < P; q++) {
-        const C2<T> t = root[(q * s) % P];
-        yr += x[q].x * t.x - x[q].y * t.y;
-        yi += x[q].x * t.y + x[q].y * t.x;
+      for (int s = 0; s < P; s++) {
+        T yr = x[0].x, yi = x[0].y;
+#pragma unroll
+        for (int q = 1; q < P; q++) {
+          const C2<T> t = root[(q * s) % P];
+          yr += x[q].x * t.x - x[q].y * t.y;
+          yi += x[q].x * t.y + x[q].y * t.x;
+        }
+        dst[(j + s * Tsub) * C + cc] = C2<T>{yr, yi};
       }
-      dst[(j + s * Tsub) * C + cc] = C2<T>{yr, yi};
     }
   }
 }

@@ -53,6 +53,7 @@ struct Tuning {
   int reduce_tile2 = 1;        // 64-bit bricks: tiled reduce with 16-byte pieces
   int bin_agg = 4;             // particles per workgroup / 256 of the aggregated binning (0: off)
   int fused_kz = 1;            // FFT solve as 2D x-y transforms around one fused z pass
+  int spread_mma = 0;          // FP32 spread on the matrix cores (int8 limbs, exact sums)
 };
 Tuning& tuning();
 // LDS bytes for the gather's staged particles (the gather_stage knob, or its
@@ -157,6 +158,8 @@ hipError_t launch_fixed_to_grid(bool ct_double, const unsigned long long* grid64
 
 // Brick-store flush: usable for this shape (LDS path, bounded brick overlap)?
 bool brick_reduce_ok(const Geom& g, const BinShape& b);
+bool spread_mma_ok(bool dbl, const Geom& g, const BinShape& b, bool normalize, bool acc32,
+                   bool bricks);
 // Which kernel converts a spread's fixed-point sums into the grid:
 enum ReducePath : int {
   kReduceAtomic = 0,   // bricks added with 64-bit atomics, then k_fixed_to_grid

@@ -604,6 +604,340 @@ __global__ void __launch_bounds__(BS, BS == 512 ? 6 : 1)
                                bricks ? bricks + (size_t)bin * vol : nullptr, bshift);
 }
 
+// ---------------------------------------------------------------------------
+// FP32 spread on the matrix cores.  Per bin the brick is a sum of outer
+// products,
+//   brick[(x, y), z] = sum_k A_k(x, y) B_k(z),
+//   A_k = round(q_k sA wx_k(x) wy_k(y)),  B_k = round(wz_k(z) kMmaSpan),
+// with sA = kMmaSpan / max|q|, so that |A| and B stay below kMmaSpan < 2^23,
+// each written as three signed 8-bit limbs (the limbs of v are the bytes of
+// (v + 0x808080) ^ 0x808080, v = l0 + 2^8 l1 + 2^16 l2).
+// v_mfma_i32_32x32x32_i8 sums the limb products over 32 particles per step:
+// rows are 32 (x, y) points of the brick plane, columns the z planes.  The
+// limb classes of weight 2^8 and above (li + lj >= 1) are summed exactly in
+// int32; l0 x l0 (below 2^14 per particle, against products up to 2^46) is
+// left out.  Every class sum is an exact integer sum, so the brick depends
+// only on the set of particles whose home bin this is, not on their order:
+// the bin's overflow particles are taken from the overflow list and summed
+// into the same brick, and the int64 total is rounded once to the bin's
+// int32 scale 2^(e - shift), the scale and shift of the LDS-atomic path.
+// Operand layouts (measured): A lane l -> row l % 32, B lane l -> column
+// l % 32, and byte j of lane half l / 32 is the same k slot in A and B; the
+// 16 int32 results of lane l are rows 8 (r / 4) + 4 (l / 32) + r % 4, column
+// l % 32.
+// ---------------------------------------------------------------------------
+typedef int mma_v4i __attribute__((ext_vector_type(4)));
+typedef int mma_v16i __attribute__((ext_vector_type(16)));
+
+// Brick shape of the matrix-core spread: 2n+1 = 13 points per axis in bins
+// of 8 cells, so 20^3 bricks: 400 plane points in 13 row tiles of 32, 20 z
+// planes in one column tile of 32.
+constexpr int kMmaNZC = 13, kMmaLX = 20, kMmaLXY = 400, kMmaLZ = 20;
+constexpr int kMmaTiles = (kMmaLXY + 31) / 32;
+constexpr int kMmaThreads = 1024;     // 16 waves; waves below kMmaTiles own one row tile
+constexpr int kMmaK = 32;             // particles per MFMA step
+constexpr int kMmaMaxCall = 32768;    // int32 class sums stay exact below this
+constexpr int kMmaOvfWindow = kMmaThreads;
+constexpr float kMmaSpan = 8355000.0f;  // |A|, B below this: v + 0x808080 < 2^24
+constexpr int kMmaNW = 3 * kMmaNZC;
+
+// LDS layout: A table [kMmaK][kMmaLXY] packed limbs (the finished brick
+// reuses it), B limb bytes [3][32][kMmaK], factors [kMmaK][3 NZC], staged
+// particles, cube origins, scaled charges, overflow window and its count.
+constexpr int kMmaOA = 0;
+constexpr int kMmaOB = kMmaOA + kMmaK * kMmaLXY * 4;
+constexpr int kMmaOW = kMmaOB + 3 * 32 * kMmaK;
+constexpr int kMmaOP = (kMmaOW + kMmaK * kMmaNW * 4 + 15) & ~15;
+constexpr int kMmaOO = kMmaOP + kMmaK * 16;
+constexpr int kMmaOQ = kMmaOO + kMmaK * 4;
+constexpr int kMmaOL = kMmaOQ + kMmaK * 4;
+constexpr int kMmaON = kMmaOL + kMmaOvfWindow * 4;
+constexpr int kMmaLds = kMmaON + 16;
+static_assert(kMmaLXY * kMmaLZ <= kMmaK * kMmaLXY, "the brick fits the A table");
+static_assert(kMmaLds <= 64 * 1024, "LDS budget");
+
+// bytes b of four packed-limb dwords -> one operand dword (byte t from d[t])
+__device__ __forceinline__ void mma_limbs4(unsigned d0, unsigned d1, unsigned d2, unsigned d3,
+                                           int& l0, int& l1, int& l2) {
+  const unsigned e01 = __builtin_amdgcn_perm(d1, d0, 0x05010400u);  // d0.b0 d1.b0 d0.b1 d1.b1
+  const unsigned e23 = __builtin_amdgcn_perm(d3, d2, 0x05010400u);
+  const unsigned f01 = __builtin_amdgcn_perm(d1, d0, 0x0c0c0602u);  // d0.b2 d1.b2 0 0
+  const unsigned f23 = __builtin_amdgcn_perm(d3, d2, 0x0c0c0602u);
+  l0 = (int)__builtin_amdgcn_perm(e23, e01, 0x05040100u);
+  l1 = (int)__builtin_amdgcn_perm(e23, e01, 0x07060302u);
+  l2 = (int)__builtin_amdgcn_perm(f23, f01, 0x05040100u);
+}
+
+__device__ __forceinline__ unsigned mma_pack(int v) {
+  return ((unsigned)v + 0x808080u) ^ 0x808080u;
+}
+
+// One chunk of kc <= kMmaK staged particles (positions, cube origins and
+// scaled charges in LDS): factors, the A table and B operand, then the MFMA
+// steps of this wave's row tile.  Starts and ends with a barrier.
+__device__ __forceinline__ void mma_chunk(const SpreadArgs& a, int kc, unsigned char* lds,
+                                          mma_v16i (&acc)[4]) {
+  unsigned* sA = (unsigned*)(lds + kMmaOA);
+  signed char* sB = (signed char*)(lds + kMmaOB);
+  float* sW = (float*)(lds + kMmaOW);
+  const V4<float>* sP = (const V4<float>*)(lds + kMmaOP);
+  const int* sO = (const int*)(lds + kMmaOO);
+  const float* sQ = (const float*)(lds + kMmaOQ);
+  const int tid = threadIdx.x;
+  // the spacings as values first: a per-lane choice between fields of the
+  // argument struct would become an indexed copy of the struct in scratch
+  const double hx = a.g.hx, hy = a.g.hy, hz = a.g.hz;
+  const double ihx = a.g.ihx, ihy = a.g.ihy, ihz = a.g.ihz;
+  // factors, one thread per (particle, factor): t = (c h - x) + j h in FP64
+  // (CpuGaussCharge.cpp:314-329), the exponential in FP32
+  for (int e = tid; e < kc * kMmaNW; e += kMmaThreads) {
+    const int k = e / kMmaNW, f = e - k * kMmaNW;
+    const int axis = f / kMmaNZC, j = f - axis * kMmaNZC;
+    const V4<float> p = sP[k];
+    const double xv = axis == 0 ? (double)p.x : axis == 1 ? (double)p.y : (double)p.z;
+    const double h = axis == 0 ? hx : axis == 1 ? hy : hz;
+    const double ih = axis == 0 ? ihx : axis == 1 ? ihy : ihz;
+    const int c = (int)round(xv * ih) - (kMmaNZC - 1) / 2;
+    const double t = add_rn(sub_rn(mul_rn((double)c, h), xv), mul_rn((double)j, h));
+    const float tc = (float)t;
+    sW[e] = gexp(mul_rn(mul_rn(-tc, tc), (float)a.inv2s2));
+  }
+  // clear the A rows of the chunk's particles (16-byte stores)
+  {
+    uint4* a4 = (uint4*)sA;
+    for (int i = tid; i < kc * (kMmaLXY / 4); i += kMmaThreads) a4[i] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  __syncthreads();
+  // A: one thread per (particle, y offset, half of the x offsets)
+  for (int e = tid; e < kc * kMmaNZC * 2; e += kMmaThreads) {
+    const int kr = e >> 1, hx = e & 1;
+    const int k = kr / kMmaNZC, dy = kr - k * kMmaNZC;
+    const int o = sO[k];
+    const float* w = sW + k * kMmaNW;
+    const float qy = sQ[k] * w[kMmaNZC + dy];
+    unsigned* row = sA + k * kMmaLXY + (o & 255) + kMmaLX * (((o >> 8) & 255) + dy);
+    const int dx0 = hx ? 7 : 0, dx1 = hx ? kMmaNZC : 7;
+#pragma unroll
+    for (int dx = 0; dx < 7; dx++)
+      if (dx0 + dx < dx1) row[dx0 + dx] = mma_pack((int)rintf(qy * w[dx0 + dx]));
+  }
+  // B: limb bytes [limb][z][slot]; zero outside the particle's z window and
+  // for slots past kc
+  for (int e = tid; e < 32 * kMmaK; e += kMmaThreads) {
+    const int z = e / kMmaK, k = e - z * kMmaK;
+    int v = 0;
+    if (k < kc && z < kMmaLZ) {
+      const int dz = z - (sO[k] >> 16);
+      if (dz >= 0 && dz < kMmaNZC) v = (int)rintf(sW[k * kMmaNW + 2 * kMmaNZC + dz] * kMmaSpan);
+    }
+    const unsigned pk = mma_pack(v);
+    sB[(0 * 32 + z) * kMmaK + k] = (signed char)(pk & 255u);
+    sB[(1 * 32 + z) * kMmaK + k] = (signed char)((pk >> 8) & 255u);
+    sB[(2 * 32 + z) * kMmaK + k] = (signed char)((pk >> 16) & 255u);
+  }
+  __syncthreads();
+  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+  if (wid < kMmaTiles) {
+    const int lane = tid & 63, half = lane >> 5, col = lane & 31;
+    const mma_v4i b0 = *(const mma_v4i*)(sB + (0 * 32 + col) * kMmaK + 16 * half);
+    const mma_v4i b1 = *(const mma_v4i*)(sB + (1 * 32 + col) * kMmaK + 16 * half);
+    const mma_v4i b2 = *(const mma_v4i*)(sB + (2 * 32 + col) * kMmaK + 16 * half);
+    const int m = 32 * wid + col;
+    const unsigned* ak = sA + 16 * half * kMmaLXY + m;
+    unsigned d[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) d[j] = m < kMmaLXY ? ak[j * kMmaLXY] : 0u;
+    mma_v4i a0, a1, a2;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int l0, l1, l2;
+      mma_limbs4(d[4 * q], d[4 * q + 1], d[4 * q + 2], d[4 * q + 3], l0, l1, l2);
+      a0[q] = l0;
+      a1[q] = l1;
+      a2[q] = l2;
+    }
+    // limb classes of weight 2^8 (a0 b1 + a1 b0), 2^16 (a0 b2 + a1 b1 + a2 b0),
+    // 2^24 (a1 b2 + a2 b1) and 2^32 (a2 b2)
+    acc[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, acc[0], 0, 0, 0);
+    acc[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b2, acc[1], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, acc[1], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a2, b0, acc[1], 0, 0, 0);
+    acc[2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b2, acc[2], 0, 0, 0);
+    acc[2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a2, b1, acc[2], 0, 0, 0);
+    acc[3] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a2, b2, acc[3], 0, 0, 0);
+  }
+  __syncthreads();
+}
+
+// Cube origin of a particle relative to its home bin (as k_bin stores it).
+__device__ __forceinline__ int mma_origin(const Geom& g, const BinShape& b, const V4<float>& p) {
+  const int ox = wrapi((int)round((double)p.x * g.ihx) - g.nx, g.X);
+  const int oy = wrapi((int)round((double)p.y * g.ihy) - g.ny, g.Y);
+  const int oz = wrapi((int)round((double)p.z * g.ihz) - g.nz, g.Z);
+  return (ox % b.Bx) | ((oy % b.By) << 8) | ((oz % b.Bz) << 16);
+}
+
+// A bin with more than kMmaMaxCall particles: every one of them into the
+// int64 grid at the bin's scale, one wave per particle
+__device__ __forceinline__ void mma_giant_bin(
+    const SpreadArgs& a, const V4<float>* __restrict__ xyzq,
+    const V4<float>* __restrict__ sorted, const int* __restrict__ ovf, int novf, int bin,
+    int cnt, int eg, int shift, unsigned long long* __restrict__ grid64) {
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  constexpr int nwv = kMmaThreads / 64;
+  const double S = ldexp(1.0, eg - shift);
+  for (int i = wid; i < cnt; i += nwv)
+    spread_one_global<float>(a.g, a.inv2s2, a.pref, S, 0, a.pref1,
+                             sorted[(size_t)bin * a.cap_bin + i], grid64, shift);
+  for (int i = wid; i < novf; i += nwv) {
+    const V4<float> p = xyzq[ovf[i]];
+    if (home_bin<float>(a.g, a.b, p) == bin)
+      spread_one_global<float>(a.g, a.inv2s2, a.pref, S, 0, a.pref1, p, grid64, shift);
+  }
+}
+
+// One workgroup per bin (no extra overflow workgroups: each bin sums its own
+// overflow particles).  A bin holding more than kMmaMaxCall particles takes
+// the per-contribution path instead: its particles go into the int64 grid at
+// its scale, as overflow particles of the LDS-atomic path do, and its brick
+// is marked empty.
+__global__ void __launch_bounds__(kMmaThreads)
+    k_spread_mma(SpreadArgs a, const V4<float>* __restrict__ xyzq,
+                 const V4<float>* __restrict__ sorted, const int* __restrict__ counts,
+                 int* __restrict__ counts_other, const BinStats* __restrict__ stats,
+                 BinStats* __restrict__ stats_other, const int* __restrict__ ovf,
+                 unsigned long long* __restrict__ grid64, unsigned* __restrict__ bricks,
+                 int* __restrict__ bshift, const int* __restrict__ loc) {
+  extern __shared__ __align__(16) unsigned char mma_smem[];
+  const BinStats st = *stats;
+  const int eg = fixed_exponent(st.qmax_bits, a.n - st.invalid, a.images, a.pref_bound);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    stats_other->qmax_bits = 0ull;
+    stats_other->invalid = 0;
+    stats_other->overflow = 0;
+  }
+  const int bin = xcd_remap(blockIdx.x, a.nwg_bins);
+  if (bin >= a.b.nbins) return;
+  const int call = counts[bin];
+  const int cnt = min(call, a.cap_bin);
+  if (threadIdx.x == 0) {
+    counts_other[bin] = 0;
+    if (call == 0 || call > kMmaMaxCall) bshift[bin] = -1;  // no brick this step
+  }
+  if (call == 0) return;
+  const Geom& g = a.g;
+  const BinShape& b = a.b;
+  const int ebin = fixed_exponent_bin(st.qmax_bits, call, a.pref_bound);
+  const int shift = eg > ebin ? eg - ebin : 0;
+  const int tid = threadIdx.x;
+  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+  if (call > kMmaMaxCall) {  // per-contribution path, one wave per particle
+    mma_giant_bin(a, xyzq, sorted, ovf, st.overflow, bin, cnt, eg, shift, grid64);
+    return;
+  }
+  // A = q qscale wx wy with qscale = kMmaSpan / qmax, so |A| <= kMmaSpan; the
+  // total T = sum A B is q wx wy wz at the scale qscale kMmaSpan and goes to
+  // the brick's q pref wx wy wz 2^(eg - shift), rounded once
+  union {
+    unsigned long long u;
+    double d;
+  } qv;
+  qv.u = st.qmax_bits;
+  const float qscale = qv.d > 0.0 ? (float)((double)kMmaSpan / qv.d) : 0.0f;
+  const double fct = qv.d > 0.0 ? ldexp(1.0, eg - shift) * a.pref /
+                                      ((double)qscale * (double)kMmaSpan)
+                                : 0.0;
+
+  unsigned char* lds = mma_smem;
+  V4<float>* sP = (V4<float>*)(lds + kMmaOP);
+  int* sO = (int*)(lds + kMmaOO);
+  float* sQ = (float*)(lds + kMmaOQ);
+  int* sL = (int*)(lds + kMmaOL);
+  int* sN = (int*)(lds + kMmaON);
+
+  mma_v16i acc[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[c][r] = 0;
+
+  // the bin's slots
+  const V4<float>* bs = sorted + (size_t)bin * a.cap_bin;
+  const int* bl = loc + (size_t)bin * a.cap_bin;
+  for (int c0 = 0; c0 < cnt; c0 += kMmaK) {
+    const int kc = min(kMmaK, cnt - c0);
+    if (tid < kc) {
+      const V4<float> p = bs[c0 + tid];
+      sP[tid] = p;
+      sO[tid] = bl[c0 + tid];
+      sQ[tid] = p.w * qscale;
+    }
+    __syncthreads();
+    mma_chunk(a, kc, lds, acc);
+  }
+  // the bin's overflow particles, found on the overflow list a window at a time
+  if (call > cnt) {
+    for (int w0 = 0; w0 < st.overflow; w0 += kMmaOvfWindow) {
+      if (tid == 0) *sN = 0;
+      __syncthreads();
+      const int i = w0 + tid;
+      if (i < st.overflow) {
+        const int pi = ovf[i];
+        if (home_bin<float>(g, b, xyzq[pi]) == bin) sL[atomicAdd(sN, 1)] = pi;
+      }
+      __syncthreads();
+      const int nm = *sN;
+      __syncthreads();  // every thread has read the count before it is reset
+      for (int c0 = 0; c0 < nm; c0 += kMmaK) {
+        const int kc = min(kMmaK, nm - c0);
+        if (tid < kc) {
+          const V4<float> p = xyzq[sL[c0 + tid]];
+          sP[tid] = p;
+          sO[tid] = mma_origin(g, b, p);
+          sQ[tid] = p.w * qscale;
+        }
+        __syncthreads();
+        mma_chunk(a, kc, lds, acc);
+      }
+    }
+  }
+  // exact class sums -> the brick at 2^(eg - shift): each class sum converts
+  // to FP64 exactly, the weighted sum is formed in a fixed order and rounded
+  // once (the same operations for any order of the particles)
+  unsigned* sOut = (unsigned*)(lds + kMmaOA);
+  if (wid < kMmaTiles) {
+    const int lane = tid & 63, half = lane >> 5, col = lane & 31;
+    const double f1 = fct * 256.0, f2 = fct * 65536.0, f3 = fct * 16777216.0,
+                 f4 = fct * 4294967296.0;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int m = 32 * wid + 8 * (r >> 2) + 4 * half + (r & 3);
+      if (m < kMmaLXY && col < kMmaLZ) {
+        double v = (double)acc[3][r] * f4;
+        v = __builtin_fma((double)acc[2][r], f3, v);
+        v = __builtin_fma((double)acc[1][r], f2, v);
+        v = __builtin_fma((double)acc[0][r], f1, v);
+        sOut[m + kMmaLXY * col] = (unsigned)(int)rint(v);
+      }
+    }
+  }
+  __syncthreads();
+  constexpr int vol = kMmaLXY * kMmaLZ;
+  uint4* out = (uint4*)(bricks + (size_t)bin * vol);
+  for (int j = tid; j < vol / 4; j += kMmaThreads) out[j] = ((const uint4*)sOut)[j];
+  if (tid == 0) bshift[bin] = shift;
+}
+
+// The matrix-core spread applies to FP32 grids with int32 bricks of
+// 20 x 20 x 20 (bins of 8 cells, 2n+1 = 13) and no normalisation.
+bool spread_mma_ok(bool dbl, const Geom& g, const BinShape& b, bool normalize, bool acc32,
+                   bool bricks) {
+  if (dbl || !acc32 || !bricks || normalize || b.direct) return false;
+  if (2 * g.nx + 1 != kMmaNZC || 2 * g.ny + 1 != kMmaNZC || 2 * g.nz + 1 != kMmaNZC) return false;
+  return b.Lx == kMmaLX && b.Ly * b.Lx == kMmaLXY && b.Lz == kMmaLZ;
+}
+
 // Bricks covering grid coordinate p on one axis: every (bin b, brick cell l)
 // with (b*B + l) == p mod M, l < L.  Returns the count (at most KR).
 constexpr int KR = 8;
@@ -1609,6 +1943,14 @@ static hipError_t spread_t(const Geom& g, const BinShape& b, double sigma, bool 
   }
   // stage as many particles as fit beside the brick in the workgroup's LDS
   const bool acc32 = sizeof(CT) == 4 && tu.spread_lds32;
+  if (tu.spread_mma && spread_mma_ok(sizeof(CT) == 8, g, b, normalize, acc32, w.bricks != nullptr)) {
+    a.cap = kMmaK;
+    a.nwg_bins = ((b.nbins + 7) / 8) * 8;
+    k_spread_mma<<<(unsigned)a.nwg_bins, kMmaThreads, kMmaLds, s>>>(
+        a, (const V4<float>*)w.xyzq, (const V4<float>*)w.sorted, w.counts, w.counts_other,
+        w.stats, w.stats_other, w.ovf, grid64, (unsigned*)w.bricks, w.bshift, w.loc);
+    return hipGetLastError();
+  }
   const int vol = b.Lx * b.Ly * b.Lz;
   const int vol_eff = vol * (acc32 ? 4 : 8) / 8 + 1;  // brick size in 8-byte units
   const size_t lds_total = (size_t)vol_eff * 8 + (size_t)tu.spread_stage;

@@ -88,7 +88,10 @@ GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
  *                 transforms around one kernel that does the z transform,
  *                 the G(k) multiply with its energy/virial sums and the
  *                 inverse z transform (grid sizes whose Z has no prime
- *                 factor above 8); 0 = 3D transforms and a separate k-pass (0/1, 1) */
+ *                 factor above 8); 0 = 3D transforms and a separate k-pass (0/1, 1)
+ *   spread_mma    FP32 spreads with int32 bricks and a 13-point support run
+ *                 on the matrix cores (int8 limb products summed exactly per
+ *                 bin, the brick rounded once); 0 = LDS integer atomics (0/1, 0) */
 GSE_API gse_status_t gse_set_tuning_param(const char* name, int value);
 GSE_API gse_status_t gse_get_tuning_param(const char* name, int* value);
 
@@ -221,8 +224,10 @@ GSE_API gse_status_t gse_gauss_charge_invalid_count(gse_gauss_charge_t h, int* c
  *       one point per thread
  *   [9] particle slots per bin (the rest goes to the overflow list)
  *   [10] k-space form of the handle's FFT solve (gse_fft_solver_get_form):
- *        -1 no solver (gauss-charge handles) */
-#define GSE_LAYOUT_FIELDS 11
+ *        -1 no solver (gauss-charge handles)
+ *   [11] kernel of the last binned spread: -1 none yet, 0 LDS integer
+ *        atomics, 1 matrix cores (tuning knob spread_mma) */
+#define GSE_LAYOUT_FIELDS 12
 GSE_API gse_status_t gse_gauss_charge_get_layout(gse_gauss_charge_t h, int* info);
 
 /* ------------------------------------------------------------------------
