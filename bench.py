@@ -72,6 +72,11 @@ def stage_bytes(N, M, ct_bytes):
     }
 
 
+def note(msg):
+    """Progress line on stderr (stdout carries only the result line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def time_steps(fn, steps, warmup, torch):
     for _ in range(warmup):
         fn()
@@ -285,6 +290,7 @@ def main():
 
     sec = time_steps(step, args.steps, args.warmup, torch)
     value = 1.0 / sec
+    note(f"{args.steps} timed steps: {sec * 1e3:.4f} ms per step")
 
     # per-stage HIP events over a second pass of the same number of steps
     gs.setProfiling(True)
@@ -316,6 +322,7 @@ def main():
     de = None
     if not args.no_cpu:
         cpu, erms, de = cpu_baseline(args, M, L, sigma, xyzq_np, f_gpu, e_gpu, N)
+        note("CPU baseline done")
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "steps/s", "n_gpus": 1,
@@ -349,13 +356,17 @@ def main():
         extras["stage_roofline_frac"] = {
             k: round(sb[k] / (ours[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 5) for k in sb}
         extras["conv_roofline"] = bench_conv(api, torch, dev)
+        note("FP64, adjoint and convolution extras done")
         om = {}
         sm = time_steps(lambda: gs.step(x, mode=1, force=True, potential=True, out=om),
                         max(20, args.steps // 5), 5, torch)
         extras["mode_R_steps_per_s"] = round(1.0 / sm, 3)
         extras["batched"] = bench_batched(api, systems, torch, dev)
+        note("batched extra done")
         extras["large"] = bench_large(api, systems, torch, dev)
+        note("large-system extra done")
         extras["comparators"] = bench_rows(api, systems, torch, dev)
+        note("comparator extras done")
         line["extras"] = extras
 
     print(json.dumps(line), flush=True)

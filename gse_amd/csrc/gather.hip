@@ -323,7 +323,10 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
   // then walks two charges per wave, one per 32-lane half: the per-charge
   // set-up and the cross-lane reduction are shared by two charges, and the
   // NZC^2 columns fill 32-lane slots instead of 64-lane ones.
-  constexpr bool PAIR = FMA_FORM && PL > 0 && sizeof(CT) == 4;
+  // FP64 pairs too (without the virial sums, whose extra z tables would not
+  // fit beside two charges' registers), with the reference's exact test
+  constexpr bool PAIR = FMA_FORM && PL > 0 && (sizeof(CT) == 4 || !VIR);
+  constexpr bool EXACT_SPH = VIR || sizeof(CT) == 8;
   constexpr int NXYP = NZA * NZA, KP = (NXYP + 31) / 32;
   int p_tx[KP], p_ty[KP];
 #pragma unroll
@@ -385,7 +388,7 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
         for (int t = 0; t < NZA; t++) {
           wzr[t] = w[2 * (2 * NZA + t)];
           const CT dz = w[2 * (2 * NZA + t) + 1];
-          dz2[t] = VIR ? mul_rn(dz, dz) : rc2 - mul_rn(dz, dz);
+          dz2[t] = EXACT_SPH ? mul_rn(dz, dz) : rc2 - mul_rn(dz, dz);
           vzr[t] = wzr[t] * dz;
           if constexpr (VIR) w2r[t] = wzr[t] * dz2[t];
         }
@@ -405,14 +408,16 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
 #pragma unroll
           for (int t = 0; t < NZA; t++) {
             const CT ph = brick[a0 + t * PL];
-            if (NEED_B0) B0 = __builtin_fmaf(wzr[t], ph, B0);
-            if (NEED_B1) B1 = __builtin_fmaf(vzr[t], ph, B1);
-            if constexpr (VIR) B2 = __builtin_fmaf(w2r[t], ph, B2);
+            if (NEED_B0) B0 = fmac((AT)wzr[t], (AT)ph, B0);
+            if (NEED_B1) B1 = fmac((AT)vzr[t], (AT)ph, B1);
+            if constexpr (VIR) B2 = fmac((AT)w2r[t], (AT)ph, B2);
             if (SPH) {
-              const bool in = VIR ? add_rn(dxy2, dz2[t]) < rc2 : dxy2 < dz2[t];
-              const CT ms = in ? ph : (CT)0;
-              A0 = __builtin_fmaf(wzr[t], ms, A0);
-              A1 = __builtin_fmaf(vzr[t], ms, A1);
+              // (dx^2 + dy^2) + dz^2 < rc^2 as CpuGaussCharge.cpp:157 rounds
+              // it (FP64, virial), else dx^2 + dy^2 < rc^2 - dz^2
+              const bool in = EXACT_SPH ? add_rn(dxy2, dz2[t]) < rc2 : dxy2 < dz2[t];
+              const AT ms = in ? (AT)ph : (AT)0;
+              A0 = fmac((AT)wzr[t], ms, A0);
+              A1 = fmac((AT)vzr[t], ms, A1);
             }
           }
           const AT wxy = wx * wy;
