@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--conv-only", action="store_true",
                     help="only the separable convolution timing (for counter passes)")
+    ap.add_argument("--rows-only", action="store_true",
+                    help="only the comparator timings (SPME, Ewald, GSE-r, GSE-u)")
     return ap.parse_args()
 
 
@@ -277,6 +279,9 @@ def main():
     torch.cuda.set_device(dev)
     if args.conv_only:
         print(json.dumps({"conv_roofline": bench_conv(api, torch, dev)}), flush=True)
+        return
+    if args.rows_only:
+        print(json.dumps({"comparators": bench_rows(api, systems, torch, dev)}), flush=True)
         return
     ct = torch.float32 if args.precision == "fp32" else torch.float64
     xyzq_np, L, M, sigma = systems.make(args.workload)

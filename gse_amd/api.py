@@ -477,7 +477,8 @@ def set_tiling(bin_edge=0, lds_budget=0):
 
 TUNING_KEYS = ("bin_edge", "brick_budget", "spread_block", "gather_block", "spread_stage",
                "gather_stage", "spread_lds32", "spread_cols", "spread_flush",
-               "reduce_tile", "reduce_nt", "reduce_tile2", "bin_agg", "fused_kz", "spread_mma")
+               "reduce_tile", "reduce_nt", "reduce_tile2", "bin_agg", "fused_kz", "spread_mma",
+               "zpass_cols")
 
 
 def set_tuning(**kw):

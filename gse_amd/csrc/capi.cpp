@@ -120,7 +120,8 @@ gse_status_t check_sigma_rcut(double sigma, double rcut) {
 struct Engine {
   bool ct_dbl = true, at_dbl = true;
   int X = 0, Y = 0, Z = 0;
-  DevBuf counts[2], stats[2], ovf, perm, loc, sorted, grid64, vir6, part, bricks, bshift;
+  DevBuf counts[2], stats[2], ovf, perm, loc, sorted, grid64, vir6, part, bricks, bshift,
+      binscan;
   bool brick_flush = false;     // the last spread stored bricks (k_brick_reduce converts)
   int reduce_path = -1;         // ReducePath of the last conversion (-1: none yet)
   int spread_path = -1;         // last spread: 0 LDS integer atomics, 1 matrix cores
@@ -133,7 +134,7 @@ struct Engine {
 
   void release() {
     for (DevBuf* b : {&counts[0], &counts[1], &stats[0], &stats[1], &ovf, &perm, &loc, &sorted,
-                      &grid64, &vir6, &part, &bricks, &bshift})
+                      &grid64, &vir6, &part, &bricks, &bshift, &binscan})
       b->release();
   }
 
@@ -161,6 +162,8 @@ struct Engine {
     GSECHK(loc.ensure(sizeof(int) * slots));
     GSECHK(ovf.ensure(sizeof(int) * (size_t)(n > 0 ? n : 1)));
     GSECHK(part.ensure(sizeof(double) * kRedBlocks * 8));
+    const size_t sb = bin_sort_bytes(n, nb);
+    if (sb) GSECHK(binscan.ensure(sb));
     for (int k = 0; k < 2; k++) {
       if (counts[k].bytes < sizeof(int) * (size_t)nb || !counts[k].p) {
         GSECHK(counts[k].ensure(sizeof(int) * (size_t)nb));
@@ -195,6 +198,7 @@ struct Engine {
     bin.perm = perm.as<int>();
     bin.loc = loc.as<int>();
     bin.sorted = sorted.p;
+    bin.scan = sb ? binscan.p : nullptr;
     bin.images = ((2 * g.nx + 1 + X - 1) / X) * ((2 * g.ny + 1 + Y - 1) / Y) *
                  ((2 * g.nz + 1 + Z - 1) / Z);
     bin.pref_bound = std::pow(2.0 * M_PI * sigma * sigma, -1.5) *
@@ -837,6 +841,7 @@ gse_status_t gse_set_tuning_param(const char* name, int value) {
   else if (n == "bin_agg" && (value == 0 || value == 1 || value == 2 || value == 4 || value == 8))
     t.bin_agg = value;
   else if (n == "fused_kz" && (value == 0 || value == 1)) t.fused_kz = value;
+  else if (n == "zpass_cols" && (value == 4 || value == 8 || value == 16)) t.zpass_cols = value;
   else if (n == "spread_mma" && (value == 0 || value == 1)) t.spread_mma = value;
   else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter or value out of range: %s=%d",
                    name, value);
@@ -861,6 +866,7 @@ gse_status_t gse_get_tuning_param(const char* name, int* value) {
   else if (n == "reduce_tile2") *value = t.reduce_tile2;
   else if (n == "bin_agg") *value = t.bin_agg;
   else if (n == "fused_kz") *value = t.fused_kz;
+  else if (n == "zpass_cols") *value = t.zpass_cols;
   else if (n == "spread_mma") *value = t.spread_mma;
   else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter: %s", name);
   return GSE_SUCCESS;

@@ -441,13 +441,14 @@ bool zplan_make(int Z, bool dbl, ZPlan* pl) {
     pl->p[pl->nst++] = p;
     m /= p;
   }
-  // columns per workgroup: 8 (64-byte FP32 / 128-byte FP64 pieces of a z
-  // row), 4 for longer columns; a workgroup holds at most
-  // kZpassThreads * kZpassPerThread values and two buffers fit 64 KiB of LDS
+  // columns per workgroup: the zpass_cols knob (4, 8 or 16; 8: 64-byte FP32 /
+  // 128-byte FP64 pieces of a z row), halved while a workgroup would hold
+  // more than kZpassThreads * kZpassPerThread values or its two buffers
+  // would pass 64 KiB of LDS
   const size_t cplx = dbl ? 16 : 8;
   const int maxv = kZpassThreads * kZpassPerThread;
-  int C = 8;
-  if (Z * C > maxv || (2 * (size_t)Z * C + Z) * cplx > 64 * 1024) C = 4;
+  int C = tuning().zpass_cols;
+  while (C > 4 && (Z * C > maxv || (2 * (size_t)Z * C + Z) * cplx > 64 * 1024)) C >>= 1;
   if (Z * C > maxv || (2 * (size_t)Z * C + Z) * cplx > 64 * 1024) return false;
   pl->C = C;
   return true;
@@ -481,12 +482,14 @@ hipError_t launch_zpass(bool dbl, int kind, int X, int Y, int Z, double bx, doub
   const KArgs a = make_kargs(X, Y, Z, bx, by, bz, sigma);
   const unsigned g = (unsigned)zpass_blocks(X, Y, pl);
   const size_t lds = zpass_lds_bytes(pl, dbl);
-  if (pl.C != 8 && pl.C != 4) return hipErrorInvalidValue;
+  if (pl.C != 16 && pl.C != 8 && pl.C != 4) return hipErrorInvalidValue;
   if (dbl) {
-    if (pl.C == 8) zpass_t<double, 8>(a, pl, kind, spec, table, tw, vir, partials, g, lds, s);
+    if (pl.C == 16) zpass_t<double, 16>(a, pl, kind, spec, table, tw, vir, partials, g, lds, s);
+    else if (pl.C == 8) zpass_t<double, 8>(a, pl, kind, spec, table, tw, vir, partials, g, lds, s);
     else zpass_t<double, 4>(a, pl, kind, spec, table, tw, vir, partials, g, lds, s);
   } else {
-    if (pl.C == 8) zpass_t<float, 8>(a, pl, kind, spec, table, tw, vir, partials, g, lds, s);
+    if (pl.C == 16) zpass_t<float, 16>(a, pl, kind, spec, table, tw, vir, partials, g, lds, s);
+    else if (pl.C == 8) zpass_t<float, 8>(a, pl, kind, spec, table, tw, vir, partials, g, lds, s);
     else zpass_t<float, 4>(a, pl, kind, spec, table, tw, vir, partials, g, lds, s);
   }
   return hipGetLastError();

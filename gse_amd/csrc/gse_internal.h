@@ -53,6 +53,7 @@ struct Tuning {
   int reduce_tile2 = 1;        // 64-bit bricks: tiled reduce with 16-byte pieces
   int bin_agg = 4;             // particles per workgroup / 256 of the aggregated binning (0: off)
   int fused_kz = 1;            // FFT solve as 2D x-y transforms around one fused z pass
+  int zpass_cols = 8;          // spectrum columns per z-pass workgroup (4, 8, 16)
   int spread_mma = 0;          // FP32 spread on the matrix cores (int8 limbs, exact sums)
 };
 Tuning& tuning();
@@ -92,6 +93,9 @@ struct Binning {
   double pref_bound = 0.0;     // bound of |q|-free contribution: pref * max weight
   void* bricks = nullptr;      // [nbins][Lz][Ly][Lx] LDS bricks (NULL: atomic flush)
   int* bshift = nullptr;       // [nbins] brick shift to 2^e, -1 for an empty bin
+  // counting-sort scratch of grids above kBinAggMaxBins bins (bin_sort_bytes):
+  // per-workgroup 16-bit bin counts, then their 32-bit exclusive offsets
+  void* scan = nullptr;
 };
 
 // Fixed-point exponent e (scale 2^e) of one spread: every contribution
@@ -141,6 +145,8 @@ hipError_t noalloc_scratch(void** out);
 // ct_double: particle/grid compute type is double (else float)
 // Clears counts[0..nbins) and the statistics of one buffer.
 hipError_t launch_bin_clear(int* counts, int nbins, BinStats* stats, hipStream_t s);
+// bytes of Binning::scan for n particles and nbins bins (0: not needed)
+size_t bin_sort_bytes(int n, int nbins);
 hipError_t launch_bin(bool ct_double, const Geom& g, const BinShape& b, Binning& w,
                       hipStream_t s);
 

@@ -207,6 +207,150 @@ __global__ void __launch_bounds__(BT)
   }
 }
 
+// Binning of grids with more than kBinAggMaxBins bins, as a counting sort in
+// three passes with no global atomics on the bin counters (a workgroup's
+// randomly ordered charges touch most of the bins, so the claims of
+// k_bin_agg would hardly aggregate):
+//   k_bin_hist     per workgroup of kBinSortChunk consecutive charges, the
+//                  charge count of every bin (16-bit, two per LDS dword)
+//   k_bin_offsets  per bin, the exclusive sum of those counts over the
+//                  workgroups (the workgroup's first slot in the bin) and
+//                  the bin's total
+//   k_bin_place    each workgroup ranks its charges within their bins in LDS
+//                  and stores them at first slot + rank
+// The slots a charge gets differ from k_bin's only in order within a bin;
+// the spread and gather do not depend on that order.
+constexpr int kBinSortThreads = 1024;
+constexpr int kBinSortPPT = 8;
+constexpr int kBinSortChunk = kBinSortThreads * kBinSortPPT;
+constexpr int kBinSortMaxBins = 32768;  // 16-bit counters in 64 KiB of LDS
+
+size_t bin_sort_bytes(int n, int nbins) {
+  if (n <= 0 || nbins <= kBinAggMaxBins || nbins > kBinSortMaxBins) return 0;
+  const size_t nwg = (size_t)(n + kBinSortChunk - 1) / kBinSortChunk;
+  const size_t h = (nwg * nbins * 2 + 255) / 256 * 256;
+  return h + nwg * nbins * 4;
+}
+
+// bin and in-bin cube origin of a particle as k_bin computes them; false for
+// a rejected particle
+template <typename CT>
+__device__ __forceinline__ bool bin_locate(const Geom& g, const BinShape& b, const V4<CT>& p,
+                                           int& bin, int& lc) {
+  const double fx = (double)p.x * g.ihx;
+  const double fy = (double)p.y * g.ihy;
+  const double fz = (double)p.z * g.ihz;
+  const double lim = 1.0e9;
+  if (!(fabs(fx) < lim && fabs(fy) < lim && fabs(fz) < lim && isfinite((double)p.w)))
+    return false;
+  const int ox = wrapi((int)round(fx) - g.nx, g.X);
+  const int oy = wrapi((int)round(fy) - g.ny, g.Y);
+  const int oz = wrapi((int)round(fz) - g.nz, g.Z);
+  const int qx = ox / b.Bx, qy = oy / b.By, qz = oz / b.Bz;
+  bin = qx + b.nbx * (qy + b.nby * qz);
+  lc = (ox - qx * b.Bx) | ((oy - qy * b.By) << 8) | ((oz - qz * b.Bz) << 16);
+  return true;
+}
+
+template <typename CT>
+__global__ void __launch_bounds__(kBinSortThreads)
+    k_bin_hist(Geom g, BinShape b, int n, const V4<CT>* __restrict__ xyzq,
+               unsigned short* __restrict__ hist, BinStats* __restrict__ stats) {
+  __shared__ unsigned h32[kBinSortMaxBins / 2];
+  __shared__ double wmax[kBinSortThreads / 64];
+  __shared__ int winv[kBinSortThreads / 64];
+  const int nh = (b.nbins + 1) >> 1;
+  for (int j = threadIdx.x; j < nh; j += kBinSortThreads) h32[j] = 0u;
+  __syncthreads();
+  const int i0 = blockIdx.x * kBinSortChunk;
+  double aq = 0.0;
+  int ninv = 0;
+#pragma unroll
+  for (int u = 0; u < kBinSortPPT; u++) {
+    const int i = i0 + u * kBinSortThreads + (int)threadIdx.x;
+    if (i >= n) continue;
+    const V4<CT> p = xyzq[i];
+    int bin, lc;
+    if (bin_locate<CT>(g, b, p, bin, lc)) {
+      atomicAdd(&h32[bin >> 1], 1u << (16 * (bin & 1)));
+      aq = fmax(aq, fabs((double)p.w));
+    } else {
+      ninv++;
+    }
+  }
+  __syncthreads();
+  unsigned short* row = hist + (size_t)blockIdx.x * b.nbins;
+  for (int j = threadIdx.x; j < b.nbins; j += kBinSortThreads)
+    row[j] = (unsigned short)((h32[j >> 1] >> (16 * (j & 1))) & 0xffffu);
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    aq = fmax(aq, __shfl_xor(aq, m, kWave));
+    ninv += __shfl_xor(ninv, m, kWave);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    wmax[threadIdx.x >> 6] = aq;
+    winv[threadIdx.x >> 6] = ninv;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    int inv = 0;
+    for (int w = 0; w < kBinSortThreads / 64; w++) {
+      m = fmax(m, wmax[w]);
+      inv += winv[w];
+    }
+    if (m > 0.0) atomicMax(&stats->qmax_bits, (unsigned long long)__double_as_longlong(m));
+    if (inv > 0) atomicAdd(&stats->invalid, inv);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_bin_offsets(int nbins, int nwg, const unsigned short* __restrict__ hist,
+                  unsigned* __restrict__ offs, int* __restrict__ counts) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nbins) return;
+  unsigned run = 0;
+  for (int w = 0; w < nwg; w++) {
+    const size_t at = (size_t)w * nbins + j;
+    offs[at] = run;
+    run += hist[at];
+  }
+  counts[j] = (int)run;
+}
+
+template <typename CT>
+__global__ void __launch_bounds__(kBinSortThreads)
+    k_bin_place(Geom g, BinShape b, int n, int cap_bin, const V4<CT>* __restrict__ xyzq,
+                const unsigned* __restrict__ offs, BinStats* __restrict__ stats,
+                int* __restrict__ ovf, int* __restrict__ perm, int* __restrict__ loc,
+                V4<CT>* __restrict__ sorted) {
+  __shared__ unsigned h32[kBinSortMaxBins / 2];
+  const int nh = (b.nbins + 1) >> 1;
+  for (int j = threadIdx.x; j < nh; j += kBinSortThreads) h32[j] = 0u;
+  __syncthreads();
+  const int i0 = blockIdx.x * kBinSortChunk;
+  const unsigned* orow = offs + (size_t)blockIdx.x * b.nbins;
+#pragma unroll
+  for (int u = 0; u < kBinSortPPT; u++) {
+    const int i = i0 + u * kBinSortThreads + (int)threadIdx.x;
+    if (i >= n) continue;
+    const V4<CT> p = xyzq[i];
+    int bin, lc;
+    if (!bin_locate<CT>(g, b, p, bin, lc)) continue;
+    const int sh = 16 * (bin & 1);
+    const int rank = (int)((atomicAdd(&h32[bin >> 1], 1u << sh) >> sh) & 0xffffu);
+    const long long slot = (long long)orow[bin] + rank;
+    if (slot < cap_bin) {
+      const size_t pos = (size_t)bin * cap_bin + (size_t)slot;
+      sorted[pos] = p;
+      perm[pos] = i;
+      loc[pos] = lc;
+    } else {
+      ovf[atomicAdd(&stats->overflow, 1)] = i;
+    }
+  }
+}
+
 struct LaneRole {
   int axis;  // 0,1,2 or -1
   int j;
@@ -1861,6 +2005,28 @@ hipError_t launch_bin(bool dbl, const Geom& g, const BinShape& b, Binning& w, hi
     }
 #undef GSE_BIN_AGG_T
 #undef GSE_BIN_AGG
+    return hipGetLastError();
+  }
+  if (w.n > 0 && w.scan && b.nbins > kBinAggMaxBins && b.nbins <= kBinSortMaxBins && agg > 0) {
+    const int nwg = (w.n + kBinSortChunk - 1) / kBinSortChunk;
+    unsigned short* hist = (unsigned short*)w.scan;
+    unsigned* offs = (unsigned*)((char*)w.scan +
+                                 ((size_t)nwg * b.nbins * 2 + 255) / 256 * 256);
+    if (dbl)
+      k_bin_hist<double><<<nwg, kBinSortThreads, 0, s>>>(g, b, w.n, (const V4<double>*)w.xyzq,
+                                                          hist, w.stats);
+    else
+      k_bin_hist<float><<<nwg, kBinSortThreads, 0, s>>>(g, b, w.n, (const V4<float>*)w.xyzq,
+                                                         hist, w.stats);
+    k_bin_offsets<<<(b.nbins + 255) / 256, 256, 0, s>>>(b.nbins, nwg, hist, offs, w.counts);
+    if (dbl)
+      k_bin_place<double><<<nwg, kBinSortThreads, 0, s>>>(
+          g, b, w.n, w.cap_bin, (const V4<double>*)w.xyzq, offs, w.stats, w.ovf, w.perm, w.loc,
+          (V4<double>*)w.sorted);
+    else
+      k_bin_place<float><<<nwg, kBinSortThreads, 0, s>>>(
+          g, b, w.n, w.cap_bin, (const V4<float>*)w.xyzq, offs, w.stats, w.ovf, w.perm, w.loc,
+          (V4<float>*)w.sorted);
     return hipGetLastError();
   }
   if (w.n > 0) {

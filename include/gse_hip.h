@@ -89,6 +89,8 @@ GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
  *                 the G(k) multiply with its energy/virial sums and the
  *                 inverse z transform (grid sizes whose Z has no prime
  *                 factor above 8); 0 = 3D transforms and a separate k-pass (0/1, 1)
+ *   zpass_cols    spectrum columns per workgroup of the fused z pass, for
+ *                 handles created afterwards (4, 8, 16; 8)
  *   spread_mma    FP32 spreads with int32 bricks and a 13-point support run
  *                 on the matrix cores (int8 limb products summed exactly per
  *                 bin, the brick rounded once); 0 = LDS integer atomics (0/1, 0) */

@@ -495,6 +495,55 @@ def test_binning_variants_give_identical_grids(dt):
         api.set_tuning(bin_agg=keys["bin_agg"])
 
 
+@pytest.mark.parametrize("dt", [F32, F64])
+def test_counting_sort_binning_above_16384_bins(dt):
+    """Grids of more than 16384 bins are binned by the three-pass counting
+    sort (per-workgroup histograms, offsets, placement).  It must give the
+    same results as one slot claim per charge, bit for bit: grids, energy
+    and the per-charge outputs, with a cluster that overfills its bin and
+    two rejected charges (M = 216: 19683 bins of 8 cells).  Which cluster
+    charges take the overflow path (gathered from global memory, summed in
+    another order) differs between the two binnings, so the cluster's
+    outputs agree to rounding only."""
+    rng = np.random.default_rng(31)
+    L, M, n = 215.3, 216, 120000
+    xyzq = np.empty((n, 4))
+    xyzq[:, :3] = rng.random((n, 3)) * L
+    xyzq[:, 3] = np.where(np.arange(n) % 2 == 0, 0.5, -0.5)
+    xyzq[:600, :3] = 50.0 + rng.random((600, 3)) * 0.9   # overfills one bin
+    xyzq[7, 0] = np.nan
+    xyzq[9, 3] = np.inf
+    x = to_dev(xyzq, dt)
+    keys = api.get_tuning()
+    outs = []
+    try:
+        for agg in (4, 0):
+            api.set_tuning(bin_agg=agg)
+            gs = api.CpuGSEk(dt, dt, M, M, M, 2.12, 0.4, 3.0, 3.0, L, L, L)
+            o = gs.step(x, force=True, potential=True)
+            lay = gs.layout()
+            assert lay["nbins"] == 19683
+            outs.append((gs.getRhoS().clone(), o["energy"].clone(), o["force"].clone(),
+                         o["pot"].clone()))
+    finally:
+        api.set_tuning(bin_agg=keys["bin_agg"])
+    # every charge of the cluster's bin (the cluster and the few others
+    # there) may take either path; the rest must agree bit for bit
+    good = np.isfinite(xyzq).all(axis=1)
+    o = np.mod(np.floor(np.nan_to_num(xyzq[:, :3]) / (L / M) + 0.5).astype(np.int64) - 6, M) // 8
+    bins = o[:, 0] + 27 * (o[:, 1] + 27 * o[:, 2])
+    hot = np.isin(bins, bins[10:600])
+    keep_np = good & ~hot
+    keep = torch.from_numpy(keep_np).to(DEV)
+    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+    assert torch.equal(outs[0][2][:, keep], outs[1][2][:, keep])
+    assert torch.equal(outs[0][3][keep], outs[1][3][keep])
+    cl = torch.from_numpy(good & hot).to(DEV)
+    tol = 1e-5 if dt == F32 else 1e-12
+    assert rel(host(outs[0][2][:, cl]), host(outs[1][2][:, cl])) <= tol
+    assert rel(host(outs[0][3][cl]), host(outs[1][3][cl])) <= tol
+
+
 def _tiling_system(seed):
     """A 48^3 grid in a 46 A box: h = 0.958, n = 6, so 8^3 bins with 20^3
     bricks (64-bit bricks fit the LDS budget) and 6 bins per axis: every
