@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(BT)
 // three passes with no global atomics on the bin counters (a workgroup's
 // randomly ordered charges touch most of the bins, so the claims of
 // k_bin_agg would hardly aggregate):
-//   k_bin_hist     per workgroup of kBinSortChunk consecutive charges, the
+//   k_bin_hist     per workgroup of kBinSortChunk (4096) consecutive charges, the
 //                  charge count of every bin (16-bit, two per LDS dword)
 //   k_bin_offsets  per bin, the exclusive sum of those counts over the
 //                  workgroups (the workgroup's first slot in the bin) and
@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(BT)
 // The slots a charge gets differ from k_bin's only in order within a bin;
 // the spread and gather do not depend on that order.
 constexpr int kBinSortThreads = 1024;
-constexpr int kBinSortPPT = 8;
+constexpr int kBinSortPPT = 4;
 constexpr int kBinSortChunk = kBinSortThreads * kBinSortPPT;
 constexpr int kBinSortMaxBins = 32768;  // 16-bit counters in 64 KiB of LDS
 
@@ -304,18 +304,33 @@ __global__ void __launch_bounds__(kBinSortThreads)
   }
 }
 
-__global__ void __launch_bounds__(256)
+// 64 bins per workgroup, the workgroups' rows split into 16 segments: each
+// thread sums one segment of one bin, the segment sums are scanned in LDS,
+// and each thread then writes its segment's offsets (short independent load
+// chains instead of one chain over every workgroup row)
+constexpr int kBinOffBins = 64, kBinOffSegs = 16;
+__global__ void __launch_bounds__(kBinOffBins * kBinOffSegs)
     k_bin_offsets(int nbins, int nwg, const unsigned short* __restrict__ hist,
                   unsigned* __restrict__ offs, int* __restrict__ counts) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ unsigned part[kBinOffSegs][kBinOffBins];
+  const int bl = threadIdx.x % kBinOffBins, seg = threadIdx.x / kBinOffBins;
+  const int j = blockIdx.x * kBinOffBins + bl;
+  const int per = (nwg + kBinOffSegs - 1) / kBinOffSegs;
+  const int w0 = seg * per, w1 = min(nwg, w0 + per);
+  unsigned sum = 0;
+  if (j < nbins)
+    for (int w = w0; w < w1; w++) sum += hist[(size_t)w * nbins + j];
+  part[seg][bl] = sum;
+  __syncthreads();
   if (j >= nbins) return;
   unsigned run = 0;
-  for (int w = 0; w < nwg; w++) {
+  for (int s2 = 0; s2 < seg; s2++) run += part[s2][bl];
+  for (int w = w0; w < w1; w++) {
     const size_t at = (size_t)w * nbins + j;
     offs[at] = run;
     run += hist[at];
   }
-  counts[j] = (int)run;
+  if (seg == kBinOffSegs - 1) counts[j] = (int)run;
 }
 
 template <typename CT>
@@ -2018,7 +2033,8 @@ hipError_t launch_bin(bool dbl, const Geom& g, const BinShape& b, Binning& w, hi
     else
       k_bin_hist<float><<<nwg, kBinSortThreads, 0, s>>>(g, b, w.n, (const V4<float>*)w.xyzq,
                                                          hist, w.stats);
-    k_bin_offsets<<<(b.nbins + 255) / 256, 256, 0, s>>>(b.nbins, nwg, hist, offs, w.counts);
+    k_bin_offsets<<<(b.nbins + kBinOffBins - 1) / kBinOffBins, kBinOffBins * kBinOffSegs, 0, s>>>(
+        b.nbins, nwg, hist, offs, w.counts);
     if (dbl)
       k_bin_place<double><<<nwg, kBinSortThreads, 0, s>>>(
           g, b, w.n, w.cap_bin, (const V4<double>*)w.xyzq, offs, w.stats, w.ovf, w.perm, w.loc,
