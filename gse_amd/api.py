@@ -294,7 +294,8 @@ class CpuFFTSolver:
 
     def form(self):
         """"fused": run() takes 2D transforms around the fused z pass;
-        "3d": 3D transforms and a separate k-pass."""
+        "fused4": the same with the four-step register z pass; "3d": 3D
+        transforms and a separate k-pass."""
         v = ctypes.c_int()
         check(self._L.gse_fft_solver_get_form(self._h, ctypes.byref(v)))
         return _lib.KSPACE_FORMS[v.value]
@@ -478,7 +479,7 @@ def set_tiling(bin_edge=0, lds_budget=0):
 TUNING_KEYS = ("bin_edge", "brick_budget", "spread_block", "gather_block", "spread_stage",
                "gather_stage", "spread_lds32", "spread_cols", "spread_flush",
                "reduce_tile", "reduce_nt", "reduce_tile2", "bin_agg", "fused_kz", "spread_mma",
-               "zpass_cols")
+               "zpass_cols", "gather_group", "zpass4")
 
 
 def set_tuning(**kw):

@@ -482,6 +482,8 @@ struct KSolver {
   double tab_sigma = NAN, tab_bx = NAN, tab_by = NAN, tab_bz = NAN;
 
   size_t nk() const { return (size_t)(X / 2 + 1) * Y * Z; }
+  // 0: 3D transforms + k-pass, 1: fused Stockham z pass, 2: fused four-step
+  int form() const { return fused ? (zp.four ? 2 : 1) : 0; }
 
   gse_status_t init(bool d, int x, int y, int z) {
     dbl = d;
@@ -842,6 +844,8 @@ gse_status_t gse_set_tuning_param(const char* name, int value) {
     t.bin_agg = value;
   else if (n == "fused_kz" && (value == 0 || value == 1)) t.fused_kz = value;
   else if (n == "zpass_cols" && (value == 4 || value == 8 || value == 16)) t.zpass_cols = value;
+  else if (n == "gather_group" && (value == 16 || value == 32)) t.gather_group = value;
+  else if (n == "zpass4" && (value == 0 || value == 1)) t.zpass4 = value;
   else if (n == "spread_mma" && (value == 0 || value == 1)) t.spread_mma = value;
   else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter or value out of range: %s=%d",
                    name, value);
@@ -867,6 +871,8 @@ gse_status_t gse_get_tuning_param(const char* name, int* value) {
   else if (n == "bin_agg") *value = t.bin_agg;
   else if (n == "fused_kz") *value = t.fused_kz;
   else if (n == "zpass_cols") *value = t.zpass_cols;
+  else if (n == "gather_group") *value = t.gather_group;
+  else if (n == "zpass4") *value = t.zpass4;
   else if (n == "spread_mma") *value = t.spread_mma;
   else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter: %s", name);
   return GSE_SUCCESS;
@@ -1286,7 +1292,7 @@ gse_status_t gse_fft_solver_c2r(gse_fft_solver_t h, void* rhoK, void* phi) {
 
 gse_status_t gse_fft_solver_get_form(gse_fft_solver_t h, int* form) {
   if (!h || !form) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
-  *form = h->ks.fused ? 1 : 0;
+  *form = h->ks.form();
   return GSE_SUCCESS;
 }
 
@@ -1447,7 +1453,7 @@ gse_status_t gse_gsek_interpolate_force(gse_gsek_t h, int n, const void* xyzq, v
 
 gse_status_t gse_gsek_get_layout(gse_gsek_t h, int* info) {
   if (!h || !info) return fail(GSE_ERROR_INVALID_VALUE, "NULL argument");
-  h->eng.layout(info, h->ks.fused ? 1 : 0);
+  h->eng.layout(info, h->ks.form());
   return GSE_SUCCESS;
 }
 

@@ -20,8 +20,10 @@
 //   k_ew_cell_count / k_ew_scan / k_ew_scatter / k_ew_sort
 //                   particles into cells of edge >= rcut, each cell's list in
 //                   ascending particle index (deterministic order)
-//   k_ew_force      one workgroup per cell: each thread owns one particle,
-//                   neighbour cells are staged through LDS, the pair term of
+//   k_ew_force      one workgroup per cell: four lanes share one particle
+//                   (each a quarter of its neighbours, summed across the four
+//                   in a fixed order), neighbour cells are staged through
+//                   LDS, the pair term of
 //                   the reference is evaluated with its operand order and
 //                   minimum-image loops; FP64 energy partial per cell
 #include <math.h>
@@ -244,15 +246,13 @@ __global__ __launch_bounds__(256) void k_pme_scalar_sum(PmeSum a, const double* 
 }
 
 // E = ccelec/2 sum, virial_ab = -ccelec/2 sum (CudaPMERecip.cu:1222-1239)
-__global__ void k_pme_finalize(const double* __restrict__ partials, int nblocks, double ccelec,
-                               double* energy, double* virial) {
-  __shared__ double sh[7];
-  if (threadIdx.x < 7) {
-    double s = 0.0;
-    for (int b = 0; b < nblocks; b++) s += partials[b * 7 + threadIdx.x];
-    sh[threadIdx.x] = s;
-  }
-  __syncthreads();
+// the workgroup's threads sum strided partials, then the waves' sums are
+// added in wave order (fixed order, deterministic)
+__global__ void __launch_bounds__(256)
+    k_pme_finalize(const double* __restrict__ partials, int nblocks, double ccelec,
+                   double* energy, double* virial) {
+  double sh[7];
+  reduce_partials_block<7>(partials, nblocks, 7, sh);
   if (threadIdx.x == 0) {
     const double h = 0.5 * ccelec;
     if (energy) energy[0] = h * sh[0];
@@ -409,7 +409,7 @@ hipError_t launch_pme_scalar_sum(bool dbl, int X, int Y, int Z, const double* re
 
 hipError_t launch_pme_finalize(const double* partials, double ccelec, double* energy,
                                double* virial, hipStream_t s) {
-  k_pme_finalize<<<1, 64, 0, s>>>(partials, kPmeSumBlocks, ccelec, energy, virial);
+  k_pme_finalize<<<1, 256, 0, s>>>(partials, kPmeSumBlocks, ccelec, energy, virial);
   return hipGetLastError();
 }
 
@@ -536,23 +536,29 @@ __device__ __forceinline__ double ew_image(double d, double L, double h) {
   return d;
 }
 
-constexpr int kEwBlock = 128;
+constexpr int kEwBlock = 128;  // particles of a cell per pass
+constexpr int kEwSplit = 4;    // threads per particle, each a quarter of the neighbours
+constexpr int kEwThreads = kEwBlock * kEwSplit;
 
-__global__ __launch_bounds__(kEwBlock) void k_ew_force(
+// One workgroup per cell; kEwSplit consecutive lanes share one particle of
+// the cell and take every kEwSplit-th neighbour of each staged neighbour
+// cell, and their sums are added across the four lanes in a fixed order.
+__global__ __launch_bounds__(kEwThreads) void k_ew_force(
     EwCells c, EwPair pr, const int* __restrict__ starts, const int* __restrict__ counts,
     const int* __restrict__ sidx, const P4<double>* __restrict__ sx,
     const int* __restrict__ xofs, const int* __restrict__ xlist, double* __restrict__ gx,
     double* __restrict__ gy, double* __restrict__ gz, double* __restrict__ partials) {
   __shared__ P4<double> shp[kEwBlock];
   __shared__ int shi[kEwBlock];
-  __shared__ double she[kEwBlock / 64];
+  __shared__ double she[kEwThreads / 64];
   const int cell = blockIdx.x;
   const int cx = cell % c.nc[0], cy = (cell / c.nc[0]) % c.nc[1], cz = cell / (c.nc[0] * c.nc[1]);
   const int st = starts[cell], cnt = counts[cell];
   const int lo0 = c.nc[0] >= 3 ? -1 : 0, lo1 = c.nc[1] >= 3 ? -1 : 0, lo2 = c.nc[2] >= 3 ? -1 : 0;
+  const int slot = threadIdx.x / kEwSplit, sp = threadIdx.x % kEwSplit;
   double esum = 0.0;
   for (int base = 0; base < cnt; base += kEwBlock) {
-    const int t = base + threadIdx.x;
+    const int t = base + slot;
     const bool own = t < cnt;
     const int a = own ? sidx[st + t] : -1;
     P4<double> pa{0.0, 0.0, 0.0, 0.0};
@@ -574,14 +580,14 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_force(
           const int nst = starts[nb], ncnt = counts[nb];
           for (int kb = 0; kb < ncnt; kb += kEwBlock) {
             __syncthreads();
-            if (kb + (int)threadIdx.x < ncnt) {
+            if ((int)threadIdx.x < kEwBlock && kb + (int)threadIdx.x < ncnt) {
               shp[threadIdx.x] = sx[nst + kb + threadIdx.x];
               shi[threadIdx.x] = sidx[nst + kb + threadIdx.x];
             }
             __syncthreads();
             const int m = min(kEwBlock, ncnt - kb);
             if (!own) continue;
-            for (int k = 0; k < m; k++) {
+            for (int k = sp; k < m; k += kEwSplit) {
               const int b = shi[k];
               if (b == a) continue;
               const P4<double> pb = shp[k];
@@ -606,7 +612,15 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_force(
             }
           }
         }
-    if (own) {
+    // the particle's four partial sums, added in a fixed order (xor 1, then
+    // xor 2: every lane of the four ends with the same value)
+#pragma unroll
+    for (int m = 1; m < kEwSplit; m <<= 1) {
+      g0 += __shfl_xor(g0, m, kWave);
+      g1 += __shfl_xor(g1, m, kWave);
+      g2 += __shfl_xor(g2, m, kWave);
+    }
+    if (own && sp == 0) {
       gx[a] = g0;
       gy[a] = g1;
       gz[a] = g2;
@@ -617,7 +631,11 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_force(
   const double w = wave_sum(esum);
   if ((threadIdx.x & 63) == 0) she[threadIdx.x >> 6] = w;
   __syncthreads();
-  if (threadIdx.x == 0) partials[cell] = 0.5 * (she[0] + she[1]);
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+    for (int k = 0; k < kEwThreads / 64; k++) tot += she[k];
+    partials[cell] = 0.5 * tot;
+  }
 }
 
 // energy = sum of the cell partials (cell order) + the self term
@@ -657,8 +675,8 @@ hipError_t launch_ew_force(const EwCells& c, const EwPair& p, const int* starts,
                            const int* counts, const int* sidx, const void* sx, const int* xofs,
                            const int* xlist, double* gx, double* gy, double* gz,
                            double* partials, hipStream_t s) {
-  k_ew_force<<<c.ncell, kEwBlock, 0, s>>>(c, p, starts, counts, sidx, (const P4<double>*)sx,
-                                          xofs, xlist, gx, gy, gz, partials);
+  k_ew_force<<<c.ncell, kEwThreads, 0, s>>>(c, p, starts, counts, sidx, (const P4<double>*)sx,
+                                            xofs, xlist, gx, gy, gz, partials);
   return hipGetLastError();
 }
 

@@ -161,7 +161,7 @@ __host__ __device__ inline int brick_pitch(int ct_bytes, int X, int Bx, int Lx) 
 // of the bin), the system's potential grid at `phi`; perm holds the output
 // index of each particle.  Ends with a barrier, so a workgroup may call it
 // for several bins in a row.
-template <typename CT, typename AT, int NZC, int FL, int BS, int PL = 0>
+template <typename CT, typename AT, int NZC, int FL, int BS, int PL = 0, int GS = 32>
 __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt,
                                            const V4<CT>* __restrict__ sorted,
                                            const int* __restrict__ perm,
@@ -327,11 +327,14 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
   // fit beside two charges' registers), with the reference's exact test
   constexpr bool PAIR = FMA_FORM && PL > 0 && (sizeof(CT) == 4 || !VIR);
   constexpr bool EXACT_SPH = VIR || sizeof(CT) == 8;
-  constexpr int NXYP = NZA * NZA, KP = (NXYP + 31) / 32;
+  // GS lanes per charge (32: two charges per wave; 16: four)
+  static_assert(GS == 32 || GS == 16, "charge groups of 32 or 16 lanes");
+  constexpr int GPW = 64 / GS;
+  constexpr int NXYP = NZA * NZA, KP = (NXYP + GS - 1) / GS;
   int p_tx[KP], p_ty[KP];
 #pragma unroll
   for (int k = 0; k < KP; k++) {
-    const int idx = (lane & 31) + 32 * k;
+    const int idx = (lane & (GS - 1)) + GS * k;
     p_ty[k] = idx < NXYP ? idx / NZA : 0;
     p_tx[k] = idx < NXYP ? idx - p_ty[k] * NZA : 0;
   }
@@ -371,14 +374,14 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
     }
     __syncthreads();
     if constexpr (PAIR) {
-      // phase 2, two charges per wave: half h of wave `wid` takes charge
-      // 2*g2 + h; a half without a charge walks the other half's charge
-      // with its sums discarded (no divergence inside the walk)
-      const int h = lane >> 5, l32 = lane & 31;
-      for (int g2 = wid; 2 * g2 < na; g2 += nwv) {
-        const int ai = 2 * g2 + h;
+      // phase 2, GPW charges per wave: lane group h of wave `wid` takes
+      // charge GPW*g2 + h; a group without a charge walks the first group's
+      // charge with its sums discarded (no divergence inside the walk)
+      const int h = lane / GS, l32 = lane & (GS - 1);
+      for (int g2 = wid; GPW * g2 < na; g2 += nwv) {
+        const int ai = GPW * g2 + h;
         const bool live = ai < na;
-        const int aic = live ? ai : 2 * g2;
+        const int aic = live ? ai : GPW * g2;
         const CT* w = sW + (size_t)aic * NW * 2;
         const int base = sB[aic];
         // dz2: rc^2 - dz^2 (no virial: one compare per point), else dz^2
@@ -397,7 +400,7 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
         for (int c = 0; c < 13; c++) f[c] = 0;
 #pragma unroll
         for (int k = 0; k < KP; k++) {
-          if (l32 + 32 * k >= NXYP) continue;
+          if (l32 + GS * k >= NXYP) continue;
           const CT wx = w[2 * p_tx[k]];
           const CT dx = w[2 * p_tx[k] + 1];
           const CT wy = w[2 * (NZA + p_ty[k])];
@@ -449,7 +452,7 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
         for (int c = 0; c < 13; c++) {
           const bool used = (c < 3 && SPH) || (c >= 3 && c < 6 && CUB) || (c == 6 && POT) ||
                             (c >= 7 && VIR);
-          if (used) f[c] = half_total(live ? f[c] : (AT)0);
+          if (used) f[c] = GS == 32 ? half_total(live ? f[c] : (AT)0) : row_total(live ? f[c] : (AT)0);
         }
         if (live && l32 == 0) gather_store<AT, FL>(a, o, sP[ai], sQ[ai], ds, f);
       }
@@ -574,7 +577,7 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
 }
 
 
-template <typename CT, typename AT, int NZC, int FL, int BS, int PL = 0>
+template <typename CT, typename AT, int NZC, int FL, int BS, int PL = 0, int GS = 32>
 // FP32 with supports up to 13: at most 128 VGPRs, so two 512-thread
 // workgroups share a CU
 __global__ void __launch_bounds__(
@@ -611,7 +614,7 @@ __global__ void __launch_bounds__(
   if (threadIdx.x == 0) counts_other[bin] = 0;
   if (cnt == 0) return;
   const size_t s0 = (size_t)bin * a.cap_bin;
-  gather_bin<CT, AT, NZC, FL, BS, PL>(a, bin, cnt, sorted + s0, perm + s0, loc + s0, phi, o, ds,
+  gather_bin<CT, AT, NZC, FL, BS, PL, GS>(a, bin, cnt, sorted + s0, perm + s0, loc + s0, phi, o, ds,
                                   smem_raw);
 }
 
@@ -843,7 +846,10 @@ static hipError_t gather_nz(GatherArgs a, const Binning& w, const void* phi, con
       // not the full output set, whose sums do not fit the register budget)
       if constexpr (sizeof(CT) == 4 && FL != (GF_SPHERE | GF_CUBE | GF_POT | GF_VIRIAL)) {
         if (bs == 512 && pxly == 400 && a.g.nx == a.g.nz && a.g.ny == a.g.nz) {
-          k_gather<CT, AT, 13, FL, 512, 400><<<nwg, 512, lds, s>>>(GSE_G_ARGS);
+          if (tu.gather_group == 16)
+            k_gather<CT, AT, 13, FL, 512, 400, 16><<<nwg, 512, lds, s>>>(GSE_G_ARGS);
+          else
+            k_gather<CT, AT, 13, FL, 512, 400><<<nwg, 512, lds, s>>>(GSE_G_ARGS);
           break;
         }
       }

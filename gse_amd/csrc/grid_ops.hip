@@ -424,7 +424,225 @@ __global__ void __launch_bounds__(kZpassThreads)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Four-step z pass for the column lengths it is built for (Z = Z1 Z2 known at
+// compile time).  With z = Z2 n1 + n2 and k = k1 + Z1 k2,
+//   X[k1 + Z1 k2] = sum_n2 w_Z2^(n2 k2) w_Z^(n2 k1) sum_n1 x[Z2 n1 + n2] w_Z1^(n1 k1):
+// thread n2 of a column holds the Z1 values of its stride-Z2 sub-column and
+// transforms them in registers, multiplies by w_Z^(n2 k1) and writes them to
+// LDS; after one barrier thread k1 reads the Z2 values of row k1 and
+// transforms them in registers.  The G(k) multiply and the FP64 energy /
+// virial sums follow in registers, then the inverse runs the two steps back
+// with conjugate twiddles.  Two LDS exchanges per direction instead of one
+// per Stockham stage.  In-register DFTs split L = P Q by decimation in time,
+//   X[q + Q s] = sum_p w_P^(p s) w_L^(p q) DFT_Q(x[p + P m])[q],
+// down to the radix-2/3/4/8 butterflies above and direct sums for 5 and 7;
+// every twiddle is w_Z^j from the table.
+// ---------------------------------------------------------------------------
+template <int L>
+constexpr int reg_dft_radix() {
+  return (L % 4 == 0 && L != 4 && L != 8) ? 4
+         : (L % 2 == 0 && L != 2 && L != 4 && L != 8) ? 2
+         : (L % 3 == 0 && L != 3) ? 3
+         : (L % 5 == 0 && L != 5) ? 5
+         : (L % 7 == 0 && L != 7) ? 7
+                                  : L;
+}
+
+template <typename T>
+__device__ __forceinline__ C2<T> zc_mul(C2<T> a, C2<T> w, bool inv) {
+  if (inv) w.y = -w.y;
+  return C2<T>{a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x};
+}
+
+// x[0..L) in place; ZW: the table length (W[j] = exp(-2 pi i j / ZW)), L | ZW
+template <typename T, int L, int ZW>
+__device__ __forceinline__ void reg_dft(C2<T>* x, const C2<T>* __restrict__ W, bool inv) {
+  if constexpr (L == 1) {
+    return;
+  } else if constexpr (L == 2 || L == 3 || L == 4 || L == 8) {
+    C2<T> y[L];
+#pragma unroll
+    for (int i = 0; i < L; i++) y[i] = x[i];
+    zdft_small<T, L>(y, inv);
+#pragma unroll
+    for (int i = 0; i < L; i++) x[i] = y[i];
+  } else if constexpr (reg_dft_radix<L>() == L) {  // 5, 7: direct sums
+    C2<T> y[L];
+#pragma unroll
+    for (int s2 = 0; s2 < L; s2++) {
+      C2<T> acc = x[0];
+#pragma unroll
+      for (int q = 1; q < L; q++) {
+        const C2<T> t = zc_mul(x[q], W[((q * s2) % L) * (ZW / L)], inv);
+        acc = zc_add(acc, t);
+      }
+      y[s2] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < L; i++) x[i] = y[i];
+  } else {
+    constexpr int P = reg_dft_radix<L>(), Q = L / P;
+    C2<T> y[P][Q];
+#pragma unroll
+    for (int p2 = 0; p2 < P; p2++) {
+#pragma unroll
+      for (int m = 0; m < Q; m++) y[p2][m] = x[p2 + P * m];
+      reg_dft<T, Q, ZW>(y[p2], W, inv);
+#pragma unroll
+      for (int q = 1; q < Q; q++)
+        if (p2 > 0) y[p2][q] = zc_mul(y[p2][q], W[p2 * q * (ZW / L)], inv);
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      C2<T> z[P];
+#pragma unroll
+      for (int p2 = 0; p2 < P; p2++) z[p2] = y[p2][q];
+      reg_dft<T, P, ZW>(z, W, inv);
+#pragma unroll
+      for (int s2 = 0; s2 < P; s2++) x[q + Q * s2] = z[s2];
+    }
+  }
+}
+
+template <int Z1, int Z2, int C>
+constexpr int zpass4_threads() {
+  return ((C * (Z1 > Z2 ? Z1 : Z2)) + 63) / 64 * 64;
+}
+
+template <typename T, int Z1, int Z2, int C, bool VIR>
+__global__ void __launch_bounds__((zpass4_threads<Z1, Z2, C>()))
+    k_zpass4(KArgs a, int kind, C2<T>* __restrict__ spec, const T* __restrict__ table,
+             const C2<T>* __restrict__ tw, double* __restrict__ partials) {
+  constexpr int Z = Z1 * Z2;
+  constexpr int NT = zpass4_threads<Z1, Z2, C>();
+  extern __shared__ __align__(16) char zsm[];
+  C2<T>* buf = (C2<T>*)zsm;  // [Z][C]: entry (k1 Z2 + n2) of every column
+  C2<T>* W = buf + (size_t)Z * C;
+  const int ncol = a.Xh * a.Y;
+  const size_t zs = (size_t)ncol;
+  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  const int col = blockIdx.x * C + c;
+  const bool okc = col < ncol;
+  // every load first: the thread's sub-column (step one) and its G(k) row
+  // (step two), all in flight at once
+  C2<T> u[Z1];
+  T g[Z2];
+#pragma unroll
+  for (int n1 = 0; n1 < Z1; n1++)
+    u[n1] = (t < Z2 && okc) ? spec[(size_t)(Z2 * n1 + t) * zs + col] : C2<T>{(T)0, (T)0};
+#pragma unroll
+  for (int k2 = 0; k2 < Z2; k2++)
+    g[k2] = (t < Z1 && okc) ? table[(size_t)(t + Z1 * k2) * zs + col] : (T)0;
+  for (int i = threadIdx.x; i < Z; i += NT) W[i] = tw[i];
+  __syncthreads();
+  if (t < Z2) {  // step one, thread n2 = t
+    reg_dft<T, Z1, Z>(u, W, false);
+#pragma unroll
+    for (int k1 = 0; k1 < Z1; k1++) {
+      const C2<T> v = k1 > 0 ? zc_mul(u[k1], W[t * k1], false) : u[k1];
+      buf[(k1 * Z2 + t) * C + c] = v;
+    }
+  }
+  __syncthreads();
+  double e = 0.0, vxx = 0.0, vxy = 0.0, vxz = 0.0, vyy = 0.0, vyz = 0.0, vzz = 0.0;
+  C2<T> v[Z2];
+  if (t < Z1) {  // step two, thread k1 = t: v[k2] = X[t + Z1 k2]
+#pragma unroll
+    for (int n2 = 0; n2 < Z2; n2++) v[n2] = buf[(t * Z2 + n2) * C + c];
+    reg_dft<T, Z2, Z>(v, W, false);
+    const int ix = col % a.Xh;
+    const double mult = (ix == 0 || 2 * ix == a.X) ? 1.0 : 2.0;
+#pragma unroll
+    for (int k2 = 0; k2 < Z2; k2++) {
+      const double re = (double)v[k2].x, im = (double)v[k2].y;
+      v[k2] = C2<T>{v[k2].x * g[k2], v[k2].y * g[k2]};
+      if (!okc) continue;
+      const double ek = mult * (double)g[k2] * (re * re + im * im);
+      e += ek;
+      if constexpr (VIR) {
+        int ixx;
+        double kx, ky, kz;
+        kvec(a, (size_t)(t + Z1 * k2) * zs + col, ixx, kx, ky, kz);
+        const double k2n = kx * kx + ky * ky + kz * kz;
+        if (k2n > 0.0) {
+          const double cf = ek * 2.0 * (a.sigmafac + (kind == 0 ? 1.0 / k2n : 0.0));
+          vxx += cf * kx * kx;
+          vxy += cf * kx * ky;
+          vxz += cf * kx * kz;
+          vyy += cf * ky * ky;
+          vyz += cf * ky * kz;
+          vzz += cf * kz * kz;
+        }
+      }
+    }
+    reg_dft<T, Z2, Z>(v, W, true);  // v[n2] = Y'[t][n2]
+#pragma unroll
+    for (int n2 = 1; n2 < Z2; n2++)
+      if (t > 0) v[n2] = zc_mul(v[n2], W[n2 * t], true);
+  }
+  __syncthreads();  // every read of the forward rows is done
+  if (t < Z1) {
+#pragma unroll
+    for (int n2 = 0; n2 < Z2; n2++) buf[(t * Z2 + n2) * C + c] = v[n2];
+  }
+  __syncthreads();
+  if (t < Z2) {  // inverse step one, thread n2 = t
+#pragma unroll
+    for (int k1 = 0; k1 < Z1; k1++) u[k1] = buf[(k1 * Z2 + t) * C + c];
+    reg_dft<T, Z1, Z>(u, W, true);
+    if (okc) {
+#pragma unroll
+      for (int n1 = 0; n1 < Z1; n1++) spec[(size_t)(Z2 * n1 + t) * zs + col] = u[n1];
+    }
+  }
+  if constexpr (VIR) {
+    double acc[7] = {e, vxx, vxy, vxz, vyy, vyz, vzz};
+    block_sum_store<7>(acc, partials);
+  } else {
+    double acc[1] = {e};
+    block_sum_store<1>(acc, partials);
+  }
+}
+
+// the four-step forms built: Z -> (Z1, Z2, C)
+struct Zpass4Shape {
+  int Z, Z1, Z2, C;
+};
+constexpr Zpass4Shape kZpass4Shapes[] = {{80, 8, 10, 16}, {216, 12, 18, 16}};
+
+static bool zpass4_shape(int Z, Zpass4Shape* out) {
+  for (const Zpass4Shape& sh : kZpass4Shapes)
+    if (sh.Z == Z) {
+      *out = sh;
+      return true;
+    }
+  return false;
+}
+
+template <typename T, int Z1, int Z2, int C>
+static void zpass4_t(const KArgs& a, int kind, void* spec, const void* table, const void* tw,
+                     bool vir, double* partials, unsigned g, hipStream_t s) {
+  constexpr int NT = zpass4_threads<Z1, Z2, C>();
+  const size_t lds = ((size_t)Z1 * Z2 * C + (size_t)Z1 * Z2) * sizeof(C2<T>);
+  if (vir)
+    k_zpass4<T, Z1, Z2, C, true><<<g, NT, lds, s>>>(a, kind, (C2<T>*)spec, (const T*)table,
+                                                    (const C2<T>*)tw, partials);
+  else
+    k_zpass4<T, Z1, Z2, C, false><<<g, NT, lds, s>>>(a, kind, (C2<T>*)spec, (const T*)table,
+                                                     (const C2<T>*)tw, partials);
+}
+
 bool zplan_make(int Z, bool dbl, ZPlan* pl) {
+  pl->four = 0;
+  Zpass4Shape sh;
+  if (tuning().zpass4 && zpass4_shape(Z, &sh)) {
+    pl->Z = Z;
+    pl->nst = 0;
+    pl->C = sh.C;
+    pl->four = 1;
+    return true;
+  }
   // greedy, largest radix first (8 x 3 x 3 x 3 at Z = 216 measured faster
   // than 6 x 6 x 6)
   pl->Z = Z;
@@ -456,6 +674,7 @@ bool zplan_make(int Z, bool dbl, ZPlan* pl) {
 
 size_t zpass_lds_bytes(const ZPlan& pl, bool dbl) {
   const size_t cplx = dbl ? 16 : 8;
+  if (pl.four) return ((size_t)pl.Z * pl.C + pl.Z) * cplx;
   return (2 * (size_t)pl.Z * pl.C + pl.Z) * cplx;
 }
 
@@ -482,6 +701,18 @@ hipError_t launch_zpass(bool dbl, int kind, int X, int Y, int Z, double bx, doub
   const KArgs a = make_kargs(X, Y, Z, bx, by, bz, sigma);
   const unsigned g = (unsigned)zpass_blocks(X, Y, pl);
   const size_t lds = zpass_lds_bytes(pl, dbl);
+  if (pl.four) {
+    if (Z == 80 && pl.C == 16) {
+      if (dbl) zpass4_t<double, 8, 10, 16>(a, kind, spec, table, tw, vir, partials, g, s);
+      else zpass4_t<float, 8, 10, 16>(a, kind, spec, table, tw, vir, partials, g, s);
+    } else if (Z == 216 && pl.C == 16) {
+      if (dbl) zpass4_t<double, 12, 18, 16>(a, kind, spec, table, tw, vir, partials, g, s);
+      else zpass4_t<float, 12, 18, 16>(a, kind, spec, table, tw, vir, partials, g, s);
+    } else {
+      return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
   if (pl.C != 16 && pl.C != 8 && pl.C != 4) return hipErrorInvalidValue;
   if (dbl) {
     if (pl.C == 16) zpass_t<double, 16>(a, pl, kind, spec, table, tw, vir, partials, g, lds, s);

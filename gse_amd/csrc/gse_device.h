@@ -80,6 +80,17 @@ __device__ __forceinline__ T wave_total(T v) {
   return (r0 + r1) + (r2 + r3);
 }
 
+// Sum over each 16-lane row of the wave, in every lane of the row (fixed
+// order, deterministic).
+template <typename T>
+__device__ __forceinline__ T row_total(T v) {
+  v += dpp_move<0xB1>(v);
+  v += dpp_move<0x4E>(v);
+  v += dpp_move<0x141>(v);
+  v += dpp_move<0x140>(v);
+  return v;
+}
+
 // Sum over each 32-lane half of the wave: lanes 0-31 get the sum of lanes
 // 0-31, lanes 32-63 the sum of lanes 32-63 (fixed order, deterministic).
 template <typename T>

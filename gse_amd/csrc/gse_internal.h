@@ -54,6 +54,8 @@ struct Tuning {
   int bin_agg = 4;             // particles per workgroup / 256 of the aggregated binning (0: off)
   int fused_kz = 1;            // FFT solve as 2D x-y transforms around one fused z pass
   int zpass_cols = 8;          // spectrum columns per z-pass workgroup (4, 8, 16)
+  int zpass4 = 1;              // four-step z pass for the column lengths it is built for
+  int gather_group = 32;       // lanes per charge in the FP32 13^3 gather walk (32, 16)
   int spread_mma = 0;          // FP32 spread on the matrix cores (int8 limbs, exact sums)
 };
 Tuning& tuning();
@@ -339,6 +341,7 @@ struct ZPlan {
   int Z, nst;
   int p[kZMaxStages];  // radices 2..8, product Z
   int C;               // spectrum columns per workgroup
+  int four;            // 1: the four-step kernel built for this Z (k_zpass4)
 };
 // false when Z has a prime factor above 8 or the columns do not fit LDS
 bool zplan_make(int Z, bool dbl, ZPlan* pl);

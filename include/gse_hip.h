@@ -89,8 +89,14 @@ GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
  *                 the G(k) multiply with its energy/virial sums and the
  *                 inverse z transform (grid sizes whose Z has no prime
  *                 factor above 8); 0 = 3D transforms and a separate k-pass (0/1, 1)
+ *   gather_group  lanes per charge in the FP32 gather of 13-point supports:
+ *                 32 (two charges per wave) or 16 (four) (32)
  *   zpass_cols    spectrum columns per workgroup of the fused z pass, for
  *                 handles created afterwards (4, 8, 16; 8)
+ *   zpass4        the fused z pass runs as a four-step transform in
+ *                 registers (two LDS exchanges per direction) for the column
+ *                 lengths it is built for (Z = 80, 216), for handles created
+ *                 afterwards; 0 = the Stockham form for every Z (0/1, 1)
  *   spread_mma    FP32 spreads with int32 bricks and a 13-point support run
  *                 on the matrix cores (int8 limb products summed exactly per
  *                 bin, the brick rounded once); 0 = LDS integer atomics (0/1, 0) */
@@ -280,9 +286,10 @@ GSE_API gse_status_t gse_fft_solver_r2c(gse_fft_solver_t h, const void* rho, voi
 GSE_API gse_status_t gse_fft_solver_c2r(gse_fft_solver_t h, void* rhoK, void* phi);
 /* Build-defined diagnostic (no reference counterpart): *form = 1 when run()
  * takes 2D x-y transforms around the fused z pass (z transform, G(k)
- * multiply and energy/virial sums, inverse z transform in one kernel), 0
- * when it takes 3D transforms and a separate k-pass.  The form is fixed at
- * create time by the fused_kz tuning knob and the factors of sizeZ. */
+ * multiply and energy/virial sums, inverse z transform in one kernel), 2
+ * when that z pass is the four-step register form, 0 when run() takes 3D
+ * transforms and a separate k-pass.  The form is fixed at create time by the
+ * fused_kz and zpass4 tuning knobs and the factors of sizeZ. */
 GSE_API gse_status_t gse_fft_solver_get_form(gse_fft_solver_t h, int* form);
 
 /* CpuGreensFunc.h:16-18  mul(nfftX, nfftY, nfftZ, box, sigma, rhoK) */

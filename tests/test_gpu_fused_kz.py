@@ -40,13 +40,13 @@ def host(t):
 
 
 @contextlib.contextmanager
-def fused(on):
-    old = api.get_tuning()["fused_kz"]
-    api.set_tuning(fused_kz=int(on))
+def fused(on, four=0):
+    old = api.get_tuning()
+    api.set_tuning(fused_kz=int(on), zpass4=int(four))
     try:
         yield
     finally:
-        api.set_tuning(fused_kz=old)
+        api.set_tuning(fused_kz=old["fused_kz"], zpass4=old["zpass4"])
 
 
 # Z factorisations: radix 8 only, 8*8*2, 6*4, 7*5, 3*3*3*... (27), 2*2 with
@@ -147,3 +147,78 @@ def test_fused_gsek_mode_r_fp64():
     assert abs(E - r.energy) <= TOL * abs(r.energy)
     assert rel(host(o["force"]).T, r.force) <= TOL
     assert rel(host(o["pot"]), r.pot) <= TOL
+
+
+# The four-step z pass (registers, two LDS exchanges per direction) is built
+# for Z = 80 (8 x 10) and Z = 216 (12 x 18); ragged column counts (Xh Y not a
+# multiple of the 16 columns per workgroup) included.
+FOUR_DIMS = [
+    ((16, 18, 80), (16.0, 18.5, 80.0)),
+    ((11, 7, 80), (11.0, 7.5, 80.0)),
+    ((12, 10, 216), (12.0, 10.5, 216.0)),
+]
+
+
+@pytest.mark.parametrize("kind,sigma", [(0, 0.9487), (1, 0.9487), (2, 0.0)])
+@pytest.mark.parametrize("dims,box", FOUR_DIMS, ids=["x".join(map(str, d[0])) for d in FOUR_DIMS])
+@pytest.mark.parametrize("dt", [F64, F32], ids=["f64", "f32"])
+def test_four_step_solver_vs_oracle_and_stockham(kind, sigma, dims, box, dt):
+    X, Y, Z = dims
+    rho = np.random.default_rng(X * 1000 + Z + 1).normal(size=(Z, Y, X))
+    gf = [api.CpuGreensFuncGSE, api.CpuGreensFuncG2, api.CpuGreensFuncHermitian][kind]()
+    if kind == 2:
+        rho -= rho.mean()
+        ref = orc.mehrstellen_solve(dims, box, rho)
+    else:
+        ref = orc.fft_solve(kind, dims, sigma, box, rho)
+    tol = TOL if dt == F64 else 2e-6
+    out = {}
+    for four in (1, 0):
+        with fused(True, four):
+            s = api.CpuFFTSolver(gf, dt, *dims)
+        assert s.form() == ("fused4" if four else "fused")
+        phi = torch.zeros((Z, Y, X), dtype=dt, device=DEV)
+        s.run(phi, to_dev(rho, dt), sigma, *box)
+        out[four] = host(phi)
+        if kind == 2:
+            out[four] -= out[four].mean()
+        assert rel(out[four], ref) <= tol, (four, rel(out[four], ref))
+    assert rel(out[1], out[0]) <= (1e-12 if dt == F64 else 2e-6)
+
+
+def test_four_step_gsek_step_fp64_vs_oracle():
+    xyzq = systems.water_box(400, 24.0, seed=31)
+    dims, box = (24, 24, 80), (24.0, 24.0, 72.0)
+    p = orc.Params(dims, box, 2.12)
+    for mode in (0, 1):
+        r = orc.gse_step(p, xyzq, mode=mode)
+        with fused(True, 1):
+            gs = api.CpuGSEk(F64, F64, *dims, p.sigma, 0.4, 3.0, 3.0, *box)
+        assert gs.layout()["kspace"] == "fused4"
+        o = gs.step(to_dev(xyzq), mode=mode, force=True, potential=True, virial=mode == 0)
+        E = float(host(o["energy"])[0])
+        assert abs(E - r.energy) <= TOL * abs(r.energy), (mode, E, r.energy)
+        assert rel(host(o["force"]).T, r.force) <= TOL
+        assert rel(host(o["pot"]), r.pot) <= TOL
+        if mode == 0:
+            assert rel(host(o["virial"]).reshape(3, 3), r.virial) <= TOL
+
+
+def test_four_step_gsek_fp32_water_48k_M80():
+    """The FP32 step at 48k charges on the 80^3 grid (the default four-step
+    z pass): force Erms (cpu_gse.cpp:131-146) against the FP64 oracle
+    <= 1e-5, energy and virial against the Stockham z pass to 1e-5."""
+    xyzq, L, M, sigma = systems.make("water_48k_M80")
+    p = orc.Params(M, L, sigma)
+    ref = orc.gse_step(p, xyzq.astype(np.float32).astype(np.float64))
+    out = {}
+    for four in (1, 0):
+        with fused(True, four):
+            gs = api.CpuGSEk(F32, F32, M, M, M, sigma, 0.4, 3.0, 3.0, L, L, L)
+        assert gs.layout()["kspace"] == ("fused4" if four else "fused")
+        o = gs.step(to_dev(xyzq, F32), force=True, potential=True, virial=True)
+        out[four] = (float(host(o["energy"])[0]), host(o["force"]), host(o["virial"]))
+        erms = orc.erms(ref.force, out[four][1].T)
+        assert erms <= 1e-5, (four, erms)
+    assert abs(out[1][0] - out[0][0]) <= 1e-5 * abs(out[0][0])
+    assert rel(out[1][2], out[0][2]) <= 1e-5
