@@ -293,9 +293,10 @@ class CpuFFTSolver:
         check(self._L.gse_fft_solver_c2r(self._h, _ptr(rhoK), _ptr(phi)))
 
     def form(self):
-        """"fused": run() takes 2D transforms around the fused z pass;
-        "fused4": the same with the four-step register z pass; "3d": 3D
-        transforms and a separate k-pass."""
+        """"fused": run() takes 2D rocFFT transforms around the fused z
+        pass; "fused4": the same with the four-step register z pass;
+        "fused/xy", "fused4/xy": those z passes between the plane kernels'
+        x-y transforms; "3d": 3D transforms and a separate k-pass."""
         v = ctypes.c_int()
         check(self._L.gse_fft_solver_get_form(self._h, ctypes.byref(v)))
         return _lib.KSPACE_FORMS[v.value]
@@ -479,7 +480,8 @@ def set_tiling(bin_edge=0, lds_budget=0):
 TUNING_KEYS = ("bin_edge", "brick_budget", "spread_block", "gather_block", "spread_stage",
                "gather_stage", "spread_lds32", "spread_cols", "spread_flush",
                "reduce_tile", "reduce_nt", "reduce_tile2", "bin_agg", "fused_kz", "spread_mma",
-               "zpass_cols", "gather_group", "zpass4")
+               "zpass_cols", "gather_group", "zpass4", "xy_plane",
+               "zpass4_cols")
 
 
 def set_tuning(**kw):

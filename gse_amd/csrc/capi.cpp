@@ -476,14 +476,35 @@ struct KSolver {
   FFT3 fft2;
   ZPlan zp{};
   DevBuf tw;
+  // x-y transforms of the fused form by the plane kernels (else rocFFT 2D)
+  bool plane = false;
+  DevBuf twx, twy;
   int zblocks = 0;
   DevBuf spec, table, part;
   int tab_kind = -1;
   double tab_sigma = NAN, tab_bx = NAN, tab_by = NAN, tab_bz = NAN;
 
   size_t nk() const { return (size_t)(X / 2 + 1) * Y * Z; }
-  // 0: 3D transforms + k-pass, 1: fused Stockham z pass, 2: fused four-step
-  int form() const { return fused ? (zp.four ? 2 : 1) : 0; }
+  // 0: 3D transforms + k-pass, 1: fused Stockham z pass, 2: fused four-step;
+  // +2 when the plane kernels do the x-y transforms
+  int form() const { return fused ? (zp.four ? 2 : 1) + (plane ? 2 : 0) : 0; }
+
+  // table exp(-2 pi i m/n), m < n, in FP64 rounded to the compute type
+  gse_status_t upload_twiddles(DevBuf& b, int n) {
+    std::vector<double> w(2 * (size_t)n);
+    for (int m = 0; m < n; m++) {
+      w[2 * m] = std::cos(2.0 * M_PI * m / n);
+      w[2 * m + 1] = -std::sin(2.0 * M_PI * m / n);
+    }
+    GSECHK(b.ensure(2 * tsize(dbl) * (size_t)n));
+    if (dbl) {
+      HIPCHK(hipMemcpy(b.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> wf(w.begin(), w.end());
+      HIPCHK(hipMemcpy(b.p, wf.data(), sizeof(float) * wf.size(), hipMemcpyHostToDevice));
+    }
+    return GSE_SUCCESS;
+  }
 
   gse_status_t init(bool d, int x, int y, int z) {
     dbl = d;
@@ -496,22 +517,16 @@ struct KSolver {
     size_t nparts = kKpassBlocks;
     fused = g_tuning.fused_kz && zplan_make(z, d, &zp);
     if (fused) {
-      GSECHK(fft2.init(d, x, y, z, 1, 2));
+      plane = g_tuning.xy_plane && xy_plane_ok(x, y, d);
+      if (plane) {
+        GSECHK(upload_twiddles(twx, x));
+        GSECHK(upload_twiddles(twy, y));
+      } else {
+        GSECHK(fft2.init(d, x, y, z, 1, 2));
+      }
       zblocks = zpass_blocks(x, y, zp);
       nparts = std::max<size_t>(nparts, (size_t)zblocks);
-      // twiddles exp(-2 pi i m/Z), m < Z, in FP64 rounded to the compute type
-      std::vector<double> w(2 * (size_t)z);
-      for (int m = 0; m < z; m++) {
-        w[2 * m] = std::cos(2.0 * M_PI * m / z);
-        w[2 * m + 1] = -std::sin(2.0 * M_PI * m / z);
-      }
-      GSECHK(tw.ensure(2 * tsize(d) * (size_t)z));
-      if (d) {
-        HIPCHK(hipMemcpy(tw.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
-      } else {
-        std::vector<float> wf(w.begin(), w.end());
-        HIPCHK(hipMemcpy(tw.p, wf.data(), sizeof(float) * wf.size(), hipMemcpyHostToDevice));
-      }
+      GSECHK(upload_twiddles(tw, z));
     }
     GSECHK(part.ensure(sizeof(double) * nparts * 7));
     return GSE_SUCCESS;
@@ -520,6 +535,8 @@ struct KSolver {
     fft.release();
     fft2.release();
     tw.release();
+    twx.release();
+    twy.release();
     spec.release();
     table.release();
     part.release();
@@ -545,7 +562,10 @@ struct KSolver {
                      gse_gsek_s* marker = nullptr, GatherOut* defer = nullptr) {
     GSECHK(ensure_table(kind, sigma, bx, by, bz, s));
     FFT3& f = fused ? fft2 : fft;
-    GSECHK(f.exec(true, rho, spec.p, s));
+    if (plane)
+      HIPCHK(launch_xy_r2c(dbl, X, Y, Z, rho, spec.p, twx.p, twy.p, s));
+    else
+      GSECHK(f.exec(true, rho, spec.p, s));
     if (marker) GSECHK(stage_mark(marker));
     int nblocks = kKpassBlocks;
     if (fused) {
@@ -572,7 +592,10 @@ struct KSolver {
       }
     }
     if (marker) GSECHK(stage_mark(marker));
-    GSECHK(f.exec(false, spec.p, phi, s));
+    if (plane)
+      HIPCHK(launch_xy_c2r(dbl, X, Y, Z, spec.p, phi, twx.p, twy.p, s));
+    else
+      GSECHK(f.exec(false, spec.p, phi, s));
     if (marker) GSECHK(stage_mark(marker));
     return GSE_SUCCESS;
   }
@@ -846,6 +869,9 @@ gse_status_t gse_set_tuning_param(const char* name, int value) {
   else if (n == "zpass_cols" && (value == 4 || value == 8 || value == 16)) t.zpass_cols = value;
   else if (n == "gather_group" && (value == 16 || value == 32)) t.gather_group = value;
   else if (n == "zpass4" && (value == 0 || value == 1)) t.zpass4 = value;
+  else if (n == "xy_plane" && (value == 0 || value == 1)) t.xy_plane = value;
+  else if (n == "zpass4_cols" && (value == 0 || value == 16 || value == 32))
+    t.zpass4_cols = value;
   else if (n == "spread_mma" && (value == 0 || value == 1)) t.spread_mma = value;
   else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter or value out of range: %s=%d",
                    name, value);
@@ -873,6 +899,8 @@ gse_status_t gse_get_tuning_param(const char* name, int* value) {
   else if (n == "zpass_cols") *value = t.zpass_cols;
   else if (n == "gather_group") *value = t.gather_group;
   else if (n == "zpass4") *value = t.zpass4;
+  else if (n == "xy_plane") *value = t.xy_plane;
+  else if (n == "zpass4_cols") *value = t.zpass4_cols;
   else if (n == "spread_mma") *value = t.spread_mma;
   else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter: %s", name);
   return GSE_SUCCESS;

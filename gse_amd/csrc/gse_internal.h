@@ -55,6 +55,8 @@ struct Tuning {
   int fused_kz = 1;            // FFT solve as 2D x-y transforms around one fused z pass
   int zpass_cols = 8;          // spectrum columns per z-pass workgroup (4, 8, 16)
   int zpass4 = 1;              // four-step z pass for the column lengths it is built for
+  int zpass4_cols = 0;         // spectrum columns per four-step workgroup (0 auto, 16, 32)
+  int xy_plane = 0;            // x-y transforms of the fused solve by the plane kernels
   int gather_group = 32;       // lanes per charge in the FP32 13^3 gather walk (32, 16)
   int spread_mma = 0;          // FP32 spread on the matrix cores (int8 limbs, exact sums)
 };
@@ -343,7 +345,10 @@ struct ZPlan {
   int C;               // spectrum columns per workgroup
   int four;            // 1: the four-step kernel built for this Z (k_zpass4)
 };
-// false when Z has a prime factor above 8 or the columns do not fit LDS
+// mixed-radix plan (radices 2..8, largest first) of a transform length;
+// false when n has a prime factor above 7 or needs too many stages
+bool radix_plan(int n, ZPlan* pl);
+// false when Z has a prime factor above 7 or the columns do not fit LDS
 bool zplan_make(int Z, bool dbl, ZPlan* pl);
 size_t zpass_lds_bytes(const ZPlan& pl, bool dbl);
 int zpass_blocks(int X, int Y, const ZPlan& pl);
@@ -352,6 +357,17 @@ int zpass_blocks(int X, int Y, const ZPlan& pl);
 hipError_t launch_zpass(bool dbl, int kind, int X, int Y, int Z, double bx, double by, double bz,
                         double sigma, const ZPlan& pl, void* spec, const void* table,
                         const void* tw, bool vir, double* partials, hipStream_t s);
+
+// x-y plane transforms of the fused solve, one workgroup per z plane in LDS
+// (in place of rocFFT's 2D r2c / c2r): rho [Z][Y][X] real <-> spec
+// [Z][Y][X/2+1] complex, unnormalised.  xy_plane_ok: both lengths have radix
+// plans and a plane fits the LDS budget.  twx / twy: device [X] / [Y]
+// complex exp(-2 pi i m/n) of the compute type.
+bool xy_plane_ok(int X, int Y, bool dbl);
+hipError_t launch_xy_r2c(bool dbl, int X, int Y, int Z, const void* rho, void* spec,
+                         const void* twx, const void* twy, hipStream_t s);
+hipError_t launch_xy_c2r(bool dbl, int X, int Y, int Z, const void* spec, void* phi,
+                         const void* twx, const void* twy, hipStream_t s);
 
 // real-space reductions
 constexpr int kRedBlocks = 512;

@@ -97,6 +97,13 @@ GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
  *                 registers (two LDS exchanges per direction) for the column
  *                 lengths it is built for (Z = 80, 216), for handles created
  *                 afterwards; 0 = the Stockham form for every Z (0/1, 1)
+ *   zpass4_cols   spectrum columns per workgroup of the four-step z pass;
+ *                 0 = 32 for FP32 with Z > 128, else 16 (0, 16, 32; 0)
+ *   xy_plane      the x-y transforms around the fused z pass run as one
+ *                 workgroup per z plane in LDS (lengths with radices 2..7,
+ *                 a plane within 160 KiB of LDS), for handles created
+ *                 afterwards; 0 = rocFFT 2D transforms (0/1, 0: rocFFT's
+ *                 2D kernels measured faster at 64^2 to 128^2 planes)
  *   spread_mma    FP32 spreads with int32 bricks and a 13-point support run
  *                 on the matrix cores (int8 limb products summed exactly per
  *                 bin, the brick rounded once); 0 = LDS integer atomics (0/1, 0) */
@@ -287,9 +294,11 @@ GSE_API gse_status_t gse_fft_solver_c2r(gse_fft_solver_t h, void* rhoK, void* ph
 /* Build-defined diagnostic (no reference counterpart): *form = 1 when run()
  * takes 2D x-y transforms around the fused z pass (z transform, G(k)
  * multiply and energy/virial sums, inverse z transform in one kernel), 2
- * when that z pass is the four-step register form, 0 when run() takes 3D
- * transforms and a separate k-pass.  The form is fixed at create time by the
- * fused_kz and zpass4 tuning knobs and the factors of sizeZ. */
+ * when that z pass is the four-step register form, 3 and 4 for those two
+ * when the x-y transforms are the plane kernels instead of rocFFT 2D
+ * transforms, 0 when run() takes 3D transforms and a separate k-pass.  The
+ * form is fixed at create time by the fused_kz, zpass4 and xy_plane tuning
+ * knobs and the factors of the grid sizes. */
 GSE_API gse_status_t gse_fft_solver_get_form(gse_fft_solver_t h, int* form);
 
 /* CpuGreensFunc.h:16-18  mul(nfftX, nfftY, nfftZ, box, sigma, rhoK) */

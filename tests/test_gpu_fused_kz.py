@@ -40,13 +40,13 @@ def host(t):
 
 
 @contextlib.contextmanager
-def fused(on, four=0):
+def fused(on, four=0, xy=1):
     old = api.get_tuning()
-    api.set_tuning(fused_kz=int(on), zpass4=int(four))
+    api.set_tuning(fused_kz=int(on), zpass4=int(four), xy_plane=int(xy))
     try:
         yield
     finally:
-        api.set_tuning(fused_kz=old["fused_kz"], zpass4=old["zpass4"])
+        api.set_tuning(fused_kz=old["fused_kz"], zpass4=old["zpass4"], xy_plane=old["xy_plane"])
 
 
 # Z factorisations: radix 8 only, 8*8*2, 6*4, 7*5, 3*3*3*... (27), 2*2 with
@@ -63,9 +63,10 @@ DIMS = [
 ]
 
 
+@pytest.mark.parametrize("xy", [1, 0], ids=["plane", "rocfft2d"])
 @pytest.mark.parametrize("kind,sigma", [(0, 0.9487), (0, 0.0), (1, 0.9487), (2, 0.0)])
 @pytest.mark.parametrize("dims,box,form", DIMS, ids=["x".join(map(str, d[0])) for d in DIMS])
-def test_fused_solver_fp64_vs_oracle(kind, sigma, dims, box, form):
+def test_fused_solver_fp64_vs_oracle(kind, sigma, dims, box, form, xy):
     X, Y, Z = dims
     rho = np.random.default_rng(X * 1000 + Z).normal(size=(Z, Y, X))
     gf = [api.CpuGreensFuncGSE, api.CpuGreensFuncG2, api.CpuGreensFuncHermitian][kind]()
@@ -76,9 +77,10 @@ def test_fused_solver_fp64_vs_oracle(kind, sigma, dims, box, form):
         ref = orc.fft_solve(kind, dims, sigma, box, rho)
     out = {}
     for on in (True, False):
-        with fused(on):
+        with fused(on, 0, xy):
             s = api.CpuFFTSolver(gf, F64, *dims)
-        assert s.form() == (form if on else "3d")
+        want = form + ("/xy" if xy and form == "fused" else "")
+        assert s.form() == (want if on else "3d")
         phi = torch.zeros((Z, Y, X), dtype=F64, device=DEV)
         s.run(phi, to_dev(rho), sigma, *box)
         out[on] = host(phi)
@@ -105,7 +107,7 @@ def test_fused_gsek_step_fp64_vs_oracle(name, make, dims, box):
     for on in (True, False):
         with fused(on):
             gs = api.CpuGSEk(F64, F64, *dims, p.sigma, 0.4, 3.0, 3.0, *box)
-        assert gs.layout()["kspace"] == ("fused" if on else "3d")
+        assert gs.layout()["kspace"] == ("fused/xy" if on else "3d")
         o = gs.step(to_dev(xyzq), mode=0, force=True, potential=True, virial=True)
         E = float(host(o["energy"])[0])
         assert abs(E - r.energy) <= TOL * abs(r.energy), (on, E, r.energy)
@@ -141,7 +143,7 @@ def test_fused_gsek_mode_r_fp64():
     r = orc.gse_step(p, xyzq, mode=1)
     with fused(True):
         gs = api.CpuGSEk(F64, F64, *p.dims, p.sigma, 0.4, 3.0, 3.0, *p.box)
-    assert gs.layout()["kspace"] == "fused"
+    assert gs.layout()["kspace"] == "fused/xy"
     o = gs.step(to_dev(xyzq), mode=1, force=True, potential=True)
     E = float(host(o["energy"])[0])
     assert abs(E - r.energy) <= TOL * abs(r.energy)
@@ -159,10 +161,12 @@ FOUR_DIMS = [
 ]
 
 
+@pytest.mark.parametrize("cols", [16, 32])
+@pytest.mark.parametrize("xy", [1, 0], ids=["plane", "rocfft2d"])
 @pytest.mark.parametrize("kind,sigma", [(0, 0.9487), (1, 0.9487), (2, 0.0)])
 @pytest.mark.parametrize("dims,box", FOUR_DIMS, ids=["x".join(map(str, d[0])) for d in FOUR_DIMS])
 @pytest.mark.parametrize("dt", [F64, F32], ids=["f64", "f32"])
-def test_four_step_solver_vs_oracle_and_stockham(kind, sigma, dims, box, dt):
+def test_four_step_solver_vs_oracle_and_stockham(kind, sigma, dims, box, dt, xy, cols):
     X, Y, Z = dims
     rho = np.random.default_rng(X * 1000 + Z + 1).normal(size=(Z, Y, X))
     gf = [api.CpuGreensFuncGSE, api.CpuGreensFuncG2, api.CpuGreensFuncHermitian][kind]()
@@ -173,10 +177,17 @@ def test_four_step_solver_vs_oracle_and_stockham(kind, sigma, dims, box, dt):
         ref = orc.fft_solve(kind, dims, sigma, box, rho)
     tol = TOL if dt == F64 else 2e-6
     out = {}
+    # X = 11 has no radix plan: the x-y transforms stay with rocFFT
+    sfx = "/xy" if xy and X != 11 else ""
+    old_cols = api.get_tuning()["zpass4_cols"]
     for four in (1, 0):
-        with fused(True, four):
-            s = api.CpuFFTSolver(gf, dt, *dims)
-        assert s.form() == ("fused4" if four else "fused")
+        api.set_tuning(zpass4_cols=cols)
+        try:
+            with fused(True, four, xy):
+                s = api.CpuFFTSolver(gf, dt, *dims)
+        finally:
+            api.set_tuning(zpass4_cols=old_cols)
+        assert s.form() == ("fused4" if four else "fused") + sfx
         phi = torch.zeros((Z, Y, X), dtype=dt, device=DEV)
         s.run(phi, to_dev(rho, dt), sigma, *box)
         out[four] = host(phi)
@@ -194,7 +205,7 @@ def test_four_step_gsek_step_fp64_vs_oracle():
         r = orc.gse_step(p, xyzq, mode=mode)
         with fused(True, 1):
             gs = api.CpuGSEk(F64, F64, *dims, p.sigma, 0.4, 3.0, 3.0, *box)
-        assert gs.layout()["kspace"] == "fused4"
+        assert gs.layout()["kspace"] == "fused4/xy"
         o = gs.step(to_dev(xyzq), mode=mode, force=True, potential=True, virial=mode == 0)
         E = float(host(o["energy"])[0])
         assert abs(E - r.energy) <= TOL * abs(r.energy), (mode, E, r.energy)
@@ -213,7 +224,7 @@ def test_four_step_gsek_fp32_water_48k_M80():
     ref = orc.gse_step(p, xyzq.astype(np.float32).astype(np.float64))
     out = {}
     for four in (1, 0):
-        with fused(True, four):
+        with fused(True, four, 0):
             gs = api.CpuGSEk(F32, F32, M, M, M, sigma, 0.4, 3.0, 3.0, L, L, L)
         assert gs.layout()["kspace"] == ("fused4" if four else "fused")
         o = gs.step(to_dev(xyzq, F32), force=True, potential=True, virial=True)
@@ -222,3 +233,36 @@ def test_four_step_gsek_fp32_water_48k_M80():
         assert erms <= 1e-5, (four, erms)
     assert abs(out[1][0] - out[0][0]) <= 1e-5 * abs(out[0][0])
     assert rel(out[1][2], out[0][2]) <= 1e-5
+
+
+# Plane kernels (one workgroup per z plane does both x-y transforms in LDS)
+# on edge shapes: lengths 2 and 3, radix 7 twice (49), odd X and Y, and a
+# plane that fits the 160 KiB LDS budget in FP32 but not in FP64 (there the
+# x-y transforms stay with rocFFT).
+PLANE_DIMS = [
+    ((2, 2, 4), (2.0, 2.0, 4.0)),
+    ((3, 5, 8), (3.0, 5.0, 8.0)),
+    ((7, 49, 6), (7.0, 49.0, 6.0)),
+    ((21, 10, 12), (21.0, 10.5, 12.0)),
+    ((128, 96, 16), (128.0, 96.0, 16.0)),
+]
+
+
+@pytest.mark.parametrize("dims,box", PLANE_DIMS, ids=["x".join(map(str, d[0])) for d in PLANE_DIMS])
+@pytest.mark.parametrize("dt", [F64, F32], ids=["f64", "f32"])
+def test_plane_kernels_vs_oracle_and_rocfft2d(dims, box, dt):
+    X, Y, Z = dims
+    rho = np.random.default_rng(X * 7 + Y * 3 + Z).normal(size=(Z, Y, X))
+    ref = orc.fft_solve(0, dims, 0.9487, box, rho)
+    tol = TOL if dt == F64 else 2e-6
+    plane_fits = not (dt == F64 and dims[0] == 128)
+    out = {}
+    for xy in (1, 0):
+        with fused(True, 0, xy):
+            s = api.CpuFFTSolver(api.CpuGreensFuncGSE(), dt, *dims)
+        assert s.form() == ("fused/xy" if xy and plane_fits else "fused")
+        phi = torch.zeros((Z, Y, X), dtype=dt, device=DEV)
+        s.run(phi, to_dev(rho, dt), 0.9487, *box)
+        out[xy] = host(phi)
+        assert rel(out[xy], ref) <= tol, (xy, rel(out[xy], ref))
+    assert rel(out[1], out[0]) <= (1e-12 if dt == F64 else 2e-6)
