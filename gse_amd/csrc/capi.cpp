@@ -162,8 +162,13 @@ struct Engine {
     GSECHK(loc.ensure(sizeof(int) * slots));
     GSECHK(ovf.ensure(sizeof(int) * (size_t)(n > 0 ? n : 1)));
     GSECHK(part.ensure(sizeof(double) * kRedBlocks * 8));
-    const size_t sb = bin_sort_bytes(n, nb);
-    if (sb) GSECHK(binscan.ensure(sb));
+    const size_t sb = bin_sort_bytes(n, shape, ct_dbl);
+    if (sb) {
+      const void* before = binscan.p;
+      GSECHK(binscan.ensure(sb));
+      // the super-bin counters start at zero; the binning leaves them zero
+      if (binscan.p != before) HIPCHK(hipMemset(binscan.p, 0, binscan.bytes));
+    }
     for (int k = 0; k < 2; k++) {
       if (counts[k].bytes < sizeof(int) * (size_t)nb || !counts[k].p) {
         GSECHK(counts[k].ensure(sizeof(int) * (size_t)nb));
@@ -870,6 +875,8 @@ gse_status_t gse_set_tuning_param(const char* name, int value) {
   else if (n == "gather_group" && (value == 16 || value == 32)) t.gather_group = value;
   else if (n == "zpass4" && (value == 0 || value == 1)) t.zpass4 = value;
   else if (n == "xy_plane" && (value == 0 || value == 1)) t.xy_plane = value;
+  else if (n == "bin_coarse_block" && (value == 256 || value == 512 || value == 1024))
+    t.bin_coarse_block = value;
   else if (n == "zpass4_cols" && (value == 0 || value == 16 || value == 32))
     t.zpass4_cols = value;
   else if (n == "spread_mma" && (value == 0 || value == 1)) t.spread_mma = value;
@@ -901,6 +908,7 @@ gse_status_t gse_get_tuning_param(const char* name, int* value) {
   else if (n == "zpass4") *value = t.zpass4;
   else if (n == "xy_plane") *value = t.xy_plane;
   else if (n == "zpass4_cols") *value = t.zpass4_cols;
+  else if (n == "bin_coarse_block") *value = t.bin_coarse_block;
   else if (n == "spread_mma") *value = t.spread_mma;
   else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter: %s", name);
   return GSE_SUCCESS;

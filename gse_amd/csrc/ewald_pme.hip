@@ -42,6 +42,7 @@ __global__ void k_clear_stats(BinStats* st) {
   st->qmax_bits = 0ull;
   st->invalid = 0;
   st->overflow = 0;
+  st->spill = 0;
 }
 
 __global__ void k_zero_i32(int* p, int n) {

@@ -599,6 +599,7 @@ __global__ void __launch_bounds__(
       stats_other->qmax_bits = 0ull;
       stats_other->invalid = 0;
       stats_other->overflow = 0;
+      stats_other->spill = 0;
     }
     const int novf = stats->overflow;
     const int gw = (blockIdx.x - a.nwg_bins) * (BS / 64) + (threadIdx.x >> 6);
@@ -636,6 +637,7 @@ __global__ void __launch_bounds__(256)
     stats_other->qmax_bits = 0ull;
     stats_other->invalid = 0;
     stats_other->overflow = 0;
+    stats_other->spill = 0;
   }
   const double ds = o.dscale ? *o.dscale : 1.0;
   const int n = min(counts[0], a.cap_bin);

@@ -57,6 +57,7 @@ struct Tuning {
   int zpass4 = 1;              // four-step z pass for the column lengths it is built for
   int zpass4_cols = 0;         // spectrum columns per four-step workgroup (0 auto, 16, 32)
   int xy_plane = 0;            // x-y transforms of the fused solve by the plane kernels
+  int bin_coarse_block = 1024; // threads per super-bin binning workgroup (256, 512, 1024)
   int gather_group = 32;       // lanes per charge in the FP32 13^3 gather walk (32, 16)
   int spread_mma = 0;          // FP32 spread on the matrix cores (int8 limbs, exact sums)
 };
@@ -76,6 +77,7 @@ struct BinStats {
   unsigned long long qmax_bits;  // bits of max |q| (as a positive double)
   int invalid;                   // particles rejected (non-finite / out of range)
   int overflow;                  // particles that did not fit their bin
+  int spill;                     // particles that did not fit their super-bin
 };
 
 // Particles binned into fixed-capacity bins: bin b owns slots
@@ -97,8 +99,8 @@ struct Binning {
   double pref_bound = 0.0;     // bound of |q|-free contribution: pref * max weight
   void* bricks = nullptr;      // [nbins][Lz][Ly][Lx] LDS bricks (NULL: atomic flush)
   int* bshift = nullptr;       // [nbins] brick shift to 2^e, -1 for an empty bin
-  // counting-sort scratch of grids above kBinAggMaxBins bins (bin_sort_bytes):
-  // per-workgroup 16-bit bin counts, then their 32-bit exclusive offsets
+  // super-bin scratch of grids above 16384 bins (bin_sort_bytes): claim
+  // counters, then the charges and their indices grouped by super-bin
   void* scan = nullptr;
 };
 
@@ -149,8 +151,11 @@ hipError_t noalloc_scratch(void** out);
 // ct_double: particle/grid compute type is double (else float)
 // Clears counts[0..nbins) and the statistics of one buffer.
 hipError_t launch_bin_clear(int* counts, int nbins, BinStats* stats, hipStream_t s);
-// bytes of Binning::scan for n particles and nbins bins (0: not needed)
-size_t bin_sort_bytes(int n, int nbins);
+// bytes of Binning::scan (super-bin scratch of grids above 16384 bins) for
+// n particles of the compute type (dbl: double) in shape b (0: not needed);
+// its first bytes are counters that must be zero when first used (the
+// binning leaves them zero)
+size_t bin_sort_bytes(int n, const BinShape& b, bool dbl);
 hipError_t launch_bin(bool ct_double, const Geom& g, const BinShape& b, Binning& w,
                       hipStream_t s);
 

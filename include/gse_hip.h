@@ -97,6 +97,8 @@ GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
  *                 registers (two LDS exchanges per direction) for the column
  *                 lengths it is built for (Z = 80, 216), for handles created
  *                 afterwards; 0 = the Stockham form for every Z (0/1, 1)
+ *   bin_coarse_block  threads per workgroup of the first (super-bin) pass of
+ *                 the binning of grids above 16384 bins (256, 512, 1024; 1024)
  *   zpass4_cols   spectrum columns per workgroup of the four-step z pass;
  *                 0 = 32 for FP32 with Z > 128, else 16 (0, 16, 32; 0)
  *   xy_plane      the x-y transforms around the fused z pass run as one

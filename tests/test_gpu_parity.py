@@ -496,21 +496,23 @@ def test_binning_variants_give_identical_grids(dt):
 
 
 @pytest.mark.parametrize("dt", [F32, F64])
-def test_counting_sort_binning_above_16384_bins(dt):
-    """Grids of more than 16384 bins are binned by the three-pass counting
-    sort (per-workgroup histograms, offsets, placement).  It must give the
-    same results as one slot claim per charge, bit for bit: grids, energy
-    and the per-charge outputs, with a cluster that overfills its bin and
-    two rejected charges (M = 216: 19683 bins of 8 cells).  Which cluster
-    charges take the overflow path (gathered from global memory, summed in
-    another order) differs between the two binnings, so the cluster's
-    outputs agree to rounding only."""
+@pytest.mark.parametrize("ncluster", [600, 3000], ids=["bin_overflow", "superbin_overflow"])
+def test_superbin_binning_above_16384_bins(dt, ncluster):
+    """Grids of more than 16384 bins are binned in two passes (super-bins of
+    4^3 bins, then the bins of each super-bin).  It must give the same
+    results as one slot claim per charge, bit for bit: grids, energy and the
+    per-charge outputs, with a cluster that overfills its bin (600 charges)
+    or its whole super-bin (3000, past the super-bin's 2x mean + 256 slots)
+    and two rejected charges (M = 216: 19683 bins of 8 cells).  Which charges
+    of the cluster's super-bin take the overflow path (gathered from global
+    memory, summed in another order) differs between the two binnings, so
+    their outputs agree to rounding only."""
     rng = np.random.default_rng(31)
     L, M, n = 215.3, 216, 120000
     xyzq = np.empty((n, 4))
     xyzq[:, :3] = rng.random((n, 3)) * L
     xyzq[:, 3] = np.where(np.arange(n) % 2 == 0, 0.5, -0.5)
-    xyzq[:600, :3] = 50.0 + rng.random((600, 3)) * 0.9   # overfills one bin
+    xyzq[:ncluster, :3] = 50.0 + rng.random((ncluster, 3)) * 0.9   # overfills one bin
     xyzq[7, 0] = np.nan
     xyzq[9, 3] = np.inf
     x = to_dev(xyzq, dt)
@@ -527,12 +529,12 @@ def test_counting_sort_binning_above_16384_bins(dt):
                          o["pot"].clone()))
     finally:
         api.set_tuning(bin_agg=keys["bin_agg"])
-    # every charge of the cluster's bin (the cluster and the few others
-    # there) may take either path; the rest must agree bit for bit
+    # every charge of the cluster's super-bin may take either path; the rest
+    # must agree bit for bit
     good = np.isfinite(xyzq).all(axis=1)
-    o = np.mod(np.floor(np.nan_to_num(xyzq[:, :3]) / (L / M) + 0.5).astype(np.int64) - 6, M) // 8
-    bins = o[:, 0] + 27 * (o[:, 1] + 27 * o[:, 2])
-    hot = np.isin(bins, bins[10:600])
+    o = np.mod(np.floor(np.nan_to_num(xyzq[:, :3]) / (L / M) + 0.5).astype(np.int64) - 6, M) // 32
+    sup = o[:, 0] + 7 * (o[:, 1] + 7 * o[:, 2])
+    hot = np.isin(sup, sup[10:ncluster])
     keep_np = good & ~hot
     keep = torch.from_numpy(keep_np).to(DEV)
     assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
