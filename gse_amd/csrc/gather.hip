@@ -448,13 +448,54 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
             }
           }
         }
+        if constexpr (GS == 32 && sizeof(AT) == 4) {
+          // FP32: the sums in fixed groups of four (sphere force and phi_i;
+          // cube force; virial moments), each group over its 32-lane half in
+          // one transposed butterfly (group4_total), so every output has
+          // the same summation order whichever other outputs are requested;
+          // lane k < 4 of the group stores entry k of each group
+          const AT z0 = (AT)0;
+          const AT sA = group4_total(live && SPH ? f[0] : z0, live && SPH ? f[1] : z0,
+                                     live && SPH ? f[2] : z0, live && POT ? f[6] : z0, lane);
+          AT sB = z0, sC = z0, sD = z0;
+          if constexpr (CUB) sB = group4_total(live ? f[3] : z0, live ? f[4] : z0,
+                                               live ? f[5] : z0, z0, lane);
+          if constexpr (VIR) {
+            sC = group4_total(live ? f[7] : z0, live ? f[8] : z0, live ? f[9] : z0,
+                              live ? f[10] : z0, lane);
+            sD = group4_total(live ? f[11] : z0, live ? f[12] : z0, z0, z0, lane);
+          }
+          if (live && l32 < 4) {
+            const int oi = sP[ai];
+            const double qv = sQ[ai];
+            const double fq = qv * a.prefF * o.fscale * ds;
+            if (SPH && l32 < 3 && o.fx) {
+              AT* dst = (AT*)(l32 == 0 ? o.fx : l32 == 1 ? o.fy : o.fz);
+              dst[oi] = (AT)(fq * (double)sA);
+            }
+            if (POT && l32 == 3 && o.pot)
+              ((AT*)o.pot)[oi] = (AT)(a.prefP * o.pscale * ds * (double)sA);
+            if (CUB && l32 < 3 && o.cx) {
+              AT* dst = (AT*)(l32 == 0 ? o.cx : l32 == 1 ? o.cy : o.cz);
+              dst[oi] = (AT)(fq * (double)sB);
+            }
+            if (VIR && o.vir6) {
+              const double vq = qv * a.prefF * o.vscale;
+              const size_t st = (size_t)o.vir_stride;
+              o.vir6[l32 * st + oi] = vq * (double)sC;
+              if (l32 < 2) o.vir6[(4 + l32) * st + oi] = vq * (double)sD;
+            }
+          }
+        } else {
 #pragma unroll
-        for (int c = 0; c < 13; c++) {
-          const bool used = (c < 3 && SPH) || (c >= 3 && c < 6 && CUB) || (c == 6 && POT) ||
-                            (c >= 7 && VIR);
-          if (used) f[c] = GS == 32 ? half_total(live ? f[c] : (AT)0) : row_total(live ? f[c] : (AT)0);
+          for (int c = 0; c < 13; c++) {
+            const bool used = (c < 3 && SPH) || (c >= 3 && c < 6 && CUB) || (c == 6 && POT) ||
+                              (c >= 7 && VIR);
+            if (used)
+              f[c] = GS == 32 ? half_total(live ? f[c] : (AT)0) : row_total(live ? f[c] : (AT)0);
+          }
+          if (live && l32 == 0) gather_store<AT, FL>(a, o, sP[ai], sQ[ai], ds, f);
         }
-        if (live && l32 == 0) gather_store<AT, FL>(a, o, sP[ai], sQ[ai], ds, f);
       }
       __syncthreads();
       continue;

@@ -103,6 +103,24 @@ __device__ __forceinline__ T half_total(T v) {
   return (threadIdx.x & 32) ? r2 + r3 : r0 + r1;
 }
 
+// Sums of four values over each 32-lane half of the wave in one transposed
+// butterfly: after the lane ^ 1 and lane ^ 2 exchanges lane l holds value
+// l & 3 summed over its quad, rotations by 4 and 8 lanes add the row's four
+// quads, and one shuffle adds the half's other row.  Lane l returns the
+// half's sum of value l & 3 (fixed order, deterministic).
+__device__ __forceinline__ float group4_total(float a, float b, float c, float d, int lane) {
+  const bool o1 = (lane & 1) != 0, o2 = (lane & 2) != 0;
+  float p = o1 ? b : a;
+  p += dpp_move<0xB1>(o1 ? a : b);
+  float r = o1 ? d : c;
+  r += dpp_move<0xB1>(o1 ? c : d);
+  float v = o2 ? r : p;
+  v += dpp_move<0x4E>(o2 ? p : r);
+  v += dpp_move<0x124>(v);  // row_ror:4
+  v += dpp_move<0x128>(v);  // row_ror:8
+  return v + shfl(v, lane ^ 16);
+}
+
 // Non-negative modulo
 __device__ __forceinline__ int wrapi(int i, int m) {
   int r = i % m;
