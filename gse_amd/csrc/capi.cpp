@@ -309,8 +309,10 @@ struct PlanPair {
 struct PlanKey {
   int dbl, X, Y, Z, batch;
   int dim = 3;  // 3: 3D transforms; 2: 2D x-y transforms of the Z planes
+  int cdist = 0;  // 2D: elements between the half spectrum's z planes (0: (X/2+1) Y)
   bool operator<(const PlanKey& o) const {
-    return std::tie(dbl, X, Y, Z, batch, dim) < std::tie(o.dbl, o.X, o.Y, o.Z, o.batch, o.dim);
+    return std::tie(dbl, X, Y, Z, batch, dim, cdist) <
+           std::tie(o.dbl, o.X, o.Y, o.Z, o.batch, o.dim, o.cdist);
   }
 };
 std::mutex g_plan_mu;
@@ -324,11 +326,12 @@ gse_status_t create_plan_pair(const PlanKey& k, PlanPair& pp) {
   rocfft_plan_description df = nullptr, db = nullptr;
   if (k.dim == 2) {
     // the Z x-y planes of a [z][y][x] grid into the [z][y][x <= X/2] half
-    // spectrum: the layout the 3D r2c leaves, before its z transform
+    // spectrum: the layout the 3D r2c leaves, before its z transform, with
+    // its planes k.cdist elements apart when that is set
     const size_t Xh = (size_t)X / 2 + 1;
     const size_t rs[2] = {1, (size_t)X};
     const size_t cs[2] = {1, Xh};
-    const size_t rd = (size_t)X * Y, cd = Xh * Y;
+    const size_t rd = (size_t)X * Y, cd = k.cdist > 0 ? (size_t)k.cdist : Xh * Y;
     if (rocfft_plan_description_create(&df) != rocfft_status_success ||
         rocfft_plan_description_create(&db) != rocfft_status_success ||
         rocfft_plan_description_set_data_layout(df, rocfft_array_type_real,
@@ -420,12 +423,13 @@ struct FFT3 {
   // batch > 1: that many transforms of packed, back-to-back grids (distance
   // X*Y*Z reals) and half spectra (distance (X/2+1)*Y*Z complex).
   // dim = 2: the 2D x-y transforms of the Z planes only.
-  gse_status_t init(bool d, int x, int y, int z, int batch = 1, int dim = 3) {
+  // cdist (dim 2): elements between the half spectrum's z planes
+  gse_status_t init(bool d, int x, int y, int z, int batch = 1, int dim = 3, int cdist = 0) {
     dbl = d;
     X = x;
     Y = y;
     Z = z;
-    key = PlanKey{d ? 1 : 0, x, y, z, batch, dim};
+    key = PlanKey{d ? 1 : 0, x, y, z, batch, dim, cdist};
     PlanPair pp;
     GSECHK(acquire_plans(key, &pp));
     fwd = pp.fwd;
@@ -485,6 +489,7 @@ struct KSolver {
   bool plane = false;
   DevBuf twx, twy;
   int zblocks = 0;
+  int zs = 0;  // elements between the spectrum's (and G table's) z planes
   DevBuf spec, table, part;
   int tab_kind = -1;
   double tab_sigma = NAN, tab_bx = NAN, tab_by = NAN, tab_bz = NAN;
@@ -517,17 +522,20 @@ struct KSolver {
     Y = y;
     Z = z;
     GSECHK(fft.init(d, x, y, z));
-    GSECHK(spec.ensure(2 * tsize(d) * nk()));
-    GSECHK(table.ensure(tsize(d) * nk()));
     size_t nparts = kKpassBlocks;
     fused = g_tuning.fused_kz && zplan_make(z, d, &zp);
+    // the fused form's planes start on whole cache lines (the z pass reads
+    // one piece of every plane per column group); the 3D form's are packed
+    zs = fused ? fused_plane_stride(x, y) : (x / 2 + 1) * y;
+    GSECHK(spec.ensure(2 * tsize(d) * (size_t)zs * z));
+    GSECHK(table.ensure(tsize(d) * (size_t)zs * z));
     if (fused) {
       plane = g_tuning.xy_plane && xy_plane_ok(x, y, d);
       if (plane) {
         GSECHK(upload_twiddles(twx, x));
         GSECHK(upload_twiddles(twy, y));
       } else {
-        GSECHK(fft2.init(d, x, y, z, 1, 2));
+        GSECHK(fft2.init(d, x, y, z, 1, 2, zs));
       }
       zblocks = zpass_blocks(x, y, zp);
       nparts = std::max<size_t>(nparts, (size_t)zblocks);
@@ -550,7 +558,7 @@ struct KSolver {
                             hipStream_t s) {
     if (kind == tab_kind && sigma == tab_sigma && bx == tab_bx && by == tab_by && bz == tab_bz)
       return GSE_SUCCESS;
-    HIPCHK(launch_greens_table(dbl, kind, X, Y, Z, bx, by, bz, sigma, table.p, s));
+    HIPCHK(launch_greens_table(dbl, kind, X, Y, Z, bx, by, bz, sigma, table.p, s, zs));
     tab_kind = kind;
     tab_sigma = sigma;
     tab_bx = bx;
@@ -568,14 +576,14 @@ struct KSolver {
     GSECHK(ensure_table(kind, sigma, bx, by, bz, s));
     FFT3& f = fused ? fft2 : fft;
     if (plane)
-      HIPCHK(launch_xy_r2c(dbl, X, Y, Z, rho, spec.p, twx.p, twy.p, s));
+      HIPCHK(launch_xy_r2c(dbl, X, Y, Z, rho, spec.p, twx.p, twy.p, s, zs));
     else
       GSECHK(f.exec(true, rho, spec.p, s));
     if (marker) GSECHK(stage_mark(marker));
     int nblocks = kKpassBlocks;
     if (fused) {
       HIPCHK(launch_zpass(dbl, kind, X, Y, Z, bx, by, bz, sigma, zp, spec.p, table.p, tw.p,
-                          virial != nullptr, part.as<double>(), s));
+                          virial != nullptr, part.as<double>(), s, zs));
       nblocks = zblocks;
     } else {
       HIPCHK(launch_kpass(dbl, kind, X, Y, Z, bx, by, bz, sigma, spec.p, table.p,
@@ -598,7 +606,7 @@ struct KSolver {
     }
     if (marker) GSECHK(stage_mark(marker));
     if (plane)
-      HIPCHK(launch_xy_c2r(dbl, X, Y, Z, spec.p, phi, twx.p, twy.p, s));
+      HIPCHK(launch_xy_c2r(dbl, X, Y, Z, spec.p, phi, twx.p, twy.p, s, zs));
     else
       GSECHK(f.exec(false, spec.p, phi, s));
     if (marker) GSECHK(stage_mark(marker));

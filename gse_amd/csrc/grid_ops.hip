@@ -51,6 +51,7 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NC], double* partial
 
 struct KArgs {
   int X, Y, Z, Xh;
+  int zs;  // elements between consecutive z planes of the spectrum / G table
   int halfX, halfY, halfZ;
   double twopi_bx, twopi_by, twopi_bz;
   double fourpi_vol, inv_vol, sigmafac;
@@ -63,6 +64,7 @@ static KArgs make_kargs(int X, int Y, int Z, double bx, double by, double bz, do
   a.Y = Y;
   a.Z = Z;
   a.Xh = X / 2 + 1;
+  a.zs = a.Xh * Y;
   a.halfX = X / 2 + (X % 2);
   a.halfY = Y / 2 + (Y % 2);
   a.halfZ = Z / 2 + (Z % 2);
@@ -118,9 +120,12 @@ __device__ __forceinline__ void kvec(const KArgs& a, size_t idx, int& ix, double
   kz = mul_rn((double)mz, a.twopi_bz);
 }
 
+// table[z zs + y Xh + x] (zs >= Xh Y: a padded plane stride leaves the
+// entries past Xh Y of each plane unwritten)
 template <typename T>
 __global__ void k_greens_table(KArgs a, int kind, T* __restrict__ table) {
-  const size_t n = (size_t)a.Xh * a.Y * a.Z;
+  const size_t plane = (size_t)a.Xh * a.Y;
+  const size_t n = plane * a.Z;
   for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n;
        idx += (size_t)gridDim.x * blockDim.x) {
     int ix;
@@ -135,7 +140,7 @@ __global__ void k_greens_table(KArgs a, int kind, T* __restrict__ table) {
     } else {
       fac = mehrstellen_g(a, idx);
     }
-    table[idx] = (T)fac;
+    table[(idx / plane) * a.zs + idx % plane] = (T)fac;
   }
 }
 
@@ -280,7 +285,7 @@ __global__ void __launch_bounds__(kZpassThreads)
   C2<T>* B = A + (size_t)Z * C;
   C2<T>* W = B + (size_t)Z * C;
   const int ncol = a.Xh * a.Y;
-  const size_t zs = (size_t)ncol;
+  const size_t zs = (size_t)a.zs;  // memory stride of z (kvec takes ncol)
   const int col0 = blockIdx.x * C;
   // every load of the workgroup's columns and of their G(k) values is issued
   // before the first LDS store, so a thread has all of them in flight; the
@@ -322,7 +327,7 @@ __global__ void __launch_bounds__(kZpassThreads)
     if constexpr (VIR) {
       int ixx;
       double kx, ky, kz;
-      kvec(a, z * zs + c, ixx, kx, ky, kz);
+      kvec(a, (size_t)z * ncol + c, ixx, kx, ky, kz);
       const double k2 = kx * kx + ky * ky + kz * kz;
       if (k2 > 0.0) {
         const double cf = ek * 2.0 * (a.sigmafac + (kind == 0 ? 1.0 / k2 : 0.0));
@@ -380,7 +385,7 @@ __global__ void __launch_bounds__((zpass4_threads<Z1, Z2, C>()))
   C2<T>* buf = (C2<T>*)zsm;  // [Z][C]: entry (k1 Z2 + n2) of every column
   C2<T>* W = buf + (size_t)Z * C;
   const int ncol = a.Xh * a.Y;
-  const size_t zs = (size_t)ncol;
+  const size_t zs = (size_t)a.zs;  // memory stride of z (kvec takes ncol)
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   const int col = blockIdx.x * C + c;
   const bool okc = col < ncol;
@@ -423,7 +428,7 @@ __global__ void __launch_bounds__((zpass4_threads<Z1, Z2, C>()))
       if constexpr (VIR) {
         int ixx;
         double kx, ky, kz;
-        kvec(a, (size_t)(t + Z1 * k2) * zs + col, ixx, kx, ky, kz);
+        kvec(a, (size_t)(t + Z1 * k2) * ncol + col, ixx, kx, ky, kz);
         const double k2n = kx * kx + ky * ky + kz * kz;
         if (k2n > 0.0) {
           const double cf = ek * 2.0 * (a.sigmafac + (kind == 0 ? 1.0 / k2n : 0.0));
@@ -568,8 +573,12 @@ static void zpass_t(const KArgs& a, const ZPlan& pl, int kind, void* spec, const
 
 hipError_t launch_zpass(bool dbl, int kind, int X, int Y, int Z, double bx, double by, double bz,
                         double sigma, const ZPlan& pl, void* spec, const void* table,
-                        const void* tw, bool vir, double* partials, hipStream_t s) {
-  const KArgs a = make_kargs(X, Y, Z, bx, by, bz, sigma);
+                        const void* tw, bool vir, double* partials, hipStream_t s, int zs) {
+  KArgs a = make_kargs(X, Y, Z, bx, by, bz, sigma);
+  if (zs > 0) {
+    if (zs < a.zs) return hipErrorInvalidValue;
+    a.zs = zs;
+  }
   const unsigned g = (unsigned)zpass_blocks(X, Y, pl);
   const size_t lds = zpass_lds_bytes(pl, dbl);
   if (pl.four) {
@@ -978,9 +987,15 @@ static unsigned grid_for(size_t count, unsigned threads, unsigned cap) {
   return (unsigned)g;
 }
 
+int fused_plane_stride(int X, int Y) { return ((X / 2 + 1) * Y + 31) / 32 * 32; }
+
 hipError_t launch_greens_table(bool dbl, int kind, int X, int Y, int Z, double bx, double by,
-                               double bz, double sigma, void* table, hipStream_t s) {
-  const KArgs a = make_kargs(X, Y, Z, bx, by, bz, sigma);
+                               double bz, double sigma, void* table, hipStream_t s, int zs) {
+  KArgs a = make_kargs(X, Y, Z, bx, by, bz, sigma);
+  if (zs > 0) {
+    if (zs < a.zs) return hipErrorInvalidValue;
+    a.zs = zs;
+  }
   const size_t n = (size_t)a.Xh * Y * Z;
   if (dbl)
     k_greens_table<double><<<grid_for(n, 256, 8192), 256, 0, s>>>(a, kind, (double*)table);

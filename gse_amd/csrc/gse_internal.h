@@ -320,9 +320,14 @@ size_t spread_lds_bytes(bool dbl, const Geom& g, const BinShape& b, int* cap);
 size_t gather_lds_bytes(bool dbl, const Geom& g, const BinShape& b, int* cap);
 
 // k-space
+// zs: elements between z planes of the table (0: (X/2+1) Y, packed)
 hipError_t launch_greens_table(bool dbl, int kind, int X, int Y, int Z,
                                double bx, double by, double bz, double sigma,
-                               void* table, hipStream_t s);
+                               void* table, hipStream_t s, int zs = 0);
+// plane stride of the fused solve's spectrum and G table: (X/2+1) Y rounded
+// up to 32 elements, so every z plane starts on a 256-byte (FP32 complex) /
+// 512-byte (FP64) boundary and the z pass reads whole cache lines
+int fused_plane_stride(int X, int Y);
 
 // In-place multiply of the half spectrum by the table, FP64 partial sums of
 // the energy (and the 6 k-space virial moments when vir is set) per block.
@@ -359,9 +364,10 @@ size_t zpass_lds_bytes(const ZPlan& pl, bool dbl);
 int zpass_blocks(int X, int Y, const ZPlan& pl);
 // tw: device [Z] complex exp(-2 pi i m/Z) of the compute type; partials
 // [zpass_blocks][vir ? 7 : 1]
+// spec / table planes zs elements apart (0: (X/2+1) Y)
 hipError_t launch_zpass(bool dbl, int kind, int X, int Y, int Z, double bx, double by, double bz,
                         double sigma, const ZPlan& pl, void* spec, const void* table,
-                        const void* tw, bool vir, double* partials, hipStream_t s);
+                        const void* tw, bool vir, double* partials, hipStream_t s, int zs = 0);
 
 // x-y plane transforms of the fused solve, one workgroup per z plane in LDS
 // (in place of rocFFT's 2D r2c / c2r): rho [Z][Y][X] real <-> spec
@@ -370,9 +376,9 @@ hipError_t launch_zpass(bool dbl, int kind, int X, int Y, int Z, double bx, doub
 // complex exp(-2 pi i m/n) of the compute type.
 bool xy_plane_ok(int X, int Y, bool dbl);
 hipError_t launch_xy_r2c(bool dbl, int X, int Y, int Z, const void* rho, void* spec,
-                         const void* twx, const void* twy, hipStream_t s);
+                         const void* twx, const void* twy, hipStream_t s, int zs = 0);
 hipError_t launch_xy_c2r(bool dbl, int X, int Y, int Z, const void* spec, void* phi,
-                         const void* twx, const void* twy, hipStream_t s);
+                         const void* twx, const void* twy, hipStream_t s, int zs = 0);
 
 // real-space reductions
 constexpr int kRedBlocks = 512;

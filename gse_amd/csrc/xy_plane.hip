@@ -116,7 +116,7 @@ static size_t plane_lds_bytes(int X, int Y, bool dbl) {
 
 template <typename T>
 __global__ void __launch_bounds__(kPlaneThreads)
-    k_xy_r2c(const T* __restrict__ rho, C2<T>* __restrict__ spec, int X, int Y, ZPlan px,
+    k_xy_r2c(const T* __restrict__ rho, C2<T>* __restrict__ spec, int X, int Y, int zs, ZPlan px,
              ZPlan py, const C2<T>* __restrict__ twx, const C2<T>* __restrict__ twy) {
   extern __shared__ __align__(16) char psm[];
   const int Xh = X / 2 + 1, Yh = (Y + 1) / 2;
@@ -161,13 +161,13 @@ __global__ void __launch_bounds__(kPlaneThreads)
   }
   __syncthreads();
   C2<T>* F = plane_fft<T>(O, R, Wy, py, Xh, Xh, 1, true, false);
-  C2<T>* dst = spec + (size_t)blockIdx.x * Xh * Y;
+  C2<T>* dst = spec + (size_t)blockIdx.x * zs;
   for (int idx = threadIdx.x; idx < Xh * Y; idx += blockDim.x) dst[idx] = F[idx];
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kPlaneThreads)
-    k_xy_c2r(const C2<T>* __restrict__ spec, T* __restrict__ phi, int X, int Y, ZPlan px,
+    k_xy_c2r(const C2<T>* __restrict__ spec, T* __restrict__ phi, int X, int Y, int zs, ZPlan px,
              ZPlan py, const C2<T>* __restrict__ twx, const C2<T>* __restrict__ twy) {
   extern __shared__ __align__(16) char psm[];
   const int Xh = X / 2 + 1, Yh = (Y + 1) / 2;
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(kPlaneThreads)
   C2<T>* B = A + nbuf;
   C2<T>* Wx = B + nbuf;
   C2<T>* Wy = Wx + X;
-  const C2<T>* src = spec + (size_t)blockIdx.x * Xh * Y;
+  const C2<T>* src = spec + (size_t)blockIdx.x * zs;
   const int nsp = Xh * Y;
   for (int base = 0; base < nsp; base += kPlaneLoads * kPlaneThreads) {
     C2<T> v[kPlaneLoads];
@@ -233,35 +233,39 @@ bool xy_plane_ok(int X, int Y, bool dbl) {
 }
 
 hipError_t launch_xy_r2c(bool dbl, int X, int Y, int Z, const void* rho, void* spec,
-                         const void* twx, const void* twy, hipStream_t s) {
+                         const void* twx, const void* twy, hipStream_t s, int zs) {
   ZPlan px, py;
   if (Z < 1 || !xy_plane_ok(X, Y, dbl) || !radix_plan(X, &px) || !radix_plan(Y, &py))
     return hipErrorInvalidValue;
+  if (zs == 0) zs = (X / 2 + 1) * Y;
+  if (zs < (X / 2 + 1) * Y) return hipErrorInvalidValue;
   const size_t lds = plane_lds_bytes(X, Y, dbl);
   if (dbl)
     k_xy_r2c<double><<<Z, kPlaneThreads, lds, s>>>((const double*)rho, (C2<double>*)spec, X, Y,
-                                                   px, py, (const C2<double>*)twx,
+                                                   zs, px, py, (const C2<double>*)twx,
                                                    (const C2<double>*)twy);
   else
-    k_xy_r2c<float><<<Z, kPlaneThreads, lds, s>>>((const float*)rho, (C2<float>*)spec, X, Y, px,
-                                                  py, (const C2<float>*)twx,
+    k_xy_r2c<float><<<Z, kPlaneThreads, lds, s>>>((const float*)rho, (C2<float>*)spec, X, Y, zs,
+                                                  px, py, (const C2<float>*)twx,
                                                   (const C2<float>*)twy);
   return hipGetLastError();
 }
 
 hipError_t launch_xy_c2r(bool dbl, int X, int Y, int Z, const void* spec, void* phi,
-                         const void* twx, const void* twy, hipStream_t s) {
+                         const void* twx, const void* twy, hipStream_t s, int zs) {
   ZPlan px, py;
   if (Z < 1 || !xy_plane_ok(X, Y, dbl) || !radix_plan(X, &px) || !radix_plan(Y, &py))
     return hipErrorInvalidValue;
+  if (zs == 0) zs = (X / 2 + 1) * Y;
+  if (zs < (X / 2 + 1) * Y) return hipErrorInvalidValue;
   const size_t lds = plane_lds_bytes(X, Y, dbl);
   if (dbl)
     k_xy_c2r<double><<<Z, kPlaneThreads, lds, s>>>((const C2<double>*)spec, (double*)phi, X, Y,
-                                                   px, py, (const C2<double>*)twx,
+                                                   zs, px, py, (const C2<double>*)twx,
                                                    (const C2<double>*)twy);
   else
-    k_xy_c2r<float><<<Z, kPlaneThreads, lds, s>>>((const C2<float>*)spec, (float*)phi, X, Y, px,
-                                                  py, (const C2<float>*)twx,
+    k_xy_c2r<float><<<Z, kPlaneThreads, lds, s>>>((const C2<float>*)spec, (float*)phi, X, Y, zs,
+                                                  px, py, (const C2<float>*)twx,
                                                   (const C2<float>*)twy);
   return hipGetLastError();
 }
