@@ -27,7 +27,8 @@ REDUCE_PATHS = {-1: "none", 0: "atomic", 1: "generic", 2: "tile4", 3: "tile4_nt"
                 5: "tile"}
 
 # k-space form of an FFT solve (gse_*_get_layout field 10, gse_fft_solver_get_form)
-KSPACE_FORMS = {-1: "none", 0: "3d", 1: "fused", 2: "fused4", 3: "fused/xy", 4: "fused4/xy"}
+KSPACE_FORMS = {-1: "none", 0: "3d", 1: "fused", 2: "fused4", 3: "fused/xy", 4: "fused4/xy",
+                5: "fused/y4", 6: "fused4/y4"}
 
 _lib = None
 

@@ -296,7 +296,9 @@ class CpuFFTSolver:
         """"fused": run() takes 2D rocFFT transforms around the fused z
         pass; "fused4": the same with the four-step register z pass;
         "fused/xy", "fused4/xy": those z passes between the plane kernels'
-        x-y transforms; "3d": 3D transforms and a separate k-pass."""
+        x-y transforms; "fused/y4", "fused4/y4": between 1D x rocFFT
+        transforms and the four-step y pass; "3d": 3D transforms and a
+        separate k-pass."""
         v = ctypes.c_int()
         check(self._L.gse_fft_solver_get_form(self._h, ctypes.byref(v)))
         return _lib.KSPACE_FORMS[v.value]
@@ -481,7 +483,7 @@ TUNING_KEYS = ("bin_edge", "brick_budget", "spread_block", "gather_block", "spre
                "gather_stage", "spread_lds32", "spread_cols", "spread_flush",
                "reduce_tile", "reduce_nt", "reduce_tile2", "bin_agg", "fused_kz", "spread_mma",
                "zpass_cols", "gather_group", "zpass4", "xy_plane",
-               "zpass4_cols", "bin_coarse_block")
+               "zpass4_cols", "bin_coarse_block", "ypass4")
 
 
 def set_tuning(**kw):

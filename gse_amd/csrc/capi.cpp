@@ -320,7 +320,7 @@ std::map<PlanKey, PlanPair> g_plans;
 
 gse_status_t create_plan_pair(const PlanKey& k, PlanPair& pp) {
   const int X = k.X, Y = k.Y, Z = k.Z;
-  const int batch = k.dim == 2 ? Z : k.batch;
+  const int batch = k.dim == 2 ? Z : k.dim == 1 ? Y * Z : k.batch;
   const size_t lengths[3] = {(size_t)X, (size_t)Y, (size_t)Z};
   const rocfft_precision prec = k.dbl ? rocfft_precision_double : rocfft_precision_single;
   rocfft_plan_description df = nullptr, db = nullptr;
@@ -345,6 +345,23 @@ gse_status_t create_plan_pair(const PlanKey& k, PlanPair& pp) {
       if (db) rocfft_plan_description_destroy(db);
       return fail(GSE_ERROR_FFT, "rocfft 2D plan description (%dx%d x %d) failed", X, Y, Z);
     }
+  } else if (k.dim == 1) {
+    // the Y Z rows of a [z][y][x] grid into packed [z][y][x <= X/2] rows
+    const size_t Xh = (size_t)X / 2 + 1;
+    const size_t one[1] = {1};
+    if (rocfft_plan_description_create(&df) != rocfft_status_success ||
+        rocfft_plan_description_create(&db) != rocfft_status_success ||
+        rocfft_plan_description_set_data_layout(df, rocfft_array_type_real,
+                                                rocfft_array_type_hermitian_interleaved,
+                                                nullptr, nullptr, 1, one, (size_t)X, 1, one,
+                                                Xh) != rocfft_status_success ||
+        rocfft_plan_description_set_data_layout(db, rocfft_array_type_hermitian_interleaved,
+                                                rocfft_array_type_real, nullptr, nullptr, 1, one,
+                                                Xh, 1, one, (size_t)X) != rocfft_status_success) {
+      if (df) rocfft_plan_description_destroy(df);
+      if (db) rocfft_plan_description_destroy(db);
+      return fail(GSE_ERROR_FFT, "rocfft 1D plan description (%d x %d) failed", X, Y * Z);
+    }
   } else if (batch > 1) {
     const size_t Xh = (size_t)X / 2 + 1;
     const size_t rs[3] = {1, (size_t)X, (size_t)X * Y};
@@ -365,7 +382,7 @@ gse_status_t create_plan_pair(const PlanKey& k, PlanPair& pp) {
                   batch);
     }
   }
-  const size_t ndim = k.dim == 2 ? 2 : 3;
+  const size_t ndim = (size_t)k.dim;
   const rocfft_status sf =
       rocfft_plan_create(&pp.fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
                          prec, ndim, lengths, (size_t)batch, df);
@@ -488,6 +505,10 @@ struct KSolver {
   // x-y transforms of the fused form by the plane kernels (else rocFFT 2D)
   bool plane = false;
   DevBuf twx, twy;
+  // else, where built: 1D x rocFFT into packed spec2, then the four-step y
+  // pass into the padded spectrum (and back)
+  bool ypass = false;
+  DevBuf spec2;
   int zblocks = 0;
   int zs = 0;  // elements between the spectrum's (and G table's) z planes
   DevBuf spec, table, part;
@@ -496,8 +517,9 @@ struct KSolver {
 
   size_t nk() const { return (size_t)(X / 2 + 1) * Y * Z; }
   // 0: 3D transforms + k-pass, 1: fused Stockham z pass, 2: fused four-step;
-  // +2 when the plane kernels do the x-y transforms
-  int form() const { return fused ? (zp.four ? 2 : 1) + (plane ? 2 : 0) : 0; }
+  // +2 when the plane kernels do the x-y transforms, +4 with the 1D x
+  // rocFFT transforms and the four-step y pass
+  int form() const { return fused ? (zp.four ? 2 : 1) + (plane ? 2 : 0) + (ypass ? 4 : 0) : 0; }
 
   // table exp(-2 pi i m/n), m < n, in FP64 rounded to the compute type
   gse_status_t upload_twiddles(DevBuf& b, int n) {
@@ -531,9 +553,14 @@ struct KSolver {
     GSECHK(table.ensure(tsize(d) * (size_t)zs * z));
     if (fused) {
       plane = g_tuning.xy_plane && xy_plane_ok(x, y, d);
+      ypass = !plane && g_tuning.ypass4 && ypass4_shape(y);
       if (plane) {
         GSECHK(upload_twiddles(twx, x));
         GSECHK(upload_twiddles(twy, y));
+      } else if (ypass) {
+        GSECHK(fft2.init(d, x, y, z, 1, 1));
+        GSECHK(upload_twiddles(twy, y));
+        GSECHK(spec2.ensure(2 * tsize(d) * nk()));
       } else {
         GSECHK(fft2.init(d, x, y, z, 1, 2, zs));
       }
@@ -550,6 +577,7 @@ struct KSolver {
     tw.release();
     twx.release();
     twy.release();
+    spec2.release();
     spec.release();
     table.release();
     part.release();
@@ -575,10 +603,15 @@ struct KSolver {
                      gse_gsek_s* marker = nullptr, GatherOut* defer = nullptr) {
     GSECHK(ensure_table(kind, sigma, bx, by, bz, s));
     FFT3& f = fused ? fft2 : fft;
-    if (plane)
+    const int packed = (X / 2 + 1) * Y;
+    if (plane) {
       HIPCHK(launch_xy_r2c(dbl, X, Y, Z, rho, spec.p, twx.p, twy.p, s, zs));
-    else
+    } else if (ypass) {
+      GSECHK(f.exec(true, rho, spec2.p, s));
+      HIPCHK(launch_ypass4(dbl, false, X, Y, Z, spec2.p, packed, spec.p, zs, twy.p, s));
+    } else {
       GSECHK(f.exec(true, rho, spec.p, s));
+    }
     if (marker) GSECHK(stage_mark(marker));
     int nblocks = kKpassBlocks;
     if (fused) {
@@ -605,10 +638,14 @@ struct KSolver {
       }
     }
     if (marker) GSECHK(stage_mark(marker));
-    if (plane)
+    if (plane) {
       HIPCHK(launch_xy_c2r(dbl, X, Y, Z, spec.p, phi, twx.p, twy.p, s, zs));
-    else
+    } else if (ypass) {
+      HIPCHK(launch_ypass4(dbl, true, X, Y, Z, spec.p, zs, spec2.p, packed, twy.p, s));
+      GSECHK(f.exec(false, spec2.p, phi, s));
+    } else {
       GSECHK(f.exec(false, spec.p, phi, s));
+    }
     if (marker) GSECHK(stage_mark(marker));
     return GSE_SUCCESS;
   }
@@ -883,6 +920,7 @@ gse_status_t gse_set_tuning_param(const char* name, int value) {
   else if (n == "gather_group" && (value == 16 || value == 32)) t.gather_group = value;
   else if (n == "zpass4" && (value == 0 || value == 1)) t.zpass4 = value;
   else if (n == "xy_plane" && (value == 0 || value == 1)) t.xy_plane = value;
+  else if (n == "ypass4" && (value == 0 || value == 1)) t.ypass4 = value;
   else if (n == "bin_coarse_block" && (value == 256 || value == 512 || value == 1024))
     t.bin_coarse_block = value;
   else if (n == "zpass4_cols" && (value == 0 || value == 16 || value == 32))
@@ -915,6 +953,7 @@ gse_status_t gse_get_tuning_param(const char* name, int* value) {
   else if (n == "gather_group") *value = t.gather_group;
   else if (n == "zpass4") *value = t.zpass4;
   else if (n == "xy_plane") *value = t.xy_plane;
+  else if (n == "ypass4") *value = t.ypass4;
   else if (n == "zpass4_cols") *value = t.zpass4_cols;
   else if (n == "bin_coarse_block") *value = t.bin_coarse_block;
   else if (n == "spread_mma") *value = t.spread_mma;

@@ -470,6 +470,79 @@ __global__ void __launch_bounds__((zpass4_threads<Z1, Z2, C>()))
   }
 }
 
+// ---------------------------------------------------------------------------
+// Four-step y pass: the y transform of every (z, kx) column of the half
+// spectrum, Y = Y1 Y2 in registers with one LDS exchange, as k_zpass4 does
+// along z.  Forward: src packed [z][y][kx] (a batched 1D x r2c's output),
+// dst planes dzs elements apart; inverse: the reverse (src planes szs
+// apart, dst packed, for the 1D x c2r).  Column q = z Xh + kx; C consecutive
+// columns per workgroup, so loads and stores run along kx.
+// ---------------------------------------------------------------------------
+template <typename T, int Y1, int Y2, int C, bool INV>
+__global__ void __launch_bounds__((zpass4_threads<Y1, Y2, C>()))
+    k_ypass4(int Xh, int Z, const C2<T>* __restrict__ src, int szs, C2<T>* __restrict__ dst,
+             int dzs, const C2<T>* __restrict__ tw) {
+  constexpr int Y = Y1 * Y2;
+  constexpr int NT = zpass4_threads<Y1, Y2, C>();
+  extern __shared__ __align__(16) char ysm[];
+  C2<T>* buf = (C2<T>*)ysm;  // [Y][C]
+  C2<T>* W = buf + (size_t)Y * C;
+  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  const int q = blockIdx.x * C + c;
+  const bool okc = q < Xh * Z;
+  const int z = okc ? q / Xh : 0, kx = okc ? q - z * Xh : 0;
+  const C2<T>* s0 = src + (size_t)z * szs + kx;
+  C2<T>* d0 = dst + (size_t)z * dzs + kx;
+  C2<T> u[Y1];
+#pragma unroll
+  for (int n1 = 0; n1 < Y1; n1++)
+    u[n1] = (t < Y2 && okc) ? s0[(size_t)(Y2 * n1 + t) * Xh] : C2<T>{(T)0, (T)0};
+  for (int i = threadIdx.x; i < Y; i += NT) W[i] = tw[i];
+  __syncthreads();
+  if (t < Y2) {  // step one, thread n2 = t
+    reg_dft<T, Y1, Y>(u, W, INV);
+#pragma unroll
+    for (int k1 = 0; k1 < Y1; k1++) {
+      const C2<T> v = k1 > 0 ? zc_mul(u[k1], W[t * k1], INV) : u[k1];
+      buf[(k1 * Y2 + t) * C + c] = v;
+    }
+  }
+  __syncthreads();
+  if (t < Y1 && okc) {  // step two, thread k1 = t: v[k2] = X[t + Y1 k2]
+    C2<T> v[Y2];
+#pragma unroll
+    for (int n2 = 0; n2 < Y2; n2++) v[n2] = buf[(t * Y2 + n2) * C + c];
+    reg_dft<T, Y2, Y>(v, W, INV);
+#pragma unroll
+    for (int k2 = 0; k2 < Y2; k2++) d0[(size_t)(t + Y1 * k2) * Xh] = v[k2];
+  }
+}
+
+bool ypass4_shape(int Y) { return Y == 216; }
+
+hipError_t launch_ypass4(bool dbl, bool inv, int X, int Y, int Z, const void* src, int szs,
+                         void* dst, int dzs, const void* tw, hipStream_t s) {
+  if (Y != 216) return hipErrorInvalidValue;
+  constexpr int Y1 = 12, Y2 = 18, C = 32;
+  const int Xh = X / 2 + 1;
+  if (szs < Xh * Y || dzs < Xh * Y) return hipErrorInvalidValue;
+  constexpr int NT = zpass4_threads<Y1, Y2, C>();
+  const unsigned g = (unsigned)(((size_t)Xh * Z + C - 1) / C);
+  const size_t lds = ((size_t)Y * C + Y) * (dbl ? 16 : 8);
+#define GSE_YPASS(TT, INVV)                                                                \
+  k_ypass4<TT, Y1, Y2, C, INVV><<<g, NT, lds, s>>>(Xh, Z, (const C2<TT>*)src, szs,         \
+                                                   (C2<TT>*)dst, dzs, (const C2<TT>*)tw)
+  if (dbl) {
+    if (inv) GSE_YPASS(double, true);
+    else GSE_YPASS(double, false);
+  } else {
+    if (inv) GSE_YPASS(float, true);
+    else GSE_YPASS(float, false);
+  }
+#undef GSE_YPASS
+  return hipGetLastError();
+}
+
 // the four-step forms built: Z -> (Z1, Z2), 16 or 32 columns per workgroup
 struct Zpass4Shape {
   int Z, Z1, Z2;

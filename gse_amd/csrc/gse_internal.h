@@ -57,6 +57,7 @@ struct Tuning {
   int zpass4 = 1;              // four-step z pass for the column lengths it is built for
   int zpass4_cols = 0;         // spectrum columns per four-step workgroup (0 auto, 16, 32)
   int xy_plane = 0;            // x-y transforms of the fused solve by the plane kernels
+  int ypass4 = 1;              // fused solve: 1D x rocFFT + four-step y pass where built
   int bin_coarse_block = 1024; // threads per super-bin binning workgroup (256, 512, 1024)
   int gather_group = 32;       // lanes per charge in the FP32 13^3 gather walk (32, 16)
   int spread_mma = 0;          // FP32 spread on the matrix cores (int8 limbs, exact sums)
@@ -368,6 +369,13 @@ int zpass_blocks(int X, int Y, const ZPlan& pl);
 hipError_t launch_zpass(bool dbl, int kind, int X, int Y, int Z, double bx, double by, double bz,
                         double sigma, const ZPlan& pl, void* spec, const void* table,
                         const void* tw, bool vir, double* partials, hipStream_t s, int zs = 0);
+
+// y transform of the half spectrum in registers (four-step, Y = 216 = 12 x 18):
+// forward from packed planes (src, szs = (X/2+1) Y) into planes dzs apart,
+// inverse the other way; tw: device [Y] complex exp(-2 pi i m/Y)
+bool ypass4_shape(int Y);
+hipError_t launch_ypass4(bool dbl, bool inv, int X, int Y, int Z, const void* src, int szs,
+                         void* dst, int dzs, const void* tw, hipStream_t s);
 
 // x-y plane transforms of the fused solve, one workgroup per z plane in LDS
 // (in place of rocFFT's 2D r2c / c2r): rho [Z][Y][X] real <-> spec

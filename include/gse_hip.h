@@ -99,6 +99,10 @@ GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
  *                 afterwards; 0 = the Stockham form for every Z (0/1, 1)
  *   bin_coarse_block  threads per workgroup of the first (super-bin) pass of
  *                 the binning of grids above 16384 bins (256, 512, 1024; 1024)
+ *   ypass4        for Y = 216 the x-y transforms around the fused z pass run
+ *                 as 1D x rocFFT transforms and a four-step y pass in
+ *                 registers (rocFFT's 2D plan takes two kernels there), for
+ *                 handles created afterwards (0/1, 1)
  *   zpass4_cols   spectrum columns per workgroup of the four-step z pass;
  *                 0 = 32 for FP32 with Z > 128, else 16 (0, 16, 32; 0)
  *   xy_plane      the x-y transforms around the fused z pass run as one
@@ -298,9 +302,10 @@ GSE_API gse_status_t gse_fft_solver_c2r(gse_fft_solver_t h, void* rhoK, void* ph
  * multiply and energy/virial sums, inverse z transform in one kernel), 2
  * when that z pass is the four-step register form, 3 and 4 for those two
  * when the x-y transforms are the plane kernels instead of rocFFT 2D
- * transforms, 0 when run() takes 3D transforms and a separate k-pass.  The
- * form is fixed at create time by the fused_kz, zpass4 and xy_plane tuning
- * knobs and the factors of the grid sizes. */
+ * transforms, 5 and 6 when they are 1D x rocFFT transforms and the
+ * four-step y pass, 0 when run() takes 3D transforms and a separate k-pass.
+ * The form is fixed at create time by the fused_kz, zpass4, xy_plane and
+ * ypass4 tuning knobs and the factors of the grid sizes. */
 GSE_API gse_status_t gse_fft_solver_get_form(gse_fft_solver_t h, int* form);
 
 /* CpuGreensFunc.h:16-18  mul(nfftX, nfftY, nfftZ, box, sigma, rhoK) */

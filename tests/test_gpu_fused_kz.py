@@ -266,3 +266,46 @@ def test_plane_kernels_vs_oracle_and_rocfft2d(dims, box, dt):
         out[xy] = host(phi)
         assert rel(out[xy], ref) <= tol, (xy, rel(out[xy], ref))
     assert rel(out[1], out[0]) <= (1e-12 if dt == F64 else 2e-6)
+
+
+# Four-step y pass (Y = 216 = 12 x 18 in registers, 32 columns per
+# workgroup) between 1D x rocFFT transforms, around both z passes; odd X and
+# column counts Xh Z that are not a multiple of 32 included.
+YPASS_DIMS = [
+    ((12, 216, 10), (12.0, 216.0, 10.5)),
+    ((7, 216, 80), (7.0, 216.0, 80.0)),
+    ((20, 216, 216), (20.0, 216.0, 216.0)),
+]
+
+
+@pytest.mark.parametrize("kind,sigma", [(0, 0.9487), (2, 0.0)])
+@pytest.mark.parametrize("dims,box", YPASS_DIMS, ids=["x".join(map(str, d[0])) for d in YPASS_DIMS])
+@pytest.mark.parametrize("dt", [F64, F32], ids=["f64", "f32"])
+def test_ypass_solver_vs_oracle_and_rocfft2d(kind, sigma, dims, box, dt):
+    X, Y, Z = dims
+    rho = np.random.default_rng(X * 5 + Z).normal(size=(Z, Y, X))
+    gf = [api.CpuGreensFuncGSE, api.CpuGreensFuncG2, api.CpuGreensFuncHermitian][kind]()
+    if kind == 2:
+        rho -= rho.mean()
+        ref = orc.mehrstellen_solve(dims, box, rho)
+    else:
+        ref = orc.fft_solve(kind, dims, sigma, box, rho)
+    tol = TOL if dt == F64 else 2e-6
+    four = Z in (80, 216)
+    old = api.get_tuning()["ypass4"]
+    out = {}
+    for yp in (1, 0):
+        api.set_tuning(ypass4=yp)
+        try:
+            with fused(True, 1, 0):
+                s = api.CpuFFTSolver(gf, dt, *dims)
+        finally:
+            api.set_tuning(ypass4=old)
+        assert s.form() == ("fused4" if four else "fused") + ("/y4" if yp else "")
+        phi = torch.zeros((Z, Y, X), dtype=dt, device=DEV)
+        s.run(phi, to_dev(rho, dt), sigma, *box)
+        out[yp] = host(phi)
+        if kind == 2:
+            out[yp] -= out[yp].mean()
+        assert rel(out[yp], ref) <= tol, (yp, rel(out[yp], ref))
+    assert rel(out[1], out[0]) <= (1e-12 if dt == F64 else 2e-6)
