@@ -39,7 +39,7 @@ MEASURED_TRAFFIC = {
 # their bytes need; their bound is VALU issue: 256 CUs x 2.4 GHz x one wave64
 # VALU instruction per CU per clock (4 SIMDs, 16 lanes each) = 614.4 G/s.
 MEASURED_VALU = {
-    ("water_48k_M80", "fp32", "gather"): 18501031,
+    ("water_48k_M80", "fp32", "gather"): 15711103,
     ("water_48k_M80", "fp32", "spread"): 8345423,
 }
 VALU_PEAK_GINST = 614.4
