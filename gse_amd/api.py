@@ -483,7 +483,8 @@ TUNING_KEYS = ("bin_edge", "brick_budget", "spread_block", "gather_block", "spre
                "gather_stage", "spread_lds32", "spread_cols", "spread_flush",
                "reduce_tile", "reduce_nt", "reduce_tile2", "bin_agg", "fused_kz", "spread_mma",
                "zpass_cols", "gather_group", "zpass4", "xy_plane",
-               "zpass4_cols", "bin_coarse_block", "ypass4")
+               "zpass4_cols", "bin_coarse_block", "ypass4",
+               "bin_order")
 
 
 def set_tuning(**kw):

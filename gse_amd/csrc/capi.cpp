@@ -121,7 +121,7 @@ struct Engine {
   bool ct_dbl = true, at_dbl = true;
   int X = 0, Y = 0, Z = 0;
   DevBuf counts[2], stats[2], ovf, perm, loc, sorted, grid64, vir6, part, bricks, bshift,
-      binscan;
+      binscan, border;
   bool brick_flush = false;     // the last spread stored bricks (k_brick_reduce converts)
   int reduce_path = -1;         // ReducePath of the last conversion (-1: none yet)
   int spread_path = -1;         // last spread: 0 LDS integer atomics, 1 matrix cores
@@ -134,7 +134,7 @@ struct Engine {
 
   void release() {
     for (DevBuf* b : {&counts[0], &counts[1], &stats[0], &stats[1], &ovf, &perm, &loc, &sorted,
-                      &grid64, &vir6, &part, &bricks, &bshift, &binscan})
+                      &grid64, &vir6, &part, &bricks, &bshift, &binscan, &border})
       b->release();
   }
 
@@ -209,6 +209,13 @@ struct Engine {
     bin.pref_bound = std::pow(2.0 * M_PI * sigma * sigma, -1.5) *
                      spread_weight_bound(g, sigma, normalize);
     HIPCHK(launch_bin(ct_dbl, g, shape, bin, s));
+    // the spread's and the gather's workgroups take bins by descending count
+    bin.order = nullptr;
+    if (!shape.direct && n > 0 && g_tuning.bin_order) {
+      GSECHK(border.ensure(sizeof(int) * (size_t)nb));
+      HIPCHK(launch_bin_order(shape, bin, border.as<int>(), s));
+      bin.order = border.as<int>();
+    }
     return GSE_SUCCESS;
   }
 
@@ -923,6 +930,7 @@ gse_status_t gse_set_tuning_param(const char* name, int value) {
   else if (n == "ypass4" && (value == 0 || value == 1)) t.ypass4 = value;
   else if (n == "bin_coarse_block" && (value == 256 || value == 512 || value == 1024))
     t.bin_coarse_block = value;
+  else if (n == "bin_order" && (value == 0 || value == 1)) t.bin_order = value;
   else if (n == "zpass4_cols" && (value == 0 || value == 16 || value == 32))
     t.zpass4_cols = value;
   else if (n == "spread_mma" && (value == 0 || value == 1)) t.spread_mma = value;
@@ -956,6 +964,7 @@ gse_status_t gse_get_tuning_param(const char* name, int* value) {
   else if (n == "ypass4") *value = t.ypass4;
   else if (n == "zpass4_cols") *value = t.zpass4_cols;
   else if (n == "bin_coarse_block") *value = t.bin_coarse_block;
+  else if (n == "bin_order") *value = t.bin_order;
   else if (n == "spread_mma") *value = t.spread_mma;
   else return fail(GSE_ERROR_INVALID_VALUE, "unknown tuning parameter: %s", name);
   return GSE_SUCCESS;

@@ -40,6 +40,7 @@ struct GatherArgs {
   int cap;       // particles staged per chunk
   int cap_bin;   // slots per bin
   int nwg_bins;
+  const int* order;  // bin of each workgroup position (NULL: the position itself)
 };
 
 __device__ __forceinline__ float fmac(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -747,8 +748,9 @@ __global__ void __launch_bounds__(
     }
     return;
   }
-  const int bin = xcd_remap(blockIdx.x, a.nwg_bins);
-  if (bin >= a.b.nbins) return;
+  const int pos = xcd_remap(blockIdx.x, a.nwg_bins);
+  if (pos >= a.b.nbins) return;
+  const int bin = a.order ? a.order[pos] : pos;
   const int cnt = min(counts[bin], a.cap_bin);
   if (threadIdx.x == 0) counts_other[bin] = 0;
   if (cnt == 0) return;
@@ -811,6 +813,7 @@ __device__ __forceinline__ GatherArgs batch_gather_args(const BatchSys& S) {
   a.cap = S.gather_cap;
   a.cap_bin = S.cap_bin;
   a.nwg_bins = 0;
+  a.order = nullptr;
   return a;
 }
 
@@ -969,6 +972,7 @@ static hipError_t gather_nz(GatherArgs a, const Binning& w, const void* phi, con
   const size_t lds = gather_lds(vol, cap, NW, sizeof(CT)).total;
   if (lds > 150 * 1024) return hipErrorInvalidConfiguration;
   a.nwg_bins = ((a.b.nbins + 7) / 8) * 8;
+  a.order = w.order;
   const unsigned nwg = (unsigned)(a.nwg_bins + kOverflowWG + (o.fin_partials ? 1 : 0));
   const int bs = tu.gather_block ? tu.gather_block : (sizeof(CT) == 4 ? 512 : 256);
 #define GSE_G_ARGS a, xq, srt, w.perm, w.counts, w.counts_other, w.stats, w.stats_other, w.ovf, ph, o, \
@@ -1035,6 +1039,7 @@ hipError_t launch_gather(bool ct_double, bool at_double, const Geom& g, const Bi
   a.cap = 0;
   a.cap_bin = w.cap_bin;
   a.nwg_bins = 0;
+  a.order = nullptr;
   if (ct_double) return gather_fl<double, double>(flags, a, w, phi, o, s);
   if (at_double) return gather_fl<float, double>(flags, a, w, phi, o, s);
   return gather_fl<float, float>(flags, a, w, phi, o, s);

@@ -59,6 +59,7 @@ struct Tuning {
   int xy_plane = 0;            // x-y transforms of the fused solve by the plane kernels
   int ypass4 = 1;              // fused solve: 1D x rocFFT + four-step y pass where built
   int bin_coarse_block = 1024; // threads per super-bin binning workgroup (256, 512, 1024)
+  int bin_order = 1;           // spread/gather workgroups take bins by descending count
   int gather_group = 32;       // lanes per charge in the FP32 13^3 gather walk (32, 16)
   int spread_mma = 0;          // FP32 spread on the matrix cores (int8 limbs, exact sums)
 };
@@ -103,6 +104,9 @@ struct Binning {
   // super-bin scratch of grids above 16384 bins (bin_sort_bytes): claim
   // counters, then the charges and their indices grouped by super-bin
   void* scan = nullptr;
+  // [nbins] the bin each spread/gather workgroup position takes (NULL: bin =
+  // position); filled by launch_bin_order
+  int* order = nullptr;
 };
 
 // Fixed-point exponent e (scale 2^e) of one spread: every contribution
@@ -159,6 +163,9 @@ hipError_t launch_bin_clear(int* counts, int nbins, BinStats* stats, hipStream_t
 size_t bin_sort_bytes(int n, const BinShape& b, bool dbl);
 hipError_t launch_bin(bool ct_double, const Geom& g, const BinShape& b, Binning& w,
                       hipStream_t s);
+// order[] of the per-bin spread and gather workgroups from this binning's
+// counts: within each XCD's eighth of the positions, bins by descending count
+hipError_t launch_bin_order(const BinShape& b, const Binning& w, int* order, hipStream_t s);
 
 hipError_t launch_spread(bool ct_double, const Geom& g, const BinShape& b,
                          double sigma, bool normalize, Binning& w,

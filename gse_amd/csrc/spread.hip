@@ -50,6 +50,10 @@ __device__ __forceinline__ int xcd_remap(int wg, int nwg) {
   return (wg & 7) * chunk + (wg >> 3);
 }
 
+// The bin taken by spread/gather workgroup position p (order: NULL or the
+// table of k_bin_order)
+__device__ __forceinline__ int bin_at(const int* order, int p) { return order ? order[p] : p; }
+
 __global__ void k_bin_clear(int* counts, int nbins, BinStats* stats) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nbins; i += gridDim.x * blockDim.x)
     counts[i] = 0;
@@ -315,7 +319,8 @@ __device__ __forceinline__ bool bin_locate(const Geom& g, const BinShape& b, con
 
 // Exclusive prefix sums out[0..n) of in[0..n) over the workgroup (BT
 // threads, each summing a segment); returns the total.  wsum: BT/64 + 1
-// entries of LDS.  Starts and ends with every thread past a barrier.
+// entries of LDS; in and out must not overlap.  Starts and ends with every
+// thread past a barrier.
 template <int BT>
 __device__ unsigned block_excl_scan(const unsigned* in, unsigned* out, int n, unsigned* wsum) {
   const int per = (n + BT - 1) / BT;
@@ -351,6 +356,32 @@ __device__ unsigned block_excl_scan(const unsigned* in, unsigned* out, int n, un
   const unsigned total = wsum[BT / 64];
   __syncthreads();
   return total;
+}
+
+// Workgroup schedule of the per-bin spread and gather: xcd_remap gives XCD x
+// the positions x*chunk .. x*chunk + chunk - 1, which its workgroups take in
+// order.  order[] holds, at those positions, the same bins sorted by
+// descending charge count (a counting sort per eighth, one workgroup each),
+// so every XCD starts its longest bins first and the short ones fill the end
+// of the launch.  Bins stay with their XCD and its L2; each bin's work, and
+// so every result, is unchanged.  Equal counts go in any order.
+constexpr int kOrderKeys = 4096;
+__global__ void __launch_bounds__(1024)
+    k_bin_order(const int* __restrict__ counts, int nbins, int chunk, int* __restrict__ order) {
+  __shared__ unsigned hist[kOrderKeys], start[kOrderKeys];
+  __shared__ unsigned wsum[1024 / 64 + 1];
+  const int p0 = blockIdx.x * chunk, p1 = min(p0 + chunk, nbins);
+  for (int k = threadIdx.x; k < kOrderKeys; k += 1024) hist[k] = 0u;
+  __syncthreads();
+  // key kOrderKeys - 1 - count: ascending keys are descending counts
+  for (int p = p0 + (int)threadIdx.x; p < p1; p += 1024)
+    atomicAdd(&hist[kOrderKeys - 1 - min(max(counts[p], 0), kOrderKeys - 1)], 1u);
+  __syncthreads();
+  block_excl_scan<1024>(hist, start, kOrderKeys, wsum);  // in and out must differ
+  for (int p = p0 + (int)threadIdx.x; p < p1; p += 1024) {
+    const int k = kOrderKeys - 1 - min(max(counts[p], 0), kOrderKeys - 1);
+    order[p0 + (int)atomicAdd(&start[k], 1u)] = p;
+  }
 }
 
 // LDS of k_bin_coarse: [nsuper] counts (then the claimed bases), [nsuper]
@@ -648,6 +679,7 @@ struct SpreadArgs {
   int images;
   double pref_bound;
   int nwg_bins;
+  const int* order;  // bin of each workgroup position (NULL: the position itself)
 };
 
 // LDS layout of k_spread: int64 brick at 0, then per staged particle (cap
@@ -942,8 +974,9 @@ __global__ void __launch_bounds__(BS, BS == 512 ? 6 : 1)
       spread_overflow<CT, ACC>(a, eg, st.qmax_bits, counts, xyzq[ovf[k]], grid64);
     return;
   }
-  const int bin = xcd_remap(blockIdx.x, a.nwg_bins);
-  if (bin >= a.b.nbins) return;
+  const int pos = xcd_remap(blockIdx.x, a.nwg_bins);
+  if (pos >= a.b.nbins) return;
+  const int bin = bin_at(a.order, pos);
   const int call = counts[bin];
   const int cnt = min(call, a.cap_bin);
   if (threadIdx.x == 0) {
@@ -1172,8 +1205,9 @@ __global__ void __launch_bounds__(kMmaThreads)
     stats_other->overflow = 0;
     stats_other->spill = 0;
   }
-  const int bin = xcd_remap(blockIdx.x, a.nwg_bins);
-  if (bin >= a.b.nbins) return;
+  const int pos = xcd_remap(blockIdx.x, a.nwg_bins);
+  if (pos >= a.b.nbins) return;
+  const int bin = bin_at(a.order, pos);
   const int call = counts[bin];
   const int cnt = min(call, a.cap_bin);
   if (threadIdx.x == 0) {
@@ -1412,8 +1446,9 @@ __global__ void __launch_bounds__(BS)
       spread_overflow<CT, ACC>(a, eg, st.qmax_bits, counts, xyzq[ovf[k]], grid64);
     return;
   }
-  const int bin = xcd_remap(blockIdx.x, a.nwg_bins);
-  if (bin >= b.nbins) return;
+  const int pos = xcd_remap(blockIdx.x, a.nwg_bins);
+  if (pos >= b.nbins) return;
+  const int bin = bin_at(a.order, pos);
   const int call = counts[bin];
   const int cnt = min(call, a.cap_bin);
   if (threadIdx.x == 0) counts_other[bin] = 0;
@@ -1877,6 +1912,7 @@ __device__ __forceinline__ SpreadArgs batch_spread_args(const BatchSys& S) {
   a.images = S.images;
   a.pref_bound = S.pref_bound;
   a.nwg_bins = 0;
+  a.order = nullptr;
   return a;
 }
 
@@ -2194,6 +2230,13 @@ hipError_t launch_bin_clear(int* counts, int nbins, BinStats* stats, hipStream_t
   return hipGetLastError();
 }
 
+hipError_t launch_bin_order(const BinShape& b, const Binning& w, int* order, hipStream_t s) {
+  if (b.direct || b.nbins <= 0 || !order) return hipErrorInvalidValue;
+  const int chunk = ((b.nbins + 7) / 8 * 8) / 8;  // xcd_remap's, for nwg_bins
+  k_bin_order<<<8, 1024, 0, s>>>(w.counts, b.nbins, chunk, order);
+  return hipGetLastError();
+}
+
 hipError_t launch_bin(bool dbl, const Geom& g, const BinShape& b, Binning& w, hipStream_t s) {
   // bin_agg: particles per workgroup / 256
   const int agg = tuning().bin_agg;
@@ -2281,6 +2324,7 @@ static hipError_t spread_t(const Geom& g, const BinShape& b, double sigma, bool 
   a.n = w.n;
   a.images = w.images;
   a.pref_bound = w.pref_bound;
+  a.order = w.order;
   const int NX = 2 * g.nx + 1, NY = 2 * g.ny + 1, NZ = 2 * g.nz + 1;
   if (b.direct) {
     a.cap = 0;

@@ -99,6 +99,10 @@ GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
  *                 afterwards; 0 = the Stockham form for every Z (0/1, 1)
  *   bin_coarse_block  threads per workgroup of the first (super-bin) pass of
  *                 the binning of grids above 16384 bins (256, 512, 1024; 1024)
+ *   bin_order     the spread's and the gather's workgroups take the bins of
+ *                 each XCD's share by descending charge count, so the longest
+ *                 start first (a small sort kernel after the binning); 0 =
+ *                 bin order (0/1, 1)
  *   ypass4        for Y = 216 the x-y transforms around the fused z pass run
  *                 as 1D x rocFFT transforms and a four-step y pass in
  *                 registers (rocFFT's 2D plan takes two kernels there), for

@@ -70,13 +70,13 @@ def test_tuning_knobs_round_trip_and_reject_bad_values(lib):
             "spread_flush": 0, "reduce_tile": 0, "reduce_nt": 1, "reduce_tile2": 0, "bin_agg": 2,
             "fused_kz": 0, "spread_mma": 0, "zpass_cols": 16, "gather_group": 16,
             "zpass4": 0, "xy_plane": 0, "zpass4_cols": 32,
-            "bin_coarse_block": 512, "ypass4": 0}
+            "bin_coarse_block": 512, "ypass4": 0, "bin_order": 0}
     bad = {"bin_edge": 0, "spread_block": 384, "gather_block": 1024, "spread_stage": 512,
            "gather_stage": 200000, "spread_lds32": 2, "reduce_tile": -1, "reduce_nt": 2,
            "reduce_tile2": 5, "bin_agg": 3, "fused_kz": 2, "spread_mma": -1, "zpass_cols": 5,
            "gather_group": 8, "zpass4": 2,
            "xy_plane": -1, "zpass4_cols": 8,
-           "bin_coarse_block": 128, "ypass4": 2}
+           "bin_coarse_block": 128, "ypass4": 2, "bin_order": 3}
     assert set(good) == set(keys)
     try:
         for k, v in good.items():

@@ -546,6 +546,59 @@ def test_superbin_binning_above_16384_bins(dt, ncluster):
     assert rel(host(outs[0][3][cl]), host(outs[1][3][cl])) <= tol
 
 
+@pytest.mark.parametrize("dt", [F32, F64])
+@pytest.mark.parametrize("wl", ["water_48k_M80", "superbin"])
+def test_bin_order_gives_identical_results(dt, wl):
+    """The spread's and the gather's workgroups take each XCD's bins by
+    descending charge count (bin_order=1) or in bin order (0).  Only the
+    schedule changes: grids, energy, forces and potentials agree bit for
+    bit, on the 48k box (1000 bins, 17 to 92 charges each) and on a 216^3
+    grid (19683 bins, super-bin binning) with a cluster that overfills one
+    bin.  Which charges of that cluster's super-bin take the overflow path
+    (gathered from global memory, summed in another order) depends on the
+    order of the binning's slot claims, so their outputs agree to rounding
+    only."""
+    if wl == "superbin":
+        rng = np.random.default_rng(5)
+        L, M, sigma, n = 215.3, 216, 2.12, 60000
+        xyzq = np.empty((n, 4))
+        xyzq[:, :3] = rng.random((n, 3)) * L
+        xyzq[:, 3] = np.where(np.arange(n) % 2 == 0, 0.5, -0.5)
+        xyzq[:400, :3] = 80.0 + rng.random((400, 3)) * 0.9
+    else:
+        xyzq, L, M, sigma = systems.make(wl)
+    x = to_dev(xyzq, dt)
+    old = api.get_tuning()["bin_order"]
+    outs = []
+    try:
+        for bo in (1, 0):
+            api.set_tuning(bin_order=bo)
+            gs = api.CpuGSEk(dt, dt, M, M, M, sigma, 0.4, 3.0, 3.0, L, L, L)
+            o = gs.step(x, force=True, potential=True, virial=True)
+            outs.append([gs.getRhoS().clone()] + [o[k].clone() for k in
+                                                 ("energy", "force", "pot", "virial")])
+    finally:
+        api.set_tuning(bin_order=old)
+    keep_np = np.ones(len(xyzq), dtype=bool)
+    if wl == "superbin":
+        o = np.mod(np.floor(xyzq[:, :3] / (L / M) + 0.5).astype(np.int64) - 6, M) // 32
+        sup = o[:, 0] + 7 * (o[:, 1] + 7 * o[:, 2])
+        keep_np = ~np.isin(sup, sup[:400])
+    keep = torch.from_numpy(keep_np).to(DEV)
+    for a, b in zip(outs[0][:2], outs[1][:2]):  # grid and energy
+        assert torch.equal(a, b)
+    assert torch.equal(outs[0][2][:, keep], outs[1][2][:, keep])
+    assert torch.equal(outs[0][3][keep], outs[1][3][keep])
+    if wl == "superbin":
+        tol = 1e-5 if dt == F32 else 1e-12
+        hot = torch.from_numpy(~keep_np).to(DEV)
+        assert rel(host(outs[0][2][:, hot]), host(outs[1][2][:, hot])) <= tol
+        assert rel(host(outs[0][3][hot]), host(outs[1][3][hot])) <= tol
+        assert rel(host(outs[0][4]), host(outs[1][4])) <= tol
+    else:
+        assert torch.equal(outs[0][4], outs[1][4])
+
+
 def _tiling_system(seed):
     """A 48^3 grid in a 46 A box: h = 0.958, n = 6, so 8^3 bins with 20^3
     bricks (64-bit bricks fit the LDS budget) and 6 bins per axis: every
