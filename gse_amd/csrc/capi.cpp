@@ -121,7 +121,11 @@ struct Engine {
   bool ct_dbl = true, at_dbl = true;
   int X = 0, Y = 0, Z = 0;
   DevBuf counts[2], stats[2], ovf, perm, loc, sorted, grid64, vir6, part, bricks, bshift,
-      binscan, border;
+      binscan, border[2];
+  // schedule of the per-bin workgroups: border[ocur] holds one for order_nb
+  // bins when order_ok; the gather writes the next into border[ocur ^ 1]
+  int order_nb = 0, ocur = 0;
+  bool order_ok = false;
   bool brick_flush = false;     // the last spread stored bricks (k_brick_reduce converts)
   int reduce_path = -1;         // ReducePath of the last conversion (-1: none yet)
   int spread_path = -1;         // last spread: 0 LDS integer atomics, 1 matrix cores
@@ -134,8 +138,11 @@ struct Engine {
 
   void release() {
     for (DevBuf* b : {&counts[0], &counts[1], &stats[0], &stats[1], &ovf, &perm, &loc, &sorted,
-                      &grid64, &vir6, &part, &bricks, &bshift, &binscan, &border})
+                      &grid64, &vir6, &part, &bricks, &bshift, &binscan, &border[0],
+                      &border[1]})
       b->release();
+    order_nb = 0;
+    order_ok = false;
   }
 
   // Bins the particles for a support of radius rcut and Gaussian sigma.
@@ -210,11 +217,19 @@ struct Engine {
                      spread_weight_bound(g, sigma, normalize);
     HIPCHK(launch_bin(ct_dbl, g, shape, bin, s));
     // the spread's and the gather's workgroups take bins by descending count
+    // of the last binning of this shape; the gather's spare workgroups sort
+    // this binning's counts for the next
     bin.order = nullptr;
+    bin.order_next = nullptr;
     if (!shape.direct && n > 0 && g_tuning.bin_order) {
-      GSECHK(border.ensure(sizeof(int) * (size_t)nb));
-      HIPCHK(launch_bin_order(shape, bin, border.as<int>(), s));
-      bin.order = border.as<int>();
+      if (order_nb != nb) {
+        GSECHK(border[0].ensure(sizeof(int) * (size_t)nb));
+        GSECHK(border[1].ensure(sizeof(int) * (size_t)nb));
+        order_nb = nb;
+        order_ok = false;
+      }
+      bin.order = order_ok ? border[ocur].as<int>() : nullptr;
+      bin.order_next = border[ocur ^ 1].as<int>();
     }
     return GSE_SUCCESS;
   }
@@ -279,6 +294,12 @@ struct Engine {
     }
     HIPCHK(launch_gather(ct_dbl, at_dbl, geom, shape, sigma, rcut, flags, bin, phi, o, s));
     consumed();
+    if (bin.order_next) {  // the schedule of these counts is ready
+      ocur ^= 1;
+      order_ok = true;
+      bin.order = border[ocur].as<int>();
+      bin.order_next = border[ocur ^ 1].as<int>();
+    }
     if ((flags & GF_VIRIAL) && virial9 && n > 0) {
       HIPCHK(launch_sum_columns(vir6.as<double>(), n, 6, part.as<double>(), s));
       HIPCHK(launch_add_sym6(part.as<double>(), kRedBlocks, virial9, s));

@@ -41,6 +41,8 @@ struct GatherArgs {
   int cap_bin;   // slots per bin
   int nwg_bins;
   const int* order;  // bin of each workgroup position (NULL: the position itself)
+  int* order_next;   // the schedule of this binning's counts (NULL: not computed)
+  int order_wg0;     // first of the kOrderWG workgroups that compute it
 };
 
 __device__ __forceinline__ float fmac(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -728,6 +730,11 @@ __global__ void __launch_bounds__(
              const CT* __restrict__ phi, GatherOut o, const int* __restrict__ loc) {
   extern __shared__ __align__(32) double smem_raw[];
   const double ds = o.dscale ? *o.dscale : 1.0;
+  if (a.order_next && (int)blockIdx.x >= a.order_wg0) {  // spare workgroups: the schedule
+    bin_order_eighth<BS>(counts, a.b.nbins, (int)blockIdx.x - a.order_wg0, a.order_next,
+                         (unsigned*)smem_raw);
+    return;
+  }
   if ((int)blockIdx.x == a.nwg_bins + kOverflowWG) {  // spare workgroup: k-pass finalize
     kpass_finalize_block(o.fin_partials, o.fin_nblocks, o.fin_ncomp, o.fin_pref, o.fin_energy,
                          o.fin_virial);
@@ -814,6 +821,8 @@ __device__ __forceinline__ GatherArgs batch_gather_args(const BatchSys& S) {
   a.cap_bin = S.cap_bin;
   a.nwg_bins = 0;
   a.order = nullptr;
+  a.order_next = nullptr;
+  a.order_wg0 = 0;
   return a;
 }
 
@@ -973,7 +982,9 @@ static hipError_t gather_nz(GatherArgs a, const Binning& w, const void* phi, con
   if (lds > 150 * 1024) return hipErrorInvalidConfiguration;
   a.nwg_bins = ((a.b.nbins + 7) / 8) * 8;
   a.order = w.order;
-  const unsigned nwg = (unsigned)(a.nwg_bins + kOverflowWG + (o.fin_partials ? 1 : 0));
+  a.order_next = w.order_next;
+  a.order_wg0 = a.nwg_bins + kOverflowWG + (o.fin_partials ? 1 : 0);
+  const unsigned nwg = (unsigned)(a.order_wg0 + (a.order_next ? kOrderWG : 0));
   const int bs = tu.gather_block ? tu.gather_block : (sizeof(CT) == 4 ? 512 : 256);
 #define GSE_G_ARGS a, xq, srt, w.perm, w.counts, w.counts_other, w.stats, w.stats_other, w.ovf, ph, o, \
                    w.loc
@@ -1040,6 +1051,8 @@ hipError_t launch_gather(bool ct_double, bool at_double, const Geom& g, const Bi
   a.cap_bin = w.cap_bin;
   a.nwg_bins = 0;
   a.order = nullptr;
+  a.order_next = nullptr;
+  a.order_wg0 = 0;
   if (ct_double) return gather_fl<double, double>(flags, a, w, phi, o, s);
   if (at_double) return gather_fl<float, double>(flags, a, w, phi, o, s);
   return gather_fl<float, float>(flags, a, w, phi, o, s);

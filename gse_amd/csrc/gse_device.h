@@ -35,6 +35,56 @@ __device__ __forceinline__ float readlane_f(float v, int src) {
 __device__ __forceinline__ double readlane(double v, int s) { return readlane_d(v, s); }
 __device__ __forceinline__ float readlane(float v, int s) { return readlane_f(v, s); }
 
+// Workgroup schedule of the per-bin spread and gather.  xcd_remap gives XCD
+// x the positions x*chunk .. x*chunk + chunk - 1, which its workgroups take
+// in order; order[] holds, at those positions, the same bins sorted by
+// descending charge count, so each XCD starts its longest bins first and
+// the short ones fill the end of the launch.  Bins stay with their XCD (and
+// its L2), and each bin's work, so every result, is the same in any order.
+// One workgroup of BT >= 256 threads sorts eighth x (a counting sort on 256
+// keys; counts of 255 and more share the first key, equal keys go in any
+// order); sm: 2 * 256 + 4 unsigned of LDS.
+constexpr int kOrderKeys = 256;
+constexpr int kOrderWG = 8;  // one workgroup per XCD's eighth
+__device__ __forceinline__ int order_key(int c) {
+  return kOrderKeys - 1 - min(max(c, 0), kOrderKeys - 1);  // ascending = descending count
+}
+template <int BT>
+__device__ void bin_order_eighth(const int* __restrict__ counts, int nbins, int x,
+                                 int* __restrict__ order, unsigned* sm) {
+  static_assert(BT >= kOrderKeys, "one thread per key in the scan");
+  unsigned* hist = sm;
+  unsigned* start = sm + kOrderKeys;
+  unsigned* wtot = start + kOrderKeys;
+  const int t = threadIdx.x;
+  const int chunk = (((nbins + 7) >> 3) << 3) >> 3;  // xcd_remap's, for ((nbins + 7) / 8) * 8
+  const int p0 = x * chunk, p1 = min(p0 + chunk, nbins);
+  if (t < kOrderKeys) hist[t] = 0u;
+  __syncthreads();
+  for (int p = p0 + t; p < p1; p += BT) atomicAdd(&hist[order_key(counts[p])], 1u);
+  __syncthreads();
+  if (t < kOrderKeys) {  // exclusive scan of the histogram
+    const unsigned h = hist[t];
+    unsigned v = h;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const unsigned sh = __shfl_up(v, d, kWave);
+      if ((t & 63) >= d) v += sh;
+    }
+    if ((t & 63) == 63) wtot[t >> 6] = v;
+    start[t] = v - h;
+  }
+  __syncthreads();
+  if (t < kOrderKeys) {
+    unsigned add = 0u;
+    for (int w = 0; w < (t >> 6); w++) add += wtot[w];
+    start[t] += add;
+  }
+  __syncthreads();
+  for (int p = p0 + t; p < p1; p += BT)
+    order[p0 + (int)atomicAdd(&start[order_key(counts[p])], 1u)] = p;
+}
+
 // Lane shuffle with a per-lane source (ds_bpermute)
 __device__ __forceinline__ double shfl_d(double v, int src) {
   int lo = __builtin_amdgcn_ds_bpermute(src << 2, __double2loint(v));

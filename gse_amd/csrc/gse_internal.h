@@ -105,8 +105,12 @@ struct Binning {
   // counters, then the charges and their indices grouped by super-bin
   void* scan = nullptr;
   // [nbins] the bin each spread/gather workgroup position takes (NULL: bin =
-  // position); filled by launch_bin_order
+  // position): the schedule from an earlier binning of the same shape (any
+  // permutation of the bins gives the same results)
   int* order = nullptr;
+  // [nbins] where the gather's spare workgroups write the schedule of this
+  // binning's counts (NULL: none)
+  int* order_next = nullptr;
 };
 
 // Fixed-point exponent e (scale 2^e) of one spread: every contribution
@@ -163,9 +167,6 @@ hipError_t launch_bin_clear(int* counts, int nbins, BinStats* stats, hipStream_t
 size_t bin_sort_bytes(int n, const BinShape& b, bool dbl);
 hipError_t launch_bin(bool ct_double, const Geom& g, const BinShape& b, Binning& w,
                       hipStream_t s);
-// order[] of the per-bin spread and gather workgroups from this binning's
-// counts: within each XCD's eighth of the positions, bins by descending count
-hipError_t launch_bin_order(const BinShape& b, const Binning& w, int* order, hipStream_t s);
 
 hipError_t launch_spread(bool ct_double, const Geom& g, const BinShape& b,
                          double sigma, bool normalize, Binning& w,

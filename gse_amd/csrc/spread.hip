@@ -51,7 +51,7 @@ __device__ __forceinline__ int xcd_remap(int wg, int nwg) {
 }
 
 // The bin taken by spread/gather workgroup position p (order: NULL or the
-// table of k_bin_order)
+// schedule of bin_order_eighth)
 __device__ __forceinline__ int bin_at(const int* order, int p) { return order ? order[p] : p; }
 
 __global__ void k_bin_clear(int* counts, int nbins, BinStats* stats) {
@@ -356,32 +356,6 @@ __device__ unsigned block_excl_scan(const unsigned* in, unsigned* out, int n, un
   const unsigned total = wsum[BT / 64];
   __syncthreads();
   return total;
-}
-
-// Workgroup schedule of the per-bin spread and gather: xcd_remap gives XCD x
-// the positions x*chunk .. x*chunk + chunk - 1, which its workgroups take in
-// order.  order[] holds, at those positions, the same bins sorted by
-// descending charge count (a counting sort per eighth, one workgroup each),
-// so every XCD starts its longest bins first and the short ones fill the end
-// of the launch.  Bins stay with their XCD and its L2; each bin's work, and
-// so every result, is unchanged.  Equal counts go in any order.
-constexpr int kOrderKeys = 4096;
-__global__ void __launch_bounds__(1024)
-    k_bin_order(const int* __restrict__ counts, int nbins, int chunk, int* __restrict__ order) {
-  __shared__ unsigned hist[kOrderKeys], start[kOrderKeys];
-  __shared__ unsigned wsum[1024 / 64 + 1];
-  const int p0 = blockIdx.x * chunk, p1 = min(p0 + chunk, nbins);
-  for (int k = threadIdx.x; k < kOrderKeys; k += 1024) hist[k] = 0u;
-  __syncthreads();
-  // key kOrderKeys - 1 - count: ascending keys are descending counts
-  for (int p = p0 + (int)threadIdx.x; p < p1; p += 1024)
-    atomicAdd(&hist[kOrderKeys - 1 - min(max(counts[p], 0), kOrderKeys - 1)], 1u);
-  __syncthreads();
-  block_excl_scan<1024>(hist, start, kOrderKeys, wsum);  // in and out must differ
-  for (int p = p0 + (int)threadIdx.x; p < p1; p += 1024) {
-    const int k = kOrderKeys - 1 - min(max(counts[p], 0), kOrderKeys - 1);
-    order[p0 + (int)atomicAdd(&start[k], 1u)] = p;
-  }
 }
 
 // LDS of k_bin_coarse: [nsuper] counts (then the claimed bases), [nsuper]
@@ -2227,13 +2201,6 @@ BinShape choose_bins(const Geom& g, size_t elem, size_t budget, int pref_edge) {
 
 hipError_t launch_bin_clear(int* counts, int nbins, BinStats* stats, hipStream_t s) {
   k_bin_clear<<<grid_for(nbins, 256), 256, 0, s>>>(counts, nbins, stats);
-  return hipGetLastError();
-}
-
-hipError_t launch_bin_order(const BinShape& b, const Binning& w, int* order, hipStream_t s) {
-  if (b.direct || b.nbins <= 0 || !order) return hipErrorInvalidValue;
-  const int chunk = ((b.nbins + 7) / 8 * 8) / 8;  // xcd_remap's, for nwg_bins
-  k_bin_order<<<8, 1024, 0, s>>>(w.counts, b.nbins, chunk, order);
   return hipGetLastError();
 }
 

@@ -101,8 +101,10 @@ GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
  *                 the binning of grids above 16384 bins (256, 512, 1024; 1024)
  *   bin_order     the spread's and the gather's workgroups take the bins of
  *                 each XCD's share by descending charge count, so the longest
- *                 start first (a small sort kernel after the binning); 0 =
- *                 bin order (0/1, 1)
+ *                 start first; the counts are those of the last binning of
+ *                 the same shape, sorted by eight spare workgroups of its
+ *                 gather (results do not depend on the order); 0 = bin
+ *                 order (0/1, 1)
  *   ypass4        for Y = 216 the x-y transforms around the fused z pass run
  *                 as 1D x rocFFT transforms and a four-step y pass in
  *                 registers (rocFFT's 2D plan takes two kernels there), for
