@@ -378,6 +378,9 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
   const int gnx = __builtin_amdgcn_readfirstlane(g.nx);
   const int gny = __builtin_amdgcn_readfirstlane(g.ny);
   const int gnz = __builtin_amdgcn_readfirstlane(g.nz);
+  // the spacings as wave-uniform values (the paired form selects per lane)
+  const double hx_u = g.hx, hy_u = g.hy, hz_u = g.hz;
+  const double ihx_u = g.ihx, ihy_u = g.ihy, ihz_u = g.ihz;
   if (lane < NX) {
     axis = 0; jj = lane; hsel = g.hx; ihsel = g.ihx; nsel = gnx;
   } else if (lane < NX + NY) {
@@ -457,14 +460,32 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
     // phase 1: one wave per particle; lane j holds the reference gather's
     // offset d = x - t*h (CpuGaussCharge.cpp:146-155, t = round(x/h) - n + j)
     // and factor exp(-d^2/2s^2)
-    for (int ai = wid; ai < na; ai += nwv) {
-      const V4<CT> p = sX[ai];
-      const double ps = axis == 0 ? (double)p.x : axis == 1 ? (double)p.y : (double)p.z;
-      if (axis >= 0) {
-        const int c = (int)round(ps * ihsel);
-        const CT d = (CT)sub_rn(ps, mul_rn((double)(c - nsel + jj), hsel));
-        sW[2 * ((size_t)ai * NW + lane)] = gexp(mul_rn(mul_rn(-d, d), inv2s2));
-        sW[2 * ((size_t)ai * NW + lane) + 1] = d;
+    if constexpr (PAIR) {
+      // cubic support: the chunk's na x 3 NZA factors one per lane over the
+      // whole workgroup, entry e = ai NW + j as above
+      constexpr int NW3 = 3 * NZA;
+      for (int e = threadIdx.x; e < na * NW3; e += BS) {
+        const int ai = e / NW3, j = e - ai * NW3;
+        const int ax = j / NZA, jx = j - ax * NZA;
+        const V4<CT> p = sX[ai];
+        const double ps = ax == 0 ? (double)p.x : ax == 1 ? (double)p.y : (double)p.z;
+        const double hs = ax == 0 ? hx_u : ax == 1 ? hy_u : hz_u;
+        const double ihs = ax == 0 ? ihx_u : ax == 1 ? ihy_u : ihz_u;
+        const int c = (int)round(ps * ihs);
+        const CT d = (CT)sub_rn(ps, mul_rn((double)(c - gnz + jx), hs));
+        sW[2 * e] = gexp(mul_rn(mul_rn(-d, d), inv2s2));
+        sW[2 * e + 1] = d;
+      }
+    } else {
+      for (int ai = wid; ai < na; ai += nwv) {
+        const V4<CT> p = sX[ai];
+        const double ps = axis == 0 ? (double)p.x : axis == 1 ? (double)p.y : (double)p.z;
+        if (axis >= 0) {
+          const int c = (int)round(ps * ihsel);
+          const CT d = (CT)sub_rn(ps, mul_rn((double)(c - nsel + jj), hsel));
+          sW[2 * ((size_t)ai * NW + lane)] = gexp(mul_rn(mul_rn(-d, d), inv2s2));
+          sW[2 * ((size_t)ai * NW + lane) + 1] = d;
+        }
       }
     }
     __syncthreads();
