@@ -581,7 +581,7 @@ struct KSolver {
     GSECHK(table.ensure(tsize(d) * (size_t)zs * z));
     if (fused) {
       plane = g_tuning.xy_plane && xy_plane_ok(x, y, d);
-      ypass = !plane && g_tuning.ypass4 && ypass4_shape(y);
+      ypass = !plane && g_tuning.ypass4 && ypass4_shape(y, d);
       if (plane) {
         GSECHK(upload_twiddles(twx, x));
         GSECHK(upload_twiddles(twy, y));

@@ -518,14 +518,16 @@ __global__ void __launch_bounds__((zpass4_threads<Y1, Y2, C>()))
   }
 }
 
-bool ypass4_shape(int Y) { return Y == 216; }
+// Y = 216 = 12 x 18 (32 columns per workgroup); Y = 80 = 8 x 10 (16) in
+// FP64 only: there it measured 1.4 us per step below rocFFT's single 2D
+// kernels (stage times r2c 13.0 against 14.2, c2r 13.9 against 14.1 us),
+// in FP32 no gain
+bool ypass4_shape(int Y, bool dbl) { return Y == 216 || (dbl && Y == 80); }
 
-hipError_t launch_ypass4(bool dbl, bool inv, int X, int Y, int Z, const void* src, int szs,
-                         void* dst, int dzs, const void* tw, hipStream_t s) {
-  if (Y != 216) return hipErrorInvalidValue;
-  constexpr int Y1 = 12, Y2 = 18, C = 32;
-  const int Xh = X / 2 + 1;
-  if (szs < Xh * Y || dzs < Xh * Y) return hipErrorInvalidValue;
+template <int Y1, int Y2, int C>
+static hipError_t ypass4_t(bool dbl, bool inv, int Xh, int Z, const void* src, int szs,
+                           void* dst, int dzs, const void* tw, hipStream_t s) {
+  constexpr int Y = Y1 * Y2;
   constexpr int NT = zpass4_threads<Y1, Y2, C>();
   const unsigned g = (unsigned)(((size_t)Xh * Z + C - 1) / C);
   const size_t lds = ((size_t)Y * C + Y) * (dbl ? 16 : 8);
@@ -541,6 +543,15 @@ hipError_t launch_ypass4(bool dbl, bool inv, int X, int Y, int Z, const void* sr
   }
 #undef GSE_YPASS
   return hipGetLastError();
+}
+
+hipError_t launch_ypass4(bool dbl, bool inv, int X, int Y, int Z, const void* src, int szs,
+                         void* dst, int dzs, const void* tw, hipStream_t s) {
+  if (!ypass4_shape(Y, dbl)) return hipErrorInvalidValue;
+  const int Xh = X / 2 + 1;
+  if (szs < Xh * Y || dzs < Xh * Y) return hipErrorInvalidValue;
+  if (Y == 80) return ypass4_t<8, 10, 16>(dbl, inv, Xh, Z, src, szs, dst, dzs, tw, s);
+  return ypass4_t<12, 18, 32>(dbl, inv, Xh, Z, src, szs, dst, dzs, tw, s);
 }
 
 // the four-step forms built: Z -> (Z1, Z2), 16 or 32 columns per workgroup

@@ -2387,6 +2387,14 @@ static hipError_t spread_t(const Geom& g, const BinShape& b, double sigma, bool 
             (ACCT*)w.bricks, w.bshift, w.loc);        \
         break;                                        \
       }                                               \
+      if constexpr (sizeof(CT) == 8) { /* FP64: 256 threads */                               \
+        if (bs == 256 && b.Lx * b.Ly == 400) {                                               \
+          k_spread<CT, 13, 256, ACCT, 400><<<nwg, 256, lds, s>>>(                           \
+              a, xq, srt, w.counts, w.counts_other, w.stats, w.stats_other, w.ovf, grid64,   \
+              (ACCT*)w.bricks, w.bshift, w.loc);      \
+          break;                                      \
+        }                                             \
+      }                                               \
       GSE_SPREAD_BS(13, ACCT) break;                  \
     case 15: GSE_SPREAD_BS(15, ACCT) break;           \
     case 17: GSE_SPREAD_BS(17, ACCT) break;           \
