@@ -31,16 +31,16 @@ HBM_PEAK_GBS = 8000.0
 # (FETCH x2.000, WRITE x1.000); the commands and per-kernel table are in
 # profiles/k_gather_k_spread_water48k_M80_fp32_hbm_traffic.md.
 MEASURED_TRAFFIC = {
-    ("water_48k_M80", "fp32", "gather"): 10597990,
-    ("water_48k_M80", "fp32", "spread"): 43813581,
+    ("water_48k_M80", "fp32", "gather"): 10649395,
+    ("water_48k_M80", "fp32", "spread"): 44080026,
 }
 # Wave-level VALU instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU, same
 # file).  The gather and the spread issue far more vector instructions than
 # their bytes need; their bound is VALU issue: 256 CUs x 2.4 GHz x one wave64
 # VALU instruction per CU per clock (4 SIMDs, 16 lanes each) = 614.4 G/s.
 MEASURED_VALU = {
-    ("water_48k_M80", "fp32", "gather"): 15711103,
-    ("water_48k_M80", "fp32", "spread"): 8345423,
+    ("water_48k_M80", "fp32", "gather"): 15666063,
+    ("water_48k_M80", "fp32", "spread"): 8510291,
 }
 VALU_PEAK_GINST = 614.4
 

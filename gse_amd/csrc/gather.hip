@@ -460,9 +460,10 @@ __device__ __forceinline__ void gather_bin(const GatherArgs& a, int bin, int cnt
     // phase 1: one wave per particle; lane j holds the reference gather's
     // offset d = x - t*h (CpuGaussCharge.cpp:146-155, t = round(x/h) - n + j)
     // and factor exp(-d^2/2s^2)
-    if constexpr (PAIR) {
-      // cubic support: the chunk's na x 3 NZA factors one per lane over the
-      // whole workgroup, entry e = ai NW + j as above
+    if constexpr (PAIR && sizeof(CT) == 8) {
+      // FP64, cubic support: the chunk's na x 3 NZA factors one per lane
+      // over the whole workgroup, entry e = ai NW + j as above (in FP32 the
+      // per-lane axis selects cost more VALU than the idle lanes saved)
       constexpr int NW3 = 3 * NZA;
       for (int e = threadIdx.x; e < na * NW3; e += BS) {
         const int ai = e / NW3, j = e - ai * NW3;
