@@ -518,11 +518,12 @@ __global__ void __launch_bounds__((zpass4_threads<Y1, Y2, C>()))
   }
 }
 
-// Y = 216 = 12 x 18 (32 columns per workgroup); Y = 80 = 8 x 10 (16) in
-// FP64 only: there it measured 1.4 us per step below rocFFT's single 2D
-// kernels (stage times r2c 13.0 against 14.2, c2r 13.9 against 14.1 us),
-// in FP32 no gain
-bool ypass4_shape(int Y, bool dbl) { return Y == 216 || (dbl && Y == 80); }
+// Y = 216 = 12 x 18 (32 columns per workgroup); Y = 80 = 8 x 10 (32
+// columns in FP32, 16 in FP64), where 1D x rocFFT transforms and this pass
+// measured below rocFFT's single 2D kernels: FP64 stage times r2c 13.0
+// against 14.2, c2r 13.9 against 14.1 us; FP32 11.8 against 12.2 and 11.9
+// against 12.3 us (8 or 16 columns: no gain in FP32)
+bool ypass4_shape(int Y, bool) { return Y == 216 || Y == 80; }
 
 template <int Y1, int Y2, int C>
 static hipError_t ypass4_t(bool dbl, bool inv, int Xh, int Z, const void* src, int szs,
@@ -550,7 +551,10 @@ hipError_t launch_ypass4(bool dbl, bool inv, int X, int Y, int Z, const void* sr
   if (!ypass4_shape(Y, dbl)) return hipErrorInvalidValue;
   const int Xh = X / 2 + 1;
   if (szs < Xh * Y || dzs < Xh * Y) return hipErrorInvalidValue;
-  if (Y == 80) return ypass4_t<8, 10, 16>(dbl, inv, Xh, Z, src, szs, dst, dzs, tw, s);
+  if (Y == 80) {
+    if (dbl) return ypass4_t<8, 10, 16>(dbl, inv, Xh, Z, src, szs, dst, dzs, tw, s);
+    return ypass4_t<8, 10, 32>(dbl, inv, Xh, Z, src, szs, dst, dzs, tw, s);
+  }
   return ypass4_t<12, 18, 32>(dbl, inv, Xh, Z, src, szs, dst, dzs, tw, s);
 }
 

@@ -381,7 +381,7 @@ hipError_t launch_zpass(bool dbl, int kind, int X, int Y, int Z, double bx, doub
 // y transform of the half spectrum in registers (four-step, Y = 216 = 12 x 18):
 // forward from packed planes (src, szs = (X/2+1) Y) into planes dzs apart,
 // inverse the other way; tw: device [Y] complex exp(-2 pi i m/Y); also
-// Y = 80 = 8 x 10 in FP64
+// Y = 80 = 8 x 10
 bool ypass4_shape(int Y, bool dbl);
 hipError_t launch_ypass4(bool dbl, bool inv, int X, int Y, int Z, const void* src, int szs,
                          void* dst, int dzs, const void* tw, hipStream_t s);

@@ -105,7 +105,7 @@ GSE_API gse_status_t gse_set_tiling(int bin_edge, int lds_budget_bytes);
  *                 the same shape, sorted by eight spare workgroups of its
  *                 gather (results do not depend on the order); 0 = bin
  *                 order (0/1, 1)
- *   ypass4        for Y = 216 (and Y = 80 in FP64) the x-y transforms
+ *   ypass4        for Y = 216 and Y = 80 the x-y transforms
  *                 around the fused z pass run as 1D x rocFFT transforms and
  *                 a four-step y pass in registers (rocFFT's 2D plan takes
  *                 two kernels at 216), for handles created afterwards (0/1,

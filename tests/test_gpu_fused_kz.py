@@ -226,7 +226,7 @@ def test_four_step_gsek_fp32_water_48k_M80():
     for four in (1, 0):
         with fused(True, four, 0):
             gs = api.CpuGSEk(F32, F32, M, M, M, sigma, 0.4, 3.0, 3.0, L, L, L)
-        assert gs.layout()["kspace"] == ("fused4" if four else "fused")
+        assert gs.layout()["kspace"] == ("fused4" if four else "fused") + "/y4"
         o = gs.step(to_dev(xyzq, F32), force=True, potential=True, virial=True)
         out[four] = (float(host(o["energy"])[0]), host(o["force"]), host(o["virial"]))
         erms = orc.erms(ref.force, out[four][1].T)
@@ -269,7 +269,7 @@ def test_plane_kernels_vs_oracle_and_rocfft2d(dims, box, dt):
 
 
 # Four-step y pass (Y = 216 = 12 x 18 in registers, 32 columns per
-# workgroup; Y = 80 = 8 x 10, 16 columns, FP64 only) between 1D x rocFFT
+# workgroup; Y = 80 = 8 x 10, 32 columns in FP32, 16 in FP64) between 1D x rocFFT
 # transforms, around both z passes; odd X and column counts Xh Z that are
 # not a multiple of the columns per workgroup included.
 YPASS_DIMS = [
@@ -295,7 +295,7 @@ def test_ypass_solver_vs_oracle_and_rocfft2d(kind, sigma, dims, box, dt):
         ref = orc.fft_solve(kind, dims, sigma, box, rho)
     tol = TOL if dt == F64 else 2e-6
     four = Z in (80, 216)
-    built = Y == 216 or (Y == 80 and dt == F64)
+    built = Y in (80, 216)
     old = api.get_tuning()["ypass4"]
     out = {}
     for yp in (1, 0):
