@@ -31,8 +31,8 @@ HBM_PEAK_GBS = 8000.0
 # (FETCH x2.000, WRITE x1.000); the commands and per-kernel table are in
 # profiles/k_gather_k_spread_water48k_M80_fp32_hbm_traffic.md.
 MEASURED_TRAFFIC = {
-    ("water_48k_M80", "fp32", "gather"): 10649395,
-    ("water_48k_M80", "fp32", "spread"): 44080026,
+    ("water_48k_M80", "fp32", "gather"): 10650317,
+    ("water_48k_M80", "fp32", "spread"): 43789312,
 }
 # Wave-level VALU instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU, same
 # file).  The gather and the spread issue far more vector instructions than
